@@ -1,0 +1,170 @@
+/*
+ * msf.h -- C ABI of the MI355X multislice ADF-STEM / FEM path.
+ *
+ * Drop-in for the MPI+FFTW hot path of ms-stem-fem (reference files cited
+ * relative to its repository root).  The reference has no FFI; its seam is
+ * the C++ call TEM_NS::adfstem() (adfstem.hpp:66-121) made from tem.cpp
+ * (:1526-1558 uncorrected, :1607-1639 Cs5-corrected), preceded by the
+ * specimen set-up of tem.cpp:944-1276 (scatterer_pap_LUT, auto-slicing,
+ * slice::update_propagator, slice::update_transmission_function).  The
+ * entry points below replace those calls one for one; results that the
+ * reference writes to files (STEM image TIFF, *_fem_V_omega.txt) are
+ * returned in caller-owned host buffers instead.
+ *
+ * Conventions: every function returns MSF_OK (0) or a negative MSF_ERR_*
+ * code; msf_last_error() gives the message of the last failure on a context.
+ * All pointers are host pointers; the library owns every device buffer.
+ * Complex arrays are interleaved (re, im) in the reference's row-major
+ * [i*ny + j] layout, i = x row.  One context per host thread.
+ */
+#ifndef MSF_H
+#define MSF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSF_OK 0
+#define MSF_ERR_INVALID (-1)     /* bad argument (flags.fail paths of check_runtime_flags) */
+#define MSF_ERR_UNSUPPORTED (-2) /* grid size without a compiled transform length */
+#define MSF_ERR_DEVICE (-3)      /* HIP runtime error */
+#define MSF_ERR_MEMORY (-4)      /* device allocation failed */
+#define MSF_ERR_DOMAIN (-5)      /* position outside the domain the shift rule covers */
+
+typedef struct msf_ctx msf_ctx;
+typedef struct msf_specimen msf_specimen;
+
+/* Simulation cell and sampling: tem.cpp:424-446 (lambda, gamma from the
+ * voltage), :683-695 (band limits), :756-883 (domains).  xmin/ymin/zmin are
+ * the domain minima; the xyz reader sets them to 0 (tem_io.cpp:4856-4858). */
+typedef struct msf_grid {
+    uint32_t nx, ny;                   /* samples_x, samples_y */
+    double xperiod, yperiod, zperiod;  /* cell periods [A] */
+    double xmin, ymin, zmin;           /* domain minima [A] */
+    double voltage;                    /* microscope_voltage [V] */
+} msf_grid;
+
+/* Probe optics: probe_wavefunction.cpp:37-239, lenses.cpp:249-290. */
+typedef struct msf_optics {
+    int32_t corrected;                 /* 0: adfstemuncorr (Cs3); 1: adfstemcorr (Cs3 + Cs5) */
+    double cs3, cs5;                   /* [A] */
+    double defocus;                    /* [A] */
+    double alpha_max;                  /* objective aperture [rad] */
+} msf_optics;
+
+/* Annular detector: sum_detected_intensity (func.cpp:2593-2641) called with
+ * k1 = inner/lambda, k2 = outer/lambda (adfstem.cpp:2247-2253). */
+#define MSF_ADF_REFERENCE 0 /* accept kx^2+ky^2 in (k1, k2], as the reference does */
+#define MSF_ADF_PHYSICAL 1  /* accept kx^2+ky^2 in (k1^2, k2^2] */
+typedef struct msf_detector {
+    int32_t enabled;                   /* --images: compute the ADF image */
+    double inner, outer;               /* detector angles [rad] */
+    int32_t threshold_mode;            /* MSF_ADF_* */
+} msf_detector;
+
+/* FEM V_omega (d1): adfstem.cpp:947-1066, variance.cpp:64-300,
+ * adfstem.cpp:2495-2601 and :2822-2866. */
+typedef struct msf_fem {
+    int32_t enabled;
+    double dr;                         /* --dr azimuthal binning size factor (default 1/sqrt 2) */
+} msf_fem;
+
+/* Transmission multiply of the slice loop (adfstem.cpp:1922-1952).  The
+ * reference overwrites Re(psi) before computing Im(psi), so Im uses the
+ * updated real part; MSF_TRANSMIT_REFERENCE reproduces that, _EXACT uses
+ * the exact complex product. */
+#define MSF_TRANSMIT_REFERENCE 0
+#define MSF_TRANSMIT_EXACT 1
+
+/* Execution settings (no reference counterpart: the reference runs one
+ * position at a time on every MPI rank). */
+typedef struct msf_config {
+    uint32_t batch;                    /* probe positions per launch; 0 = automatic */
+    int32_t transmit_mode;             /* MSF_TRANSMIT_* */
+    int32_t profile_kernels;           /* 1: time every hot launch with HIP events */
+} msf_config;
+
+/* Timings of the last msf_run_stem() on the context. */
+typedef struct msf_stats {
+    double setup_ms;                   /* probe, shifts, detector tables */
+    double slice_loop_ms;              /* all batches: slice passes + reductions */
+    double total_ms;
+    uint64_t probe_slices;             /* positions x slices propagated */
+    uint32_t batch;                    /* batch depth used */
+    uint32_t n_batches;
+    /* per-kernel averages (profile_kernels = 1 only) */
+    double row_pass_us, col_pass_us, first_row_pass_us, last_col_pass_us, reduce_us;
+    uint64_t row_pass_count, col_pass_count;
+    double row_pass_bytes, col_pass_bytes; /* compulsory HBM bytes of one launch */
+} msf_stats;
+
+/* ------------------------------------------------------------ context */
+int msf_create(msf_ctx** ctx, int device);
+int msf_destroy(msf_ctx* ctx);
+const char* msf_last_error(const msf_ctx* ctx);
+int msf_set_config(msf_ctx* ctx, const msf_config* cfg);
+int msf_get_stats(const msf_ctx* ctx, msf_stats* out);
+/* 1 if transforms of length n are compiled in, else 0 */
+int msf_supported_length(uint32_t n);
+
+/* ------------------------------------------------------------ scalars */
+/* tem.cpp:424-438 */
+int msf_wavelength(double voltage, double* lambda, double* gamma);
+/* scherzer.cpp:61-79 (uncorrected) and :131-168 (corrected to Cs5) */
+double msf_scherzer_defocus(int32_t corrected, double lambda, double cs);
+double msf_scherzer_alphamax(int32_t corrected, double lambda, double cs);
+
+/* ------------------------------------------------------------ specimen */
+/* Replaces tem.cpp:944-1276: species templates (scatterer_pap_LUT,
+ * scatterer_pap.hpp:57-231), assign_atoms_to_slices_z_auto
+ * (slice.cpp:469-740), slice::update_propagator (slice.cpp:1293-1363) and
+ * slice::update_transmission_function (slice.cpp:1451-1757) for every
+ * slice.  q_xyz holds 3*n_atoms positions [A], Z the atomic numbers.  The
+ * transmission functions stay resident on the device until msf_specimen_free. */
+int msf_specimen_build(msf_ctx* ctx, const msf_grid* grid, const double* q_xyz,
+                       const uint32_t* Z, uint32_t n_atoms, double min_slice,
+                       msf_specimen** out);
+int msf_specimen_free(msf_specimen* spec);
+uint32_t msf_specimen_slices(const msf_specimen* spec);
+/* lower_bound, thickness: n_slices doubles each; atoms: n_slices counts (any may be NULL) */
+int msf_specimen_slice_info(const msf_specimen* spec, double* lower_bound, double* thickness,
+                            uint32_t* atoms);
+/* transmission function of one slice, FP64 as built (2*nx*ny doubles) */
+int msf_specimen_transmission(const msf_specimen* spec, uint32_t slice, double* t);
+
+/* ------------------------------------------------------------ probe */
+/* Normalised real-space probe of adfstem.cpp:586-817 at (xx[0], yy[0]):
+ * 2*nx*ny doubles. */
+int msf_probe_realspace(msf_ctx* ctx, const msf_grid* grid, const msf_optics* optics,
+                        double* probe);
+
+/* ------------------------------------------------------------ run */
+/* Number of V_omega bins (adfstem.cpp:947-972) and their lower edges
+ * (n_bins doubles, may be NULL). */
+int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lower_edges);
+
+/* Replaces TEM_NS::adfstem() (adfstem.hpp:66-121, body adfstem.cpp:37-4096)
+ * for the default modes: raster loop over x_p (outer) and y_p (inner),
+ * probe shift, slice loop, ADF integration and FEM V_omega.
+ * adf_image: n_xp*n_yp doubles, x-major (stem_image[py + px*n_yp],
+ * adfstem.cpp:2264), written when det->enabled.
+ * fem_v: n_bins doubles (msf_fem_bins), written when fem->enabled. */
+int msf_run_stem(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
+                 const msf_detector* det, const msf_fem* fem,
+                 const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
+                 double* adf_image, double* fem_v);
+
+/* Exit waves psi(k) after the last slice for the same raster, FP32 complex,
+ * n_xp*n_yp*nx*ny*2 floats in the reference layout (zeros outside the band
+ * limit).  Parity entry point: the reference keeps these only in memory. */
+int msf_exit_waves(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
+                   const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
+                   float* psi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+// fft_engine.hpp -- register/LDS Stockham FFT for CDNA4 (gfx950).
+//
+// One N-point transform is computed by T = N/E cooperating lanes, each holding
+// E complex values.  On entry and exit lane t holds elements t + T*p,
+// p = 0..E-1 (the "distributed natural order"), so global loads and stores of
+// a row are coalesced and a transform can be followed directly by an
+// element-wise operation and by another transform without any LDS traffic.
+// Between radix stages the values go through an LDS buffer of padded length
+// lds_len(N).  Stage s with radix R and span NS (product of the previous
+// radices) is the autosort Stockham step
+//     y[(j/NS)*NS*R + j%NS + q*NS] = sum_r x[j + r*N/R] W_{NS R}^{r (j%NS)} W_R^{rq}
+// for butterfly j = t + T*m, m = 0..E/R-1.
+//
+// Twiddles come from a table of W_N^e = exp(-2 pi i e / N), e in [0, N),
+// built on the host in FP64 and rounded once to the working precision.
+// FFT sign conventions match FFTW: forward = exp(-2 pi i), inverse =
+// exp(+2 pi i), both unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace msf {
+
+template <typename R> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b)
+{
+    return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(a.x) s) { return C{a.x * s, a.y * s}; }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C> __device__ __forceinline__ C mul_mi(C a)
+{
+    return INV ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+// ------------------------------------------------------------ butterflies
+template <int R, bool INV, typename C> struct Dft;
+
+template <bool INV, typename C> struct Dft<2, INV, C> {
+    static __device__ __forceinline__ void run(C* v)
+    {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV, typename C> struct Dft<4, INV, C> {
+    static __device__ __forceinline__ void run(C* v)
+    {
+        const C s0 = cadd(v[0], v[2]), s1 = csub(v[0], v[2]);
+        const C s2 = cadd(v[1], v[3]), s3 = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(s0, s2);
+        v[1] = cadd(s1, s3);
+        v[2] = csub(s0, s2);
+        v[3] = csub(s1, s3);
+    }
+};
+
+template <bool INV, typename C> struct Dft<8, INV, C> {
+    static __device__ __forceinline__ void run(C* v)
+    {
+        using R = decltype(v[0].x);
+        const R h = R(0.70710678118654752440);
+        C e[4] = {v[0], v[2], v[4], v[6]};
+        C o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV, C>::run(e);
+        Dft<4, INV, C>::run(o);
+        // o[k] *= W8^k
+        C o1, o3;
+        if (!INV) {
+            o1 = C{(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};
+            o3 = C{(o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h};
+        } else {
+            o1 = C{(o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h};
+            o3 = C{-(o[3].x + o[3].y) * h, (o[3].x - o[3].y) * h};
+        }
+        const C o2 = mul_mi<INV>(o[2]);
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+template <bool INV, typename C> struct Dft<3, INV, C> {
+    static __device__ __forceinline__ void run(C* v)
+    {
+        using R = decltype(v[0].x);
+        const R s3 = R(0.86602540378443864676);
+        const C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        const C m = C{v[0].x - R(0.5) * t1.x, v[0].y - R(0.5) * t1.y};
+        const C n = cscale(mul_mi<INV>(t2), s3);
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, n);
+        v[2] = csub(m, n);
+    }
+};
+
+template <bool INV, typename C> struct Dft<6, INV, C> {
+    static __device__ __forceinline__ void run(C* v)
+    {
+        using R = decltype(v[0].x);
+        const R s3 = R(0.86602540378443864676);
+        C e[3] = {v[0], v[2], v[4]};
+        C o[3] = {v[1], v[3], v[5]};
+        Dft<3, INV, C>::run(e);
+        Dft<3, INV, C>::run(o);
+        // W6 = 1/2 -/+ i sqrt(3)/2, W6^2 = -1/2 -/+ i sqrt(3)/2 (forward/inverse)
+        C o1, o2;
+        if (!INV) {
+            o1 = C{R(0.5) * o[1].x + s3 * o[1].y, R(0.5) * o[1].y - s3 * o[1].x};
+            o2 = C{-R(0.5) * o[2].x + s3 * o[2].y, -R(0.5) * o[2].y - s3 * o[2].x};
+        } else {
+            o1 = C{R(0.5) * o[1].x - s3 * o[1].y, R(0.5) * o[1].y + s3 * o[1].x};
+            o2 = C{-R(0.5) * o[2].x - s3 * o[2].y, -R(0.5) * o[2].y + s3 * o[2].x};
+        }
+        v[0] = cadd(e[0], o[0]);
+        v[3] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[4] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[5] = csub(e[2], o2);
+    }
+};
+
+// ------------------------------------------------------------ LDS layout
+// one padding slot every 8 elements breaks the power-of-two strides of the
+// Stockham write pattern across LDS banks
+__host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 3); }
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3) + 1; }
+
+// ------------------------------------------------------------ one stage
+template <typename C, int N, int E, int R, int NS, bool INV, bool LAST>
+__device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
+{
+    constexpr int T = N / E;
+    constexpr int BPT = E / R; // butterflies per lane
+    static_assert(E % R == 0, "radix must divide the per-lane element count");
+    C y[E];
+#pragma unroll
+    for (int m = 0; m < BPT; ++m) {
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = x[m + q * BPT];
+        if constexpr (NS > 1) {
+            const int k = (t + T * m) % NS;
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const C w = tw[q * k * (N / (NS * R))];
+                v[q] = INV ? cmulc(v[q], w) : cmul(v[q], w);
+            }
+        }
+        Dft<R, INV, C>::run(v);
+#pragma unroll
+        for (int q = 0; q < R; ++q) y[m * R + q] = v[q];
+    }
+    if constexpr (LAST) {
+#pragma unroll
+        for (int m = 0; m < BPT; ++m)
+#pragma unroll
+            for (int q = 0; q < R; ++q) x[m + q * BPT] = y[m * R + q];
+    } else {
+        __syncthreads(); // previous readers of this buffer are done
+#pragma unroll
+        for (int m = 0; m < BPT; ++m) {
+            const int j = t + T * m;
+            const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lds_pad(base + q * NS)] = y[m * R + q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
+    }
+}
+
+template <typename C, int N, int E, bool INV, int NS, int R, int... Rest>
+__device__ __forceinline__ void run_stages(C (&x)[E], C* lds, const C* tw, int t)
+{
+    fft_stage<C, N, E, R, NS, INV, sizeof...(Rest) == 0>(x, lds, tw, t);
+    if constexpr (sizeof...(Rest) > 0) run_stages<C, N, E, INV, NS * R, Rest...>(x, lds, tw, t);
+}
+
+// ------------------------------------------------------------ plans
+// E = values per lane; radices applied first to last.
+template <int N> struct Plan;
+#define MSF_PLAN(N_, E_, ...)                                                                  \
+    template <> struct Plan<N_> {                                                              \
+        static constexpr int N = N_, E = E_, T = N_ / E_;                                      \
+        template <bool INV, typename C>                                                        \
+        static __device__ __forceinline__ void run(C (&x)[E_], C* lds, const C* tw, int t)     \
+        {                                                                                      \
+            run_stages<C, N_, E_, INV, 1, __VA_ARGS__>(x, lds, tw, t);                         \
+        }                                                                                      \
+    };
+MSF_PLAN(16, 8, 8, 2)
+MSF_PLAN(32, 8, 8, 4)
+MSF_PLAN(64, 8, 8, 8)
+MSF_PLAN(96, 12, 6, 4, 4)
+MSF_PLAN(128, 8, 8, 4, 4)
+MSF_PLAN(192, 24, 6, 8, 4)
+MSF_PLAN(256, 8, 8, 8, 4)
+MSF_PLAN(384, 12, 6, 4, 4, 4)
+MSF_PLAN(512, 8, 8, 8, 8)
+MSF_PLAN(768, 24, 6, 8, 4, 4)
+MSF_PLAN(1024, 16, 8, 8, 4, 4)
+#undef MSF_PLAN
+
+} // namespace msf

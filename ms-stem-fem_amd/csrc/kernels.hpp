@@ -1,0 +1,112 @@
+// kernels.hpp -- host-callable launchers of the device kernels.
+//
+// Wave layout of the hot path (FP32 complex), for a batch of B probe
+// positions on an nx x ny grid whose slice step keeps the columns
+// [0, ka) U [ny - kb, ny) (the 2/3 band limit; every other column of the
+// (x, ky) representation is identically zero after the first slice):
+//   psi[b][i][jc], jc < ncp, ncp >= ka + kb rounded up to the column tile.
+// Between the two passes of a slice the wave is held with x along i and k
+// along j.  A slice step is
+//   row pass    : [IFFT_j] -> transmit -> FFT_j              (one pass over psi)
+//   column pass : FFT_i -> mask * P_x * P_y / s -> [IFFT_i]  (one pass over psi)
+// Only the scattered part phi = psi - psi0 of the wave is propagated in FP32;
+// the vacuum wave psi0 (t = 1) of every slice is computed once per run in FP64.
+// The reference's slice operator is linear over the reals (also with its
+// in-place transmission update), so phi obeys
+//   phi' = M P F[ T(F^-1 phi) + T(u0) - T_vac(u0) ],   u0 = s * shifted psi0(x),
+// and FP32 round-off scales with |phi| instead of |psi|.  The first row pass
+// has phi = 0; the last column pass adds the shifted vacuum exit wave back.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace msf {
+
+struct RowPassArgs {
+    float2* psi;            // [B][nx][ncp] scattered wave phi
+    const float2* t;        // [nx][ny] (t_re/s - 1/s, t_im/s) of this slice
+    const float2* u0;       // [nx][ny] vacuum wave entering this slice, s * real space
+    const int2* shift;      // [B] probe shifts of the batch
+    const float2* tw;       // [ny] forward roots
+    float inv_s;            // 1/sqrtNxNy
+    int nx, ncp, ka, kb, batch;
+};
+
+struct ColPassArgs {
+    float2* psi;            // [B][nx][ncp]
+    const float2* px;       // [nx] propagator along x of this slice
+    const float2* py;       // [ny] propagator along y of this slice, scaled by 1/s
+    const int2* keep;       // [nx] (a_i, b_i) kept-column ranges of row i
+    const float2* tw;       // [nx] forward roots
+    const float2* psi0;     // [nx][ny] vacuum exit wave in k-space (last pass only)
+    const int2* shift;      // [B] probe shifts (last pass only)
+    const float2* twy;      // [ny] forward roots (last pass only)
+    int ny, ncp, ka, kb, batch;
+};
+
+bool slice_lengths_ok(int n);
+// tile width (columns per workgroup) of the column pass for length nx
+int col_tile(int nx);
+
+hipError_t launch_row_pass(int ny, bool first, int transmit_mode, const RowPassArgs& a,
+                           hipStream_t s);
+hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s);
+
+// per (position, group) FP64 sums over pixel lists: kind 0 -> |psi|^2, 1 -> |psi|
+hipError_t launch_reduce_groups(const float2* psi, int64_t plane, const int32_t* offsets,
+                                const int32_t* pixels, const int32_t* kind, int groups,
+                                int batch, double* out, hipStream_t s);
+// ADF column of the partial sums -> adf[pos0 + b]
+hipError_t launch_collect_adf(const double* partial, int groups, int batch, double* adf,
+                              hipStream_t s);
+// FEM Welford update over the batch in raster order; state[3*nbins] = (n, mean, M2)
+hipError_t launch_fem_welford(const double* partial, int groups, int batch, int nbins,
+                              const int32_t* counts, const uint8_t* zeroed, double* state,
+                              hipStream_t s);
+hipError_t launch_fem_finalize(const double* state, int nbins, double* v, hipStream_t s);
+// compact exit wave -> full reference layout (FP32)
+hipError_t launch_expand_exit(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
+                              int batch, float2* out, hipStream_t s);
+
+// ---------------------------------------------------------------- FP64 set-up
+bool setup_lengths_ok(int n);
+// in-place 2-D FFT of `count` planes of nx x ny complex doubles
+hipError_t fft2d_f64(double2* data, int nx, int ny, int count, bool inverse, const double2* twx,
+                     const double2* twy, hipStream_t s);
+// probe H(k) of probe_wavefunction.cpp:37-239 inside the aperture ranges
+hipError_t launch_probe_k(double2* h, const double* kx, const double* ky, int nx, int ny,
+                          const int2* aperture, int corrected, double cs3, double cs5,
+                          double defocus, double lambda, double x0, double y0,
+                          hipStream_t s);
+// sum |a|^2 over n values (one workgroup, fixed order)
+hipError_t launch_norm2(const double2* a, int64_t n, double* out, hipStream_t s);
+// probe_s = (float) (a * scale)
+hipError_t launch_scale_to_f32(const double2* a, int64_t n, const double* norm2,
+                               double numerator, float2* out, hipStream_t s);
+hipError_t launch_fill_f64(double2* a, int64_t n, hipStream_t s);
+// atom count maps: maps[(slot*nsp + sp)][sx][sy] += 1
+hipError_t launch_scatter_atoms(double2* maps, const int4* atoms, int n_atoms, int nx, int ny,
+                                int nsp, hipStream_t s);
+// sigma_k[slot] = sum_sp T_sp(k) C_sp(k)   (scatterer_pap.hpp:104-130 template)
+hipError_t launch_combine_species(const double2* maps, double2* sigma, int slots, int nsp,
+                                  const double* params /* nsp*12 */, const double* kx,
+                                  const double* ky, int nx, int ny, const int2* support,
+                                  double lgab, double xmin, double ymin, hipStream_t s);
+// t = exp(-Im v) (cos Re v, sin Re v), v = sigma * scale
+hipError_t launch_exp_transmit(double2* a, int64_t n, double scale, hipStream_t s);
+// zero outside the kept ranges
+hipError_t launch_apply_keep(double2* a, int nx, int ny, int count, const int2* keep,
+                             hipStream_t s);
+// t = a / nxny (FP64, kept for queries); t32 = (t_re/s - 1/s, t_im/s) in FP32
+hipError_t launch_finish_transmit(double2* a, int64_t n, double nxny, double s, float2* t32,
+                                  hipStream_t st);
+// one vacuum slice step in k-space: a *= keep ? P_x[i] P_y[j] * scale : 0
+hipError_t launch_vacuum_step(double2* a, int nx, int ny, const double2* px, const double2* py,
+                              const int2* keep, double scale, hipStream_t s);
+// out = (float2) a
+hipError_t launch_to_f32(const double2* a, int64_t n, float2* out, hipStream_t s);
+// a *= f / sqrt(*norm2)
+hipError_t launch_scale_f64(double2* a, int64_t n, const double* norm2, double f, hipStream_t s);
+
+} // namespace msf

@@ -1,0 +1,381 @@
+// kernels_slice.hip -- FP32 slice-step passes and FP64 detector reductions.
+//
+// Reference behaviour restated (paths relative to the reference root):
+//   row pass    = adfstem.cpp:1891 (BACKWARD FFT, the y half) + :1922-1952
+//                 (transmission multiply, /sqrtNxNy) + :1986 (FORWARD FFT,
+//                 y half); the first pass starts from the circularly shifted
+//                 real-space probe of adfstem.cpp:1632-1671.
+//   column pass = adfstem.cpp:1986-1992 (FORWARD FFT, x half, /sqrtNxNy) +
+//                 bw_limit (bw_limit.cpp:30-47) + slice::propagate
+//                 (slice.cpp:1366-1421) + the next slice's BACKWARD FFT (x half).
+//   reductions  = sum_detected_intensity (func.cpp:2593-2641) and
+//                 integrate_out_phi_fftw (variance.cpp:64-300), FP64.
+#include "fft_engine.hpp"
+#include "kernels.hpp"
+
+namespace msf {
+
+// ------------------------------------------------------------------ row pass
+// Transmission of one row.  With t' = t/s = (tau + 1/s, ti) and the
+// reference's in-place update (TMODE 0: Im uses the updated Re), for the
+// scattered wave u and the vacuum wave u0:
+//   a  = t're u_re - ti u_im,            b  = t're u_im + ti a
+//   Da = tau u0_re - ti u0_im,           Db = tau u0_im + ti (u0_re/s + Da)
+// and TMODE 1 (exact product) uses b = t're u_im + ti u_re, Db = tau u0_im + ti u0_re.
+template <int NY, bool FIRST, int TMODE>
+__global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
+{
+    using P = Plan<NY>;
+    constexpr int E = P::E, T = P::T;
+    constexpr int RPW = 256 / T;
+    constexpr int LEN = lds_len(NY);
+    __shared__ float2 s_tw[NY];
+    __shared__ float2 s_buf[RPW * LEN];
+
+    for (int e = threadIdx.x; e < NY; e += blockDim.x) s_tw[e] = a.tw[e];
+    __syncthreads();
+
+    const int lr = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int64_t r = (int64_t)blockIdx.x * RPW + lr;
+    const bool active = r < (int64_t)a.batch * a.nx;
+    const int b = active ? (int)(r % a.batch) : 0;
+    const int i = active ? (int)(r / a.batch) : 0;
+    float2* row = a.psi + ((int64_t)b * a.nx + i) * a.ncp;
+    float2* lds = s_buf + lr * LEN;
+    const int hi = NY - a.kb;           // first kept column of the negative-ky band
+    const int off = a.ka + a.kb - NY;   // compact index = j + off for j >= hi
+
+    float2 x[E];
+    if constexpr (!FIRST) {
+#pragma unroll
+        for (int p = 0; p < E; ++p) {
+            const int j = t + T * p;
+            float2 v{0.f, 0.f};
+            if (active) {
+                if (j < a.ka) v = row[j];
+                else if (j >= hi) v = row[j + off];
+            }
+            x[p] = v;
+        }
+        P::template run<true>(x, lds, s_tw, t);
+    }
+
+    const int2 sh = active ? a.shift[b] : int2{0, 0};
+    int ii = i - sh.x;
+    if (ii < 0) ii += a.nx;
+    const float2* urow = a.u0 + (int64_t)ii * NY;
+    const float2* trow = a.t + (int64_t)i * NY;
+    const float is = a.inv_s;
+#pragma unroll
+    for (int p = 0; p < E; ++p) {
+        const int j = t + T * p;
+        int jj = j - sh.y;
+        if (jj < 0) jj += NY;
+        const float2 tv = trow[j];
+        const float2 u0 = urow[jj];
+        const float da = tv.x * u0.x - tv.y * u0.y;
+        const float db = (TMODE == 0) ? tv.x * u0.y + tv.y * (u0.x * is + da) : tv.x * u0.y + tv.y * u0.x;
+        if constexpr (FIRST) {
+            x[p] = float2{da, db};
+        } else {
+            const float tr = tv.x + is;
+            const float2 u = x[p];
+            const float re = tr * u.x - tv.y * u.y;
+            const float im = (TMODE == 0) ? tr * u.y + tv.y * re : tr * u.y + tv.y * u.x;
+            x[p] = float2{re + da, im + db};
+        }
+    }
+
+    P::template run<false>(x, lds, s_tw, t);
+
+    if (active) {
+#pragma unroll
+        for (int p = 0; p < E; ++p) {
+            const int j = t + T * p;
+            if (j < a.ka) row[j] = x[p];
+            else if (j >= hi) row[j + off] = x[p];
+        }
+    }
+}
+
+// --------------------------------------------------------------- column pass
+template <int NX> struct ColTile {
+    static constexpr int T = Plan<NX>::T;
+    static constexpr int W0 = 1024 / T < 16 ? 1024 / T : 16;
+    // keep the tile's LDS (W * lds_len * 8 B + twiddles) well inside 160 KiB
+    static constexpr int W = (W0 * lds_len(NX) * 8 > 96 * 1024) ? W0 / 2 : W0;
+};
+
+template <int NX, bool LAST>
+__global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
+{
+    using P = Plan<NX>;
+    constexpr int E = P::E, T = P::T;
+    constexpr int W = ColTile<NX>::W;
+    constexpr int LEN = lds_len(NX);
+    __shared__ float2 s_tw[NX];
+    __shared__ float2 s_buf[W * LEN];
+
+    for (int e = threadIdx.x; e < NX; e += blockDim.x) s_tw[e] = a.tw[e];
+    __syncthreads();
+
+    const int c = threadIdx.x % W;
+    const int t = threadIdx.x / W;
+    const int ntiles = a.ncp / W;
+    const int b = blockIdx.x / ntiles;
+    const int jc = (blockIdx.x % ntiles) * W + c;
+    const int nc = a.ka + a.kb;
+    const bool active = (b < a.batch) && (jc < nc);
+    const int j = (jc < a.ka) ? jc : jc - nc + a.ny;
+    float2* col = a.psi + (int64_t)b * NX * a.ncp + jc;
+    float2* lds = s_buf + c * LEN;
+
+    float2 x[E];
+#pragma unroll
+    for (int p = 0; p < E; ++p) x[p] = active ? col[(int64_t)(t + T * p) * a.ncp] : float2{0.f, 0.f};
+
+    P::template run<false>(x, lds, s_tw, t);
+
+    // band limit (bw_limit + propagate's k^2 test, decided on the host) and
+    // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
+    const float2 pyj = active ? a.py[j] : float2{0.f, 0.f};
+#pragma unroll
+    for (int p = 0; p < E; ++p) {
+        const int i = t + T * p;
+        const int2 kr = a.keep[i];
+        const bool keep = (j < kr.x) || (j >= a.ny - kr.y);
+        const float2 pxi = a.px[i];
+        const float2 pr = float2{pxi.x * pyj.x - pxi.y * pyj.y, pxi.x * pyj.y + pxi.y * pyj.x};
+        const float2 f = x[p];
+        x[p] = keep ? float2{pr.x * f.x - pr.y * f.y, pr.x * f.y + pr.y * f.x} : float2{0.f, 0.f};
+    }
+
+    if constexpr (!LAST) {
+        P::template run<true>(x, lds, s_tw, t);
+    } else if (active) {
+        // add the vacuum exit wave, shifted: psi0(k) e^{-2 pi i (i sx / nx + j sy / ny)}
+        const int2 sh = a.shift[b];
+        const float2 wy = a.twy[(int)(((int64_t)j * sh.y) % a.ny)];
+#pragma unroll
+        for (int p = 0; p < E; ++p) {
+            const int i = t + T * p;
+            const float2 wx = s_tw[(int)(((int64_t)i * sh.x) % NX)];
+            const float2 w = float2{wx.x * wy.x - wx.y * wy.y, wx.x * wy.y + wx.y * wy.x};
+            const float2 v0 = a.psi0[(int64_t)i * a.ny + j];
+            x[p].x += v0.x * w.x - v0.y * w.y;
+            x[p].y += v0.x * w.y + v0.y * w.x;
+        }
+    }
+
+    if (active) {
+#pragma unroll
+        for (int p = 0; p < E; ++p) col[(int64_t)(t + T * p) * a.ncp] = x[p];
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+bool slice_lengths_ok(int n)
+{
+    switch (n) {
+    case 16: case 32: case 64: case 96: case 128: case 192: case 256: case 384: case 512:
+    case 768: case 1024:
+        return true;
+    default:
+        return false;
+    }
+}
+
+int col_tile(int nx)
+{
+    switch (nx) {
+#define MSF_CASE(N) case N: return ColTile<N>::W;
+        MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
+        MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
+#undef MSF_CASE
+    default:
+        return 0;
+    }
+}
+
+template <int NY>
+static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStream_t s)
+{
+    constexpr int RPW = 256 / Plan<NY>::T;
+    const int64_t rows = (int64_t)a.batch * a.nx;
+    const dim3 grid((unsigned)((rows + RPW - 1) / RPW));
+    const dim3 block(RPW * Plan<NY>::T);
+    if (first) {
+        if (tmode == 0) hipLaunchKernelGGL((row_pass_kernel<NY, true, 0>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((row_pass_kernel<NY, true, 1>), grid, block, 0, s, a);
+    } else {
+        if (tmode == 0) hipLaunchKernelGGL((row_pass_kernel<NY, false, 0>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((row_pass_kernel<NY, false, 1>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, hipStream_t s)
+{
+    switch (ny) {
+#define MSF_CASE(N) case N: return row_pass_n<N>(first, tmode, a, s);
+        MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
+        MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
+#undef MSF_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int NX>
+static hipError_t col_pass_n(bool last, const ColPassArgs& a, hipStream_t s)
+{
+    constexpr int W = ColTile<NX>::W;
+    if (a.ncp % W) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((int64_t)a.batch * (a.ncp / W)));
+    const dim3 block(W * Plan<NX>::T);
+    if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((col_pass_kernel<NX, false>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s)
+{
+    switch (nx) {
+#define MSF_CASE(N) case N: return col_pass_n<N>(last, a, s);
+        MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
+        MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
+#undef MSF_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------- reductions
+// One wavefront per (group, position): lanes stride the group's pixel list,
+// accumulate in FP64, then a fixed shuffle tree; the result does not depend
+// on the batch composition.
+__global__ void __launch_bounds__(256) reduce_groups_kernel(const float2* psi, int64_t plane,
+                                                             const int32_t* offsets,
+                                                             const int32_t* pixels,
+                                                             const int32_t* kind, int groups,
+                                                             double* out)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int b = blockIdx.y;
+    if (g >= groups) return;
+    const float2* w = psi + (int64_t)b * plane;
+    const int k0 = offsets[g], k1 = offsets[g + 1];
+    const bool amp = kind[g] != 0;
+    double acc = 0.0;
+    for (int k = k0 + lane; k < k1; k += 64) {
+        const float2 v = w[pixels[k]];
+        const double re = v.x, im = v.y;
+        const double i2 = re * re + im * im;
+        acc += amp ? sqrt(i2) : i2;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
+    if (lane == 0) out[(int64_t)b * groups + g] = acc;
+}
+
+hipError_t launch_reduce_groups(const float2* psi, int64_t plane, const int32_t* offsets,
+                                const int32_t* pixels, const int32_t* kind, int groups,
+                                int batch, double* out, hipStream_t s)
+{
+    const dim3 grid((unsigned)((groups + 3) / 4), (unsigned)batch);
+    hipLaunchKernelGGL(reduce_groups_kernel, grid, dim3(256), 0, s, psi, plane, offsets, pixels,
+                       kind, groups, out);
+    return hipGetLastError();
+}
+
+__global__ void collect_adf_kernel(const double* partial, int groups, int batch, double* adf)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch) adf[b] = partial[(int64_t)b * groups];
+}
+
+hipError_t launch_collect_adf(const double* partial, int groups, int batch, double* adf,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(collect_adf_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, partial,
+                       groups, batch, adf);
+    return hipGetLastError();
+}
+
+// r = radial sum / count (adfstem.cpp:2591-2599), direct-beam bins zeroed
+// first (:2518-2522); Welford over positions in raster order.
+__global__ void fem_welford_kernel(const double* partial, int groups, int batch, int nbins,
+                                   const int32_t* counts, const uint8_t* zeroed, double* state)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbins) return;
+    double n = state[3 * k], mean = state[3 * k + 1], m2 = state[3 * k + 2];
+    const double c = (double)counts[k];
+    const bool z = zeroed[k] != 0;
+    for (int b = 0; b < batch; ++b) {
+        const double r = z ? 0.0 : partial[(int64_t)b * groups + 1 + k];
+        const double x = r / c;
+        n += 1.0;
+        const double d = x - mean;
+        mean += d / n;
+        m2 += d * (x - mean);
+    }
+    state[3 * k] = n;
+    state[3 * k + 1] = mean;
+    state[3 * k + 2] = m2;
+}
+
+hipError_t launch_fem_welford(const double* partial, int groups, int batch, int nbins,
+                              const int32_t* counts, const uint8_t* zeroed, double* state,
+                              hipStream_t s)
+{
+    if (nbins <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fem_welford_kernel, dim3((nbins + 255) / 256), dim3(256), 0, s, partial,
+                       groups, batch, nbins, counts, zeroed, state);
+    return hipGetLastError();
+}
+
+// V = Npos sum r^2 / (sum r)^2 - 1 = M2 / (n mean^2); 0 when sum r == 0
+// (adfstem.cpp:2837-2856)
+__global__ void fem_finalize_kernel(const double* state, int nbins, double* v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbins) return;
+    const double n = state[3 * k], mean = state[3 * k + 1], m2 = state[3 * k + 2];
+    v[k] = (mean == 0.0) ? 0.0 : m2 / (n * mean * mean);
+}
+
+hipError_t launch_fem_finalize(const double* state, int nbins, double* v, hipStream_t s)
+{
+    if (nbins <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fem_finalize_kernel, dim3((nbins + 255) / 256), dim3(256), 0, s, state,
+                       nbins, v);
+    return hipGetLastError();
+}
+
+__global__ void expand_exit_kernel(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
+                                   int batch, float2* out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)batch * nx * ny;
+    if (idx >= total) return;
+    const int j = (int)(idx % ny);
+    const int64_t r = idx / ny; // b * nx + i
+    float2 v{0.f, 0.f};
+    if (j < ka) v = psi[r * ncp + j];
+    else if (j >= ny - kb) v = psi[r * ncp + j - ny + ka + kb];
+    out[idx] = v;
+}
+
+hipError_t launch_expand_exit(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
+                              int batch, float2* out, hipStream_t s)
+{
+    const int64_t total = (int64_t)batch * nx * ny;
+    hipLaunchKernelGGL(expand_exit_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       psi, nx, ny, ncp, ka, kb, batch, out);
+    return hipGetLastError();
+}
+
+} // namespace msf

@@ -1,0 +1,730 @@
+// msf_api.cpp -- C ABI (include/msf.h): specimen set-up, probe, raster loop.
+//
+// Host orchestration of the device kernels.  Reference seams replaced:
+//   tem.cpp:944-1276 (species templates, auto-slicing, propagators,
+//   transmission functions)  -> msf_specimen_build
+//   TEM_NS::adfstem (adfstem.hpp:66-121)  -> msf_run_stem
+#include "../../include/msf.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "msf_host.hpp"
+
+namespace {
+
+const double kKirkland[103][12] = {
+#include "kirkland_params.inc"
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        release();
+        if (n == 0) n = 16;
+        bytes = n;
+        return hipMalloc(&p, n);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+struct msf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0};
+    msf_stats stats{};
+    std::map<int, DevBuf> tw32, tw64;
+};
+
+struct msf_specimen {
+    msf_ctx* ctx = nullptr;
+    msf_grid grid{};
+    msf::Domain dom;
+    int n_slices = 0;
+    std::vector<double> lower, thick;
+    std::vector<uint32_t> counts;
+    DevBuf t32;  // [S][nx][ny] float2, t / sqrtNxNy
+    DevBuf t64;  // [S][nx][ny] double2, t as built
+    DevBuf px;   // [S][nx] float2
+    DevBuf py;   // [S][ny] float2, P_y / sqrtNxNy
+    DevBuf px64; // [S][nx] double2
+    DevBuf py64; // [S][ny] double2
+};
+
+namespace {
+
+int fail(msf_ctx* c, int code, const std::string& m)
+{
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE,    \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+template <typename T> hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s)
+{
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+// forward roots W_N^e = exp(-2 pi i e / N), FP64 then rounded
+int twiddles(msf_ctx* c, int n, const float2** f32, const double2** f64)
+{
+    if (!c->tw32.count(n)) {
+        std::vector<double2> d(n);
+        std::vector<float2> f(n);
+        for (int e = 0; e < n; ++e) {
+            const double a = 2.0 * M_PI * (double)e / (double)n;
+            d[e] = double2{std::cos(a), -std::sin(a)};
+            f[e] = float2{(float)d[e].x, (float)d[e].y};
+        }
+        HIPCHK(c, upload(c->tw64[n], d, c->stream));
+        HIPCHK(c, upload(c->tw32[n], f, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (f32) *f32 = c->tw32[n].as<float2>();
+    if (f64) *f64 = c->tw64[n].as<double2>();
+    return MSF_OK;
+}
+
+int check_grid(msf_ctx* c, const msf_grid* g)
+{
+    if (!g) return fail(c, MSF_ERR_INVALID, "grid is NULL");
+    if (!msf::slice_lengths_ok((int)g->nx) || !msf::slice_lengths_ok((int)g->ny))
+        return fail(c, MSF_ERR_UNSUPPORTED,
+                    "no compiled transform for samples " + std::to_string(g->nx) + " x " +
+                        std::to_string(g->ny));
+    if (!(g->xperiod > 0.0) || !(g->yperiod > 0.0) || !(g->zperiod > 0.0))
+        return fail(c, MSF_ERR_INVALID, "periods must be positive (tem_io.cpp:4681-4719)");
+    if (!(g->voltage > 0.0)) return fail(c, MSF_ERR_INVALID, "microscope_voltage must be positive");
+    return MSF_OK;
+}
+
+std::vector<int2> to_int2(const msf::RowRanges& r)
+{
+    std::vector<int2> v(r.ab.size() / 2);
+    for (size_t i = 0; i < v.size(); ++i) v[i] = int2{r.ab[2 * i], r.ab[2 * i + 1]};
+    return v;
+}
+
+// H(k) -> BACKWARD FFT; leaves h = IFFT(H) (unnormalised) and its squared norm
+int probe_device(msf_ctx* c, const msf::Domain& d, const msf_optics* o, DevBuf& h, DevBuf& norm2)
+{
+    const int nx = d.nx, ny = d.ny;
+    const msf::RowRanges ap = msf::aperture_keep_ranges(d, o->alpha_max);
+    if (!ap.ok) return fail(c, MSF_ERR_INVALID, "aperture is not a disk in FFT order");
+    const double2 *twx, *twy;
+    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+    DevBuf dkx, dky, dap;
+    HIPCHK(c, upload(dkx, d.kx, c->stream));
+    HIPCHK(c, upload(dky, d.ky, c->stream));
+    HIPCHK(c, upload(dap, to_int2(ap), c->stream));
+    HIPCHK(c, h.alloc((size_t)nx * ny * sizeof(double2)));
+    HIPCHK(c, norm2.alloc(sizeof(double)));
+    HIPCHK(c, msf::launch_probe_k(h.as<double2>(), dkx.as<double>(), dky.as<double>(), nx, ny,
+                                  dap.as<int2>(), o->corrected, o->cs3, o->cs5, o->defocus,
+                                  d.lambda, d.xx[0], d.yy[0], c->stream));
+    HIPCHK(c, msf::fft2d_f64(h.as<double2>(), nx, ny, 1, true, twx, twy, c->stream));
+    HIPCHK(c, msf::launch_norm2(h.as<double2>(), (int64_t)nx * ny, norm2.as<double>(), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MSF_OK;
+}
+
+int check_optics(msf_ctx* c, const msf_optics* o)
+{
+    if (!o) return fail(c, MSF_ERR_INVALID, "optics is NULL");
+    if (!(o->alpha_max > 0.0)) return fail(c, MSF_ERR_INVALID, "alphamax must be positive");
+    if (o->corrected != 0 && o->corrected != 1)
+        return fail(c, MSF_ERR_INVALID, "corrected must be 0 or 1");
+    return MSF_OK;
+}
+
+struct RunOut {
+    double* adf = nullptr;
+    double* fem = nullptr;
+    float* exit = nullptr;
+};
+
+int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
+                 const msf_detector* det, const msf_fem* fem, const double* x_p, uint32_t n_xp,
+                 const double* y_p, uint32_t n_yp, RunOut out)
+{
+    const double t_start = now_ms();
+    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+    if (int r = check_optics(c, o)) return r;
+    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+    const msf::Domain& d = sp->dom;
+    const int nx = d.nx, ny = d.ny, S = sp->n_slices;
+    const bool want_adf = det && det->enabled;
+    const bool want_fem = fem && fem->enabled;
+    if (want_adf) {
+        if (det->inner < 0.0 || det->outer < 0.0)
+            return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
+        if (det->inner > d.bw_t * d.lambda)
+            return fail(c, MSF_ERR_INVALID,
+                        "detector inner angle falls beyond bandwidth limits (tem.cpp:721-732)");
+    }
+    if (want_fem && !(fem->dr > 0.0)) return fail(c, MSF_ERR_INVALID, "dr must be positive");
+    const int64_t P = (int64_t)n_xp * n_yp;
+    const int tmode = c->cfg.transmit_mode;
+
+    // probe shifts in raster order, persisting when no index qualifies
+    std::vector<int2> shifts(P);
+    {
+        int64_t sx = 0, sy = 0;
+        for (uint32_t a = 0; a < n_xp; ++a)
+            for (uint32_t b = 0; b < n_yp; ++b) {
+                sx = msf::shift_index(d.xx, x_p[a], sx);
+                sy = msf::shift_index(d.yy, y_p[b], sy);
+                if (sx < 0 || sy < 0)
+                    return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
+                shifts[(size_t)a * n_yp + b] = int2{(int)sx, (int)sy};
+            }
+    }
+
+    // band-limited column set of the slice step
+    const msf::RowRanges keep = msf::slice_keep_ranges(d);
+    if (!keep.ok) return fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order");
+    const int W = msf::col_tile(nx);
+    const int ka = keep.amax, kb = keep.bmax;
+    const int ncp = std::max(W, ((ka + kb + W - 1) / W) * W);
+
+    double k1 = 0.0, k2 = 0.0;
+    if (want_adf) {
+        k1 = det->inner / d.lambda;
+        k2 = det->outer / d.lambda;
+        if (det->threshold_mode == MSF_ADF_PHYSICAL) {
+            k1 = k1 * k1;
+            k2 = k2 * k2;
+        }
+    }
+    const msf::Groups g = msf::build_groups(d, keep, ncp, want_adf, k1, k2, want_fem,
+                                            want_fem ? fem->dr : 1.0, o->alpha_max);
+    const int G = (int)g.kind.size();
+    const int nbins = g.nbins;
+
+    // vacuum wave (t = 1) of every slice, FP64, from the probe of
+    // adfstem.cpp:586-817: u0_0 = s * A_p * IFFT(H)/s; then per slice
+    // w = FFT(u0_n) / s^2 * M P_n (adfstem.cpp:1986-2043), u0_{n+1} = IFFT(w)
+    const int64_t M = (int64_t)nx * ny;
+    DevBuf h, norm2, dU0, dPsi0;
+    if (int r = probe_device(c, d, o, h, norm2)) return r;
+    const double2 *twx64, *twy64;
+    if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
+    if (int r = twiddles(c, ny, nullptr, &twy64)) return r;
+    HIPCHK(c, dU0.alloc((size_t)S * M * sizeof(float2)));
+    HIPCHK(c, dPsi0.alloc((size_t)M * sizeof(float2)));
+    {
+        DevBuf dkeep64;
+        HIPCHK(c, upload(dkeep64, to_int2(keep), c->stream));
+        double2* v = h.as<double2>();
+        HIPCHK(c, msf::launch_scale_f64(v, M, norm2.as<double>(), d.sqrt_nxny, c->stream));
+        for (int n = 0; n < S; ++n) {
+            HIPCHK(c, msf::launch_to_f32(v, M, dU0.as<float2>() + (int64_t)n * M, c->stream));
+            HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, false, twx64, twy64, c->stream));
+            HIPCHK(c, msf::launch_vacuum_step(v, nx, ny, sp->px64.as<double2>() + (int64_t)n * nx,
+                                              sp->py64.as<double2>() + (int64_t)n * ny,
+                                              dkeep64.as<int2>(), 1.0 / (d.sqrt_nxny * d.sqrt_nxny),
+                                              c->stream));
+            if (n + 1 < S) HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, true, twx64, twy64, c->stream));
+        }
+        HIPCHK(c, msf::launch_to_f32(v, M, dPsi0.as<float2>(), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    h.release();
+
+    const float2 *twx, *twy;
+    if (int r = twiddles(c, nx, &twx, nullptr)) return r;
+    if (int r = twiddles(c, ny, &twy, nullptr)) return r;
+
+    int64_t B = c->cfg.batch ? (int64_t)c->cfg.batch : std::min<int64_t>(P, 128);
+    B = std::max<int64_t>(1, std::min<int64_t>(B, P));
+
+    DevBuf dshift, dkeep, doff, dpix, dkind, dcnt, dzero, dpsi, dpart, dadf, dstate, dfemv, dexit;
+    HIPCHK(c, upload(dshift, shifts, c->stream));
+    HIPCHK(c, upload(dkeep, to_int2(keep), c->stream));
+    HIPCHK(c, upload(doff, g.offsets, c->stream));
+    HIPCHK(c, upload(dpix, g.pixels, c->stream));
+    HIPCHK(c, upload(dkind, g.kind, c->stream));
+    HIPCHK(c, upload(dcnt, g.counts, c->stream));
+    HIPCHK(c, upload(dzero, g.zeroed, c->stream));
+    const int64_t plane = (int64_t)nx * ncp;
+    HIPCHK(c, dpsi.alloc((size_t)B * plane * sizeof(float2)));
+    HIPCHK(c, hipMemsetAsync(dpsi.p, 0, dpsi.bytes, c->stream));
+    HIPCHK(c, dpart.alloc((size_t)B * G * sizeof(double)));
+    HIPCHK(c, dadf.alloc((size_t)P * sizeof(double)));
+    HIPCHK(c, dstate.alloc((size_t)std::max(nbins, 1) * 3 * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(dstate.p, 0, dstate.bytes, c->stream));
+    HIPCHK(c, dfemv.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
+    if (out.exit) HIPCHK(c, dexit.alloc((size_t)B * nx * ny * sizeof(float2)));
+
+    // events for per-kernel timing
+    const bool prof = c->cfg.profile_kernels != 0;
+    const int64_t n_batches = (P + B - 1) / B;
+    enum { K_ROW0, K_ROW, K_COL, K_COLL, K_RED, K_NUM };
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kind;
+    auto ev_begin = [&](int kind) -> hipError_t {
+        if (!prof) return hipSuccess;
+        hipEvent_t a, b;
+        hipError_t e = hipEventCreate(&a);
+        if (e == hipSuccess) e = hipEventCreate(&b);
+        if (e != hipSuccess) return e;
+        ev.push_back(a);
+        ev.push_back(b);
+        ev_kind.push_back(kind);
+        return hipEventRecord(a, c->stream);
+    };
+    auto ev_end = [&]() -> hipError_t { return prof ? hipEventRecord(ev.back(), c->stream) : hipSuccess; };
+
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double t_loop = now_ms();
+    msf::RowPassArgs ra{};
+    ra.nx = nx;
+    ra.ncp = ncp;
+    ra.ka = ka;
+    ra.kb = kb;
+    ra.tw = twy;
+    ra.inv_s = (float)(1.0 / d.sqrt_nxny);
+    msf::ColPassArgs ca{};
+    ca.ny = ny;
+    ca.ncp = ncp;
+    ca.ka = ka;
+    ca.kb = kb;
+    ca.keep = dkeep.as<int2>();
+    ca.tw = twx;
+    ca.psi0 = dPsi0.as<float2>();
+    ca.twy = twy;
+    const float2* t32 = sp->t32.as<float2>();
+    const float2* px = sp->px.as<float2>();
+    const float2* py = sp->py.as<float2>();
+    for (int64_t b0 = 0; b0 < P; b0 += B) {
+        const int nb = (int)std::min<int64_t>(B, P - b0);
+        ra.psi = dpsi.as<float2>();
+        ra.batch = nb;
+        ra.shift = dshift.as<int2>() + b0;
+        ca.psi = dpsi.as<float2>();
+        ca.batch = nb;
+        ca.shift = dshift.as<int2>() + b0;
+        for (int n = 0; n < S; ++n) {
+            ra.t = t32 + (int64_t)n * M;
+            ra.u0 = dU0.as<float2>() + (int64_t)n * M;
+            HIPCHK(c, ev_begin(n == 0 ? K_ROW0 : K_ROW));
+            HIPCHK(c, msf::launch_row_pass(ny, n == 0, tmode, ra, c->stream));
+            HIPCHK(c, ev_end());
+            ca.px = px + (int64_t)n * nx;
+            ca.py = py + (int64_t)n * ny;
+            HIPCHK(c, ev_begin(n == S - 1 ? K_COLL : K_COL));
+            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, ca, c->stream));
+            HIPCHK(c, ev_end());
+        }
+        HIPCHK(c, ev_begin(K_RED));
+        HIPCHK(c, msf::launch_reduce_groups(dpsi.as<float2>(), plane, doff.as<int32_t>(),
+                                            dpix.as<int32_t>(), dkind.as<int32_t>(), G, nb,
+                                            dpart.as<double>(), c->stream));
+        HIPCHK(c, ev_end());
+        if (want_adf)
+            HIPCHK(c, msf::launch_collect_adf(dpart.as<double>(), G, nb, dadf.as<double>() + b0,
+                                              c->stream));
+        if (want_fem)
+            HIPCHK(c, msf::launch_fem_welford(dpart.as<double>(), G, nb, nbins, dcnt.as<int32_t>(),
+                                              dzero.as<uint8_t>(), dstate.as<double>(),
+                                              c->stream));
+        if (out.exit) {
+            HIPCHK(c, msf::launch_expand_exit(dpsi.as<float2>(), nx, ny, ncp, ka, kb, nb,
+                                              dexit.as<float2>(), c->stream));
+            HIPCHK(c, hipMemcpyAsync(out.exit + (size_t)b0 * nx * ny * 2, dexit.p,
+                                     (size_t)nb * nx * ny * sizeof(float2), hipMemcpyDeviceToHost,
+                                     c->stream));
+        }
+    }
+    if (want_fem)
+        HIPCHK(c, msf::launch_fem_finalize(dstate.as<double>(), nbins, dfemv.as<double>(), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double t_end = now_ms();
+
+    if (want_adf && out.adf)
+        HIPCHK(c, hipMemcpy(out.adf, dadf.p, (size_t)P * sizeof(double), hipMemcpyDeviceToHost));
+    if (want_fem && out.fem && nbins > 0)
+        HIPCHK(c, hipMemcpy(out.fem, dfemv.p, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost));
+
+    msf_stats st{};
+    st.setup_ms = t_loop - t_start;
+    st.slice_loop_ms = t_end - t_loop;
+    st.total_ms = now_ms() - t_start;
+    st.probe_slices = (uint64_t)P * (uint64_t)S;
+    st.batch = (uint32_t)B;
+    st.n_batches = (uint32_t)n_batches;
+    // compulsory traffic of a full-batch pass: read + write of the compact wave
+    st.row_pass_bytes = 2.0 * (double)B * nx * (ka + kb) * sizeof(float2);
+    st.col_pass_bytes = st.row_pass_bytes;
+    if (prof) {
+        double sum[K_NUM] = {0}, cnt[K_NUM] = {0};
+        for (size_t k = 0; k < ev_kind.size(); ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess) {
+                sum[ev_kind[k]] += ms;
+                cnt[ev_kind[k]] += 1;
+            }
+        }
+        auto avg = [&](int k) { return cnt[k] > 0 ? 1000.0 * sum[k] / cnt[k] : 0.0; };
+        st.first_row_pass_us = avg(K_ROW0);
+        st.row_pass_us = avg(K_ROW);
+        st.col_pass_us = avg(K_COL);
+        st.last_col_pass_us = avg(K_COLL);
+        st.reduce_us = avg(K_RED);
+        st.row_pass_count = (uint64_t)cnt[K_ROW];
+        st.col_pass_count = (uint64_t)cnt[K_COL];
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    c->stats = st;
+    return MSF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int msf_create(msf_ctx** out, int device)
+{
+    if (!out) return MSF_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MSF_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return MSF_ERR_DEVICE;
+    msf_ctx* c = new msf_ctx;
+    c->device = device;
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return MSF_ERR_DEVICE;
+    }
+    *out = c;
+    return MSF_OK;
+}
+
+int msf_destroy(msf_ctx* c)
+{
+    if (!c) return MSF_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->tw32.clear();
+    c->tw64.clear();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MSF_OK;
+}
+
+const char* msf_last_error(const msf_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int msf_set_config(msf_ctx* c, const msf_config* cfg)
+{
+    if (!c || !cfg) return MSF_ERR_INVALID;
+    if (cfg->transmit_mode != MSF_TRANSMIT_REFERENCE && cfg->transmit_mode != MSF_TRANSMIT_EXACT)
+        return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
+    c->cfg = *cfg;
+    return MSF_OK;
+}
+
+int msf_get_stats(const msf_ctx* c, msf_stats* out)
+{
+    if (!c || !out) return MSF_ERR_INVALID;
+    *out = c->stats;
+    return MSF_OK;
+}
+
+int msf_supported_length(uint32_t n) { return msf::slice_lengths_ok((int)n) ? 1 : 0; }
+
+int msf_wavelength(double voltage, double* lambda, double* gamma)
+{
+    if (!(voltage > 0.0) || !lambda || !gamma) return MSF_ERR_INVALID;
+    msf::wavelength(voltage, *lambda, *gamma);
+    return MSF_OK;
+}
+
+double msf_scherzer_defocus(int32_t corrected, double lambda, double cs)
+{
+    if (corrected) return -0.983 * std::pow(msf::ref_pow2(lambda) * cs, 1.0 / 3.0); // scherzer.cpp:152-158
+    return 0.87 * std::sqrt(cs * lambda);                                          // scherzer.cpp:61-69
+}
+
+double msf_scherzer_alphamax(int32_t corrected, double lambda, double cs)
+{
+    if (corrected) return 1.513 * std::pow(lambda / cs, 1.0 / 6.0); // scherzer.cpp:160-166
+    return 1.34 * std::pow(lambda / cs, 0.25);                      // scherzer.cpp:71-79
+}
+
+int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const uint32_t* Z,
+                       uint32_t n, double min_slice, msf_specimen** out)
+{
+    if (!c || !out) return MSF_ERR_INVALID;
+    *out = nullptr;
+    if (int r = check_grid(c, grid)) return r;
+    if (n > 0 && (!q || !Z)) return fail(c, MSF_ERR_INVALID, "atom arrays are NULL");
+    if (!(min_slice >= 0.0)) return fail(c, MSF_ERR_INVALID, "minslice must be >= 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nx = (int)grid->nx, ny = (int)grid->ny;
+    msf_specimen* sp = new msf_specimen;
+    sp->ctx = c;
+    sp->grid = *grid;
+    sp->dom = msf::make_domain(nx, ny, grid->xperiod, grid->yperiod, grid->xmin, grid->ymin,
+                               grid->voltage);
+    const msf::Domain& d = sp->dom;
+
+    // species present, ascending Z (tem.cpp:944-972)
+    std::vector<int> species_of(104, -1), zlist;
+    for (uint32_t a = 0; a < n; ++a) {
+        if (Z[a] < 1 || Z[a] > 103) {
+            delete sp;
+            return fail(c, MSF_ERR_INVALID, "atomic number outside 1..103 (tem_io.cpp:4768-4781)");
+        }
+        species_of[Z[a]] = 0;
+    }
+    for (int z = 1; z <= 103; ++z)
+        if (species_of[z] == 0) {
+            species_of[z] = (int)zlist.size();
+            zlist.push_back(z);
+        }
+    const int nsp = (int)zlist.size();
+
+    // auto-slicing and per-atom nearest-pixel shifts
+    msf::Slices sl = msf::autoslice(q, n, grid->zmin, grid->zperiod, min_slice);
+    const int S = (int)sl.lower.size();
+    sp->n_slices = S;
+    sp->lower = sl.lower;
+    sp->thick = sl.thick;
+    sp->counts.resize(S);
+    std::vector<int4> atoms; // (slice, species, sx, sy)
+    atoms.reserve(n);
+    for (int s = 0; s < S; ++s) {
+        sp->counts[s] = (uint32_t)sl.atoms[s].size();
+        for (int32_t a : sl.atoms[s]) {
+            const int64_t sx = msf::shift_index(d.xx, q[3 * a], 0);
+            const int64_t sy = msf::shift_index(d.yy, q[3 * a + 1], 0);
+            if (sx < 0 || sy < 0) {
+                delete sp;
+                return fail(c, MSF_ERR_DOMAIN, "atom position below the domain minimum");
+            }
+            atoms.push_back(int4{s, species_of[Z[a]], (int)sx, (int)sy});
+        }
+    }
+
+    // propagators (slice.cpp:1293-1363), FP64 -> FP32; P_y carries 1/sqrtNxNy
+    std::vector<float2> pxs((size_t)S * nx), pys((size_t)S * ny);
+    std::vector<double2> pxd((size_t)S * nx), pyd((size_t)S * ny);
+    for (int s = 0; s < S; ++s) {
+        std::vector<double> p;
+        msf::propagator(d.lambda, sl.thick[s], d.kx, d.bw_t, p);
+        for (int i = 0; i < nx; ++i) {
+            pxd[(size_t)s * nx + i] = double2{p[2 * i], p[2 * i + 1]};
+            pxs[(size_t)s * nx + i] = float2{(float)p[2 * i], (float)p[2 * i + 1]};
+        }
+        msf::propagator(d.lambda, sl.thick[s], d.ky, d.bw_t, p);
+        for (int j = 0; j < ny; ++j) {
+            pyd[(size_t)s * ny + j] = double2{p[2 * j], p[2 * j + 1]};
+            pys[(size_t)s * ny + j] = float2{(float)(p[2 * j] / d.sqrt_nxny), (float)(p[2 * j + 1] / d.sqrt_nxny)};
+        }
+    }
+    int rc = MSF_OK;
+    auto bail = [&](int code) {
+        delete sp;
+        return code;
+    };
+    if (hipError_t e = upload(sp->px, pxs, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+    if (hipError_t e = upload(sp->py, pys, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+    if (hipError_t e = upload(sp->px64, pxd, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+    if (hipError_t e = upload(sp->py64, pyd, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+
+    // transmission functions, FP64 on the device, in chunks of slices
+    const msf::RowRanges sup = msf::template_keep_ranges(d);
+    const msf::RowRanges bw = msf::bwlimit_keep_ranges(d);
+    if (!sup.ok || !bw.ok) return bail(fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order"));
+    std::vector<double> params;
+    for (int z : zlist) params.insert(params.end(), kKirkland[z - 1], kKirkland[z - 1] + 12);
+    const int64_t M = (int64_t)nx * ny;
+    const double2 *twx, *twy;
+    if ((rc = twiddles(c, nx, nullptr, &twx))) return bail(rc);
+    if ((rc = twiddles(c, ny, nullptr, &twy))) return bail(rc);
+    DevBuf dkx, dky, dsup, dbw, dpar, dmaps, datoms;
+    hipError_t e = hipSuccess;
+    if (!e) e = upload(dkx, d.kx, c->stream);
+    if (!e) e = upload(dky, d.ky, c->stream);
+    if (!e) e = upload(dsup, to_int2(sup), c->stream);
+    if (!e) e = upload(dbw, to_int2(bw), c->stream);
+    if (!e) e = upload(dpar, params, c->stream);
+    if (!e) e = sp->t64.alloc((size_t)S * M * sizeof(double2));
+    if (!e) e = sp->t32.alloc((size_t)S * M * sizeof(float2));
+    const int64_t per_slot = (int64_t)std::max(nsp, 1) * M * (int64_t)sizeof(double2);
+    const int K = (int)std::max<int64_t>(1, std::min<int64_t>(S, (256ll << 20) / per_slot));
+    if (!e) e = dmaps.alloc((size_t)K * per_slot);
+    if (!e) e = datoms.alloc(std::max<size_t>(atoms.size(), 1) * sizeof(int4));
+    if (!e && !atoms.empty())
+        e = hipMemcpyAsync(datoms.p, atoms.data(), atoms.size() * sizeof(int4), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e) return bail(fail(c, e == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE, hipGetErrorString(e)));
+    const double lgab = d.lambda * d.gamma * (1 / (grid->xperiod * grid->yperiod));
+    size_t a0 = 0;
+    for (int s0 = 0; s0 < S && !e; s0 += K) {
+        const int k = std::min(K, S - s0);
+        size_t a1 = a0;
+        while (a1 < atoms.size() && atoms[a1].x < s0 + k) ++a1;
+        // slots are relative to the chunk
+        std::vector<int4> chunk(atoms.begin() + a0, atoms.begin() + a1);
+        for (auto& at : chunk) at.x -= s0;
+        if (!chunk.empty())
+            e = hipMemcpyAsync(datoms.p, chunk.data(), chunk.size() * sizeof(int4), hipMemcpyHostToDevice,
+                               c->stream);
+        double2* maps = dmaps.as<double2>();
+        double2* sig = sp->t64.as<double2>() + (int64_t)s0 * M;
+        if (!e) e = msf::launch_fill_f64(maps, (int64_t)k * std::max(nsp, 1) * M, c->stream);
+        if (!e) e = msf::launch_scatter_atoms(maps, datoms.as<int4>(), (int)chunk.size(), nx, ny,
+                                              std::max(nsp, 1), c->stream);
+        if (!e && nsp > 0) e = msf::fft2d_f64(maps, nx, ny, k * nsp, false, twx, twy, c->stream);
+        if (!e) e = msf::launch_combine_species(maps, sig, k, nsp, dpar.as<double>(), dkx.as<double>(),
+                                                dky.as<double>(), nx, ny, dsup.as<int2>(), lgab,
+                                                grid->xmin, grid->ymin, c->stream);
+        if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
+        if (!e) e = msf::launch_exp_transmit(sig, (int64_t)k * M, 1.0 / std::sqrt((double)M), c->stream);
+        if (!e) e = msf::fft2d_f64(sig, nx, ny, k, false, twx, twy, c->stream);
+        if (!e) e = msf::launch_apply_keep(sig, nx, ny, k, dbw.as<int2>(), c->stream);
+        if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
+        if (!e) e = msf::launch_finish_transmit(sig, (int64_t)k * M, d.nxny, d.sqrt_nxny,
+                                                sp->t32.as<float2>() + (int64_t)s0 * M, c->stream);
+        // the atom list buffer is reused by the next chunk
+        if (!e) e = hipStreamSynchronize(c->stream);
+        a0 = a1;
+    }
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) return bail(fail(c, MSF_ERR_DEVICE, std::string("transmission build: ") + hipGetErrorString(e)));
+    *out = sp;
+    return MSF_OK;
+}
+
+int msf_specimen_free(msf_specimen* sp)
+{
+    if (!sp) return MSF_ERR_INVALID;
+    if (sp->ctx) (void)hipSetDevice(sp->ctx->device);
+    delete sp;
+    return MSF_OK;
+}
+
+uint32_t msf_specimen_slices(const msf_specimen* sp) { return sp ? (uint32_t)sp->n_slices : 0; }
+
+int msf_specimen_slice_info(const msf_specimen* sp, double* lower, double* thick, uint32_t* atoms)
+{
+    if (!sp) return MSF_ERR_INVALID;
+    for (int s = 0; s < sp->n_slices; ++s) {
+        if (lower) lower[s] = sp->lower[s];
+        if (thick) thick[s] = sp->thick[s];
+        if (atoms) atoms[s] = sp->counts[s];
+    }
+    return MSF_OK;
+}
+
+int msf_specimen_transmission(const msf_specimen* sp, uint32_t slice, double* t)
+{
+    if (!sp || !t || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
+    const size_t M = (size_t)sp->dom.nx * sp->dom.ny;
+    (void)hipSetDevice(sp->ctx->device);
+    if (hipMemcpy(t, sp->t64.as<double2>() + slice * M, M * sizeof(double2), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the transmission function failed");
+    return MSF_OK;
+}
+
+int msf_probe_realspace(msf_ctx* c, const msf_grid* grid, const msf_optics* o, double* probe)
+{
+    if (!c || !probe) return MSF_ERR_INVALID;
+    if (int r = check_grid(c, grid)) return r;
+    if (int r = check_optics(c, o)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    DevBuf h, norm2;
+    if (int r = probe_device(c, d, o, h, norm2)) return r;
+    const size_t M = (size_t)d.nx * d.ny;
+    HIPCHK(c, hipMemcpy(probe, h.p, M * sizeof(double2), hipMemcpyDeviceToHost));
+    // adfstem.cpp:699-704 (/ sqrtNxNy), probe_wavefunction.cpp:300-321 (A_p), :813-817
+    for (size_t k = 0; k < 2 * M; ++k) probe[k] = probe[k] / d.sqrt_nxny;
+    double ap = 0.0;
+    for (size_t k = 0; k < M; ++k) ap += probe[2 * k] * probe[2 * k] + probe[2 * k + 1] * probe[2 * k + 1];
+    ap = 1.0 / std::sqrt(ap);
+    for (size_t k = 0; k < 2 * M; ++k) probe[k] = ap * probe[k];
+    return MSF_OK;
+}
+
+int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lower_edges)
+{
+    if (!grid || !n_bins || !(dr > 0.0) || grid->nx < 2 || grid->ny < 2) return MSF_ERR_INVALID;
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    double dk = 0.0;
+    const int nb = msf::fem_bin_count(d, dr, dk);
+    *n_bins = (uint32_t)nb;
+    if (lower_edges)
+        for (int i = 0; i < nb; ++i) lower_edges[i] = (unsigned int)i * dk;
+    return MSF_OK;
+}
+
+int msf_run_stem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const msf_detector* det,
+                 const msf_fem* fem, const double* x_p, uint32_t n_xp, const double* y_p,
+                 uint32_t n_yp, double* adf_image, double* fem_v)
+{
+    if (!c) return MSF_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
+    if (fem && fem->enabled && !fem_v) return fail(c, MSF_ERR_INVALID, "fem_v is NULL");
+    RunOut out;
+    out.adf = adf_image;
+    out.fem = fem_v;
+    return run_internal(c, sp, o, det, fem, x_p, n_xp, y_p, n_yp, out);
+}
+
+int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const double* x_p,
+                   uint32_t n_xp, const double* y_p, uint32_t n_yp, float* psi)
+{
+    if (!c || !psi) return MSF_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    RunOut out;
+    out.exit = psi;
+    return run_internal(c, sp, o, nullptr, nullptr, x_p, n_xp, y_p, n_yp, out);
+}
+
+} // extern "C"

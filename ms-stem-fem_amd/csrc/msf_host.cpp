@@ -1,0 +1,226 @@
+// msf_host.cpp -- host-side FP64 rules (see msf_host.hpp).
+#include "msf_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace msf {
+
+double ref_pow2(double x)
+{
+    volatile double two = 2.0;
+    return std::pow(x, (double)two);
+}
+
+// tem.cpp:424-438 (constants as written there)
+void wavelength(double VV, double& lambda, double& gamma)
+{
+    const double m_e0 = 9.109e-31;
+    const double cc = 299792458;
+    const double ee = 1.60217662e-19;
+    const double hh = 6.62607004e-34;
+    volatile double ten = 10.0;
+    const double e10 = std::pow((double)ten, (double)ten);
+    lambda = hh * cc * e10 / std::sqrt(ee * VV * (2.0 * m_e0 * cc * cc + ee * VV));
+    gamma = (hh * e10 / lambda) * (hh * e10 / lambda) * (m_e0 * cc * cc + ee * VV)
+            / ((2 * m_e0 * cc * cc + ee * VV) * ee * VV * m_e0);
+}
+
+Domain make_domain(int nx, int ny, double xperiod, double yperiod, double xmin, double ymin,
+                   double voltage)
+{
+    Domain d;
+    d.nx = nx;
+    d.ny = ny;
+    wavelength(voltage, d.lambda, d.gamma);
+    // tem.cpp:683-695
+    if (nx / (2 * xperiod) < ny / (2 * yperiod))
+        d.bw_pap = nx / (2 * xperiod);
+    else
+        d.bw_pap = ny / (2 * yperiod);
+    d.bw_t = (2.0 / 3.0) * d.bw_pap;
+    d.nxny = (double)((int64_t)nx * ny);
+    d.sqrt_nxny = (nx == ny) ? (double)nx : std::sqrt(d.nxny);
+    // tem.cpp:763-768 spacings; func.cpp:92-106 and :147-167 grids
+    const double dx = xperiod / nx, dy = yperiod / ny;
+    const double dkx = 1.0 / xperiod, dky = 1.0 / yperiod;
+    const double kxp = nx / xperiod, kyp = ny / yperiod;
+    d.kx.resize(nx);
+    d.ky.resize(ny);
+    d.xx.resize(nx);
+    d.yy.resize(ny);
+    for (int i = 0; i < nx / 2; i++) d.kx[i] = i * dkx;
+    for (int i = nx / 2; i < nx; i++) d.kx[i] = i * dkx - kxp;
+    for (int i = 0; i < ny / 2; i++) d.ky[i] = i * dky;
+    for (int i = ny / 2; i < ny; i++) d.ky[i] = i * dky - kyp;
+    for (int i = 0; i < nx; i++) d.xx[i] = xmin + (i * dx);
+    for (int i = 0; i < ny; i++) d.yy[i] = ymin + (i * dy);
+    return d;
+}
+
+RowRanges slice_keep_ranges(const Domain& d)
+{
+    const double c_bw = ref_pow2(d.bw_t);   // bw_limit.cpp:36
+    const double c_pr = d.bw_t * d.bw_t;     // slice.cpp:1375
+    return row_ranges(d.nx, d.ny, [&](int i, int j) {
+        const double k2 = ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]);
+        return !(k2 >= c_bw) && (k2 < c_pr);
+    });
+}
+
+RowRanges bwlimit_keep_ranges(const Domain& d)
+{
+    const double c_bw = ref_pow2(d.bw_t);
+    return row_ranges(d.nx, d.ny, [&](int i, int j) {
+        return !(ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]) >= c_bw);
+    });
+}
+
+RowRanges template_keep_ranges(const Domain& d)
+{
+    const double c2 = d.bw_pap * d.bw_pap; // scatterer_pap.hpp:82
+    return row_ranges(d.nx, d.ny, [&](int i, int j) {
+        return ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]) < c2;
+    });
+}
+
+RowRanges aperture_keep_ranges(const Domain& d, double alpha_max)
+{
+    const double l2 = d.lambda * d.lambda;      // adfstem.cpp:95
+    const double a2 = alpha_max * alpha_max;    // adfstem.cpp:96
+    return row_ranges(d.nx, d.ny, [&](int i, int j) {
+        const double ksqr = d.kx[i] * d.kx[i] + d.ky[j] * d.ky[j];
+        return l2 * ksqr < a2;
+    });
+}
+
+int64_t shift_index(const std::vector<double>& grid, double q, int64_t previous)
+{
+    const double half = 0.5 * (grid[1] - grid[0]);
+    for (size_t i = 0; i < grid.size(); ++i)
+        if (grid[i] - q >= half) return (int64_t)i - 1;
+    return previous;
+}
+
+Slices autoslice(const double* q, uint32_t n, double zmin, double zperiod, double min_thickness)
+{
+    Slices s;
+    if (n == 0) { // slice.cpp:632-639
+        s.lower.push_back(zmin);
+        s.thick.push_back(zperiod);
+        s.atoms.emplace_back();
+        return s;
+    }
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return q[3 * a + 2] < q[3 * b + 2]; });
+    // slice.cpp:657-682: an atom joins the current slice while
+    // z - z_first <= min_thickness; lower_bound accumulates member z
+    s.lower.push_back(0.0);
+    s.atoms.emplace_back();
+    double tmp_lower = q[3 * order[0] + 2];
+    for (uint32_t k = 0; k < n; ++k) {
+        const int32_t a = order[k];
+        const double z = q[3 * a + 2];
+        if (z - tmp_lower <= min_thickness) {
+            s.atoms.back().push_back(a);
+            s.lower.back() += z;
+        } else {
+            s.atoms.emplace_back(1, a);
+            s.lower.push_back(z);
+            tmp_lower = z;
+        }
+    }
+    const size_t ns = s.lower.size();
+    for (size_t i = 0; i < ns; ++i) s.lower[i] = s.lower[i] / s.atoms[i].size(); // :690-699
+    s.thick.resize(ns);
+    for (size_t i = 0; i < ns; ++i) { // :709-730
+        if (i + 1 == ns)
+            s.thick[i] = q[3 * s.atoms[i].back() + 2] + min_thickness - s.lower[i];
+        else
+            s.thick[i] = s.lower[i + 1] - s.lower[i];
+    }
+    return s;
+}
+
+void propagator(double lambda, double thickness, const std::vector<double>& k, double bw_t,
+                std::vector<double>& p)
+{
+    p.assign(2 * k.size(), 0.0);
+    for (size_t i = 0; i < k.size(); ++i) {
+        if (k[i] < bw_t && k[i] > -1.0 * bw_t) {
+            const double trig = kRefPi * lambda * thickness * ref_pow2(k[i]);
+            p[2 * i] = std::cos(trig);
+            p[2 * i + 1] = -std::sin(trig);
+        }
+    }
+}
+
+int fem_bin_count(const Domain& d, double dr, double& delta_k)
+{
+    delta_k = dr * std::sqrt((d.kx[1] - d.kx[0]) * (d.kx[1] - d.kx[0])
+                             + (d.ky[1] - d.ky[0]) * (d.ky[1] - d.ky[0]));
+    const unsigned int nb = (unsigned int)(d.bw_t / delta_k);
+    return (int)nb;
+}
+
+Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
+                    double k2, bool fem, double dr, double alpha_max)
+{
+    Groups g;
+    const int nx = d.nx, ny = d.ny;
+    const int A = keep.amax, B = keep.bmax;
+    auto kept = [&](int i, int j) {
+        return j < keep.ab[2 * i] || j >= ny - keep.ab[2 * i + 1];
+    };
+    auto compact = [&](int i, int j) -> int32_t {
+        const int jc = (j < A) ? j : j - ny + A + B;
+        return (int32_t)((int64_t)i * ncp + jc);
+    };
+    g.offsets.push_back(0);
+    // group 0: ADF ring, k_sqr = kx*kx + ky*ky in (k1, k2] (func.cpp:2630-2637)
+    if (adf) {
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                const double k_sqr = d.kx[i] * d.kx[i] + d.ky[j] * d.ky[j];
+                if (k_sqr > k1 && k_sqr <= k2 && kept(i, j)) g.pixels.push_back(compact(i, j));
+            }
+    }
+    g.offsets.push_back((int32_t)g.pixels.size());
+    g.kind.push_back(0);
+    if (fem) {
+        double dk = 0.0;
+        g.nbins = fem_bin_count(d, dr, dk);
+        // sort key of adfstem.cpp:1043-1062; ties keep row-major order
+        std::vector<std::pair<double, int64_t>> m;
+        m.reserve((size_t)nx * ny);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                m.emplace_back(d.kx[i] * d.kx[i] + d.ky[j] * d.ky[j], (int64_t)i * ny + j);
+        std::stable_sort(m.begin(), m.end(),
+                         [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+                             return a.first < b.first;
+                         });
+        size_t it = 0;
+        for (int ii = 0; ii < g.nbins; ++ii) {
+            const double lower = (unsigned int)ii * dk;
+            const double upper = (unsigned int)(ii + 1) * dk;
+            g.edges.push_back(lower);
+            int32_t count = 0;
+            for (; it < m.size() && m[it].first <= (upper * upper); ++it) {
+                const int i = (int)(m[it].second / ny), j = (int)(m[it].second % ny);
+                ++count;
+                if (kept(i, j)) g.pixels.push_back(compact(i, j));
+            }
+            g.counts.push_back(count);
+            g.zeroed.push_back(d.lambda * upper <= alpha_max ? 1 : 0);
+            g.offsets.push_back((int32_t)g.pixels.size());
+            g.kind.push_back(1);
+        }
+    }
+    return g;
+}
+
+} // namespace msf

@@ -1,0 +1,269 @@
+"""ctypes binding of libmsf.so (include/msf.h), the MI355X multislice
+ADF-STEM / FEM path.
+
+Mirrors the reference's call surface for this path: Specimen replaces the
+set-up of tem.cpp:944-1276 (scatterer_pap_LUT, assign_atoms_to_slices_z_auto,
+slice::update_propagator, slice::update_transmission_function), Context.
+adfstem() replaces TEM_NS::adfstem (adfstem.hpp:66-121) for the default ADF
+and V_omega modes.  Errors raise MsfError carrying the library's message, the
+way the reference prints to cerr and returns EXIT_FAILURE.
+
+There is no CPU fallback: loading fails loudly when libmsf.so is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmsf.so")
+
+TRANSMIT_REFERENCE, TRANSMIT_EXACT = 0, 1
+ADF_REFERENCE, ADF_PHYSICAL = 0, 1
+
+_ERRORS = {-1: "invalid argument", -2: "unsupported size", -3: "device error",
+           -4: "out of device memory", -5: "position outside the domain"}
+
+
+class MsfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"msf error {code} ({_ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32),
+                ("xperiod", C.c_double), ("yperiod", C.c_double), ("zperiod", C.c_double),
+                ("xmin", C.c_double), ("ymin", C.c_double), ("zmin", C.c_double),
+                ("voltage", C.c_double)]
+
+
+class Optics(C.Structure):
+    _fields_ = [("corrected", C.c_int32), ("cs3", C.c_double), ("cs5", C.c_double),
+                ("defocus", C.c_double), ("alpha_max", C.c_double)]
+
+
+class Detector(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("inner", C.c_double), ("outer", C.c_double),
+                ("threshold_mode", C.c_int32)]
+
+
+class Fem(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("dr", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
+                ("profile_kernels", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("setup_ms", C.c_double), ("slice_loop_ms", C.c_double),
+                ("total_ms", C.c_double), ("probe_slices", C.c_uint64),
+                ("batch", C.c_uint32), ("n_batches", C.c_uint32),
+                ("row_pass_us", C.c_double), ("col_pass_us", C.c_double),
+                ("first_row_pass_us", C.c_double), ("last_col_pass_us", C.c_double),
+                ("reduce_us", C.c_double), ("row_pass_count", C.c_uint64),
+                ("col_pass_count", C.c_uint64), ("row_pass_bytes", C.c_double),
+                ("col_pass_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf_get_stats",
+           "msf_supported_length", "msf_wavelength", "msf_scherzer_defocus",
+           "msf_scherzer_alphamax", "msf_specimen_build", "msf_specimen_free",
+           "msf_specimen_slices", "msf_specimen_slice_info", "msf_specimen_transmission",
+           "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves"]
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    """Load libmsf.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "(make -C ms-stem-fem_amd/csrc); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.msf_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.msf_destroy.argtypes = [vp]
+        L.msf_last_error.argtypes = [vp]
+        L.msf_last_error.restype = C.c_char_p
+        L.msf_set_config.argtypes = [vp, C.POINTER(Config)]
+        L.msf_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.msf_supported_length.argtypes = [C.c_uint32]
+        L.msf_wavelength.argtypes = [C.c_double, _dp, _dp]
+        L.msf_scherzer_defocus.argtypes = [C.c_int32, C.c_double, C.c_double]
+        L.msf_scherzer_defocus.restype = C.c_double
+        L.msf_scherzer_alphamax.argtypes = [C.c_int32, C.c_double, C.c_double]
+        L.msf_scherzer_alphamax.restype = C.c_double
+        L.msf_specimen_build.argtypes = [vp, C.POINTER(Grid), _dp, C.POINTER(C.c_uint32),
+                                         C.c_uint32, C.c_double, C.POINTER(vp)]
+        L.msf_specimen_free.argtypes = [vp]
+        L.msf_specimen_slices.argtypes = [vp]
+        L.msf_specimen_slices.restype = C.c_uint32
+        L.msf_specimen_slice_info.argtypes = [vp, _dp, _dp, C.POINTER(C.c_uint32)]
+        L.msf_specimen_transmission.argtypes = [vp, C.c_uint32, _dp]
+        L.msf_probe_realspace.argtypes = [vp, C.POINTER(Grid), C.POINTER(Optics), _dp]
+        L.msf_fem_bins.argtypes = [C.POINTER(Grid), C.c_double, C.POINTER(C.c_uint32), _dp]
+        L.msf_run_stem.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(Detector),
+                                   C.POINTER(Fem), _dp, C.c_uint32, _dp, C.c_uint32, _dp, _dp]
+        L.msf_exit_waves.argtypes = [vp, vp, C.POINTER(Optics), _dp, C.c_uint32, _dp,
+                                     C.c_uint32, C.POINTER(C.c_float)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def wavelength(voltage):
+    lam, gam = C.c_double(), C.c_double()
+    rc = lib().msf_wavelength(voltage, C.byref(lam), C.byref(gam))
+    if rc:
+        raise MsfError(rc, "msf_wavelength")
+    return lam.value, gam.value
+
+
+def scherzer_defocus(lam, cs, corrected=0):
+    return lib().msf_scherzer_defocus(corrected, lam, cs)
+
+
+def scherzer_alphamax(lam, cs, corrected=0):
+    return lib().msf_scherzer_alphamax(corrected, lam, cs)
+
+
+def supported_length(n):
+    return bool(lib().msf_supported_length(n))
+
+
+def make_grid(nx, ny, xperiod, yperiod, zperiod, voltage, xmin=0.0, ymin=0.0, zmin=0.0):
+    return Grid(nx, ny, xperiod, yperiod, zperiod, xmin, ymin, zmin, voltage)
+
+
+def fem_bins(grid, dr):
+    n = C.c_uint32()
+    rc = lib().msf_fem_bins(C.byref(grid), dr, C.byref(n), None)
+    if rc:
+        raise MsfError(rc, "msf_fem_bins")
+    edges = np.zeros(max(n.value, 1))
+    lib().msf_fem_bins(C.byref(grid), dr, C.byref(n), _p(edges))
+    return edges[:n.value]
+
+
+class Context:
+    """One device context (msf_create); one per host thread."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().msf_create(C.byref(self._h), device)
+        if rc:
+            raise MsfError(rc, f"msf_create(device={device})")
+
+    def close(self):
+        if self._h:
+            lib().msf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise MsfError(rc, f"{what}: {lib().msf_last_error(self._h).decode()}")
+
+    def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False):
+        cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0)
+        self._check(lib().msf_set_config(self._h, C.byref(cfg)), "msf_set_config")
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().msf_get_stats(self._h, C.byref(s)), "msf_get_stats")
+        return s.as_dict()
+
+    def specimen(self, grid, q, Z, min_slice):
+        return Specimen(self, grid, q, Z, min_slice)
+
+    def probe_realspace(self, grid, optics):
+        out = np.zeros((grid.nx, grid.ny), np.complex128)
+        self._check(lib().msf_probe_realspace(self._h, C.byref(grid), C.byref(optics),
+                                              _p(out.view(np.float64))), "msf_probe_realspace")
+        return out
+
+    def adfstem(self, spec, optics, x_p, y_p, detector=None, fem=None):
+        """Returns (adf image [n_xp, n_yp] or None, V_omega [n_bins] or None)."""
+        x_p = np.ascontiguousarray(x_p, np.float64)
+        y_p = np.ascontiguousarray(y_p, np.float64)
+        det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
+        fm = fem if fem is not None else Fem(0, 0.0)
+        img = np.zeros(x_p.size * y_p.size) if det.enabled else None
+        v = None
+        if fm.enabled:
+            v = np.zeros(max(fem_bins(spec.grid, fm.dr).size, 1))
+        self._check(lib().msf_run_stem(self._h, spec._h, C.byref(optics), C.byref(det),
+                                       C.byref(fm), _p(x_p), x_p.size, _p(y_p), y_p.size,
+                                       _p(img), _p(v)), "msf_run_stem")
+        if img is not None:
+            img = img.reshape(x_p.size, y_p.size)
+        if v is not None:
+            v = v[:fem_bins(spec.grid, fm.dr).size]
+        return img, v
+
+    def exit_waves(self, spec, optics, x_p, y_p):
+        x_p = np.ascontiguousarray(x_p, np.float64)
+        y_p = np.ascontiguousarray(y_p, np.float64)
+        out = np.zeros((x_p.size * y_p.size, spec.grid.nx, spec.grid.ny), np.complex64)
+        self._check(lib().msf_exit_waves(self._h, spec._h, C.byref(optics), _p(x_p), x_p.size,
+                                         _p(y_p), y_p.size, _p(out.view(np.float32), C.c_float)),
+                    "msf_exit_waves")
+        return out
+
+
+class Specimen:
+    """Device-resident slices of one specimen (msf_specimen_build)."""
+
+    def __init__(self, ctx, grid, q, Z, min_slice):
+        self.ctx = ctx
+        self.grid = grid
+        q = np.ascontiguousarray(q, np.float64).reshape(-1)
+        Z = np.ascontiguousarray(Z, np.uint32)
+        self._h = C.c_void_p()
+        ctx._check(lib().msf_specimen_build(ctx._h, C.byref(grid), _p(q), _p(Z, C.c_uint32),
+                                            Z.size, min_slice, C.byref(self._h)),
+                   "msf_specimen_build")
+        self.n_slices = lib().msf_specimen_slices(self._h)
+
+    def free(self):
+        if self._h:
+            lib().msf_specimen_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def slice_info(self):
+        n = self.n_slices
+        lb, th, na = np.zeros(n), np.zeros(n), np.zeros(n, np.uint32)
+        rc = lib().msf_specimen_slice_info(self._h, _p(lb), _p(th), _p(na, C.c_uint32))
+        if rc:
+            raise MsfError(rc, "msf_specimen_slice_info")
+        return lb, th, na
+
+    def transmission(self, s):
+        out = np.zeros((self.grid.nx, self.grid.ny), np.complex128)
+        rc = lib().msf_specimen_transmission(self._h, s, _p(out.view(np.float64)))
+        if rc:
+            raise MsfError(rc, "msf_specimen_transmission")
+        return out

@@ -1,0 +1,758 @@
+/*
+ * msf_oracle.c -- FP64 CPU restatement of ms-stem-fem's multislice
+ * ADF-STEM / FEM path (tem.cpp setup + adfstem()).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests use;
+ * the product library never links or calls it.
+ *
+ * Each function cites the reference lines (paths relative to the reference
+ * repository root) whose arithmetic it restates, in the same operation order
+ * and with the same comparison operators.  Where the reference writes
+ * pow(x, 2) it is compiled at -O0 (build/Makefile:37-45) and so calls libm
+ * pow(); ref_pow2() below does the same here (a plain x*x differs from glibc
+ * pow(x, 2.0) in ~1e-3 of cases, which moves band-limit boundary pixels).
+ *
+ * FFTW (not vendored; FFTW >= 3.3.6 per README:29) is replaced by an own
+ * mixed-radix FP64 DFT with table twiddles (orc_fft1d), pinned by the tests
+ * against numpy.fft and a brute-force O(n^2) DFT.
+ */
+#include "msf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define REF_PI 3.14159265369 /* slice.cpp:41, lenses.hpp:39, ... */
+
+static const double kirkland[103][12] = {
+#include "../ms-stem-fem_amd/csrc/kirkland_params.inc"
+};
+
+static double ref_pow2(double x)
+{
+    volatile double two = 2.0; /* keep the libm call, as in the -O0 reference build */
+    return pow(x, two);
+}
+
+/* ------------------------------------------------------------------ */
+/* scalars: tem.cpp:424-438 (lambda, gamma), tem.cpp:683-695 (cut-offs) */
+void orc_wavelength(double VV, double* lambda, double* gamma)
+{
+    const double m_e0 = 9.109e-31;
+    const double cc = 299792458;
+    const double ee = 1.60217662e-19;
+    const double hh = 6.62607004e-34;
+    volatile double ten = 10.0;
+    const double e10 = pow(ten, ten);
+    const double l = hh * cc * e10 / sqrt(ee * VV * (2.0 * m_e0 * cc * cc + ee * VV));
+    *lambda = l;
+    *gamma = (hh * e10 / l) * (hh * e10 / l) * (m_e0 * cc * cc + ee * VV)
+             / ((2 * m_e0 * cc * cc + ee * VV) * ee * VV * m_e0);
+}
+
+void orc_bwcutoffs(int Nx, int Ny, double xperiod, double yperiod,
+                   double* bwcutoff_pap, double* bwcutoff_t)
+{
+    double pap;
+    if (Nx / (2 * xperiod) < Ny / (2 * yperiod))
+        pap = Nx / (2 * xperiod);
+    else
+        pap = Ny / (2 * yperiod);
+    *bwcutoff_pap = pap;
+    *bwcutoff_t = (2.0 / 3.0) * pap;
+}
+
+/* func.cpp:92-106 (domain_2D_periodic), func.cpp:147-167
+ * (domain_2D_periodic_recip), spacings from tem.cpp:763-768. */
+void orc_domains(int Nx, int Ny, double xperiod, double yperiod,
+                 double xmin, double ymin,
+                 double* kx, double* ky, double* xx, double* yy)
+{
+    const double dx = xperiod / Nx, dy = yperiod / Ny;
+    const double dkx = 1.0 / xperiod, dky = 1.0 / yperiod;
+    const double kxperiod = Nx / xperiod, kyperiod = Ny / yperiod;
+    for (int i = 0; i < Nx / 2; i++) kx[i] = i * dkx;
+    for (int i = Nx / 2; i < Nx; i++) kx[i] = i * dkx - kxperiod;
+    for (int i = 0; i < Ny / 2; i++) ky[i] = i * dky;
+    for (int i = Ny / 2; i < Ny; i++) ky[i] = i * dky - kyperiod;
+    for (int i = 0; i < Nx; i++) xx[i] = xmin + (i * dx);
+    for (int i = 0; i < Ny; i++) yy[i] = ymin + (i * dy);
+}
+
+/* scherzer.cpp:61-79 and :131-168 */
+double orc_scherzer_defocus_uncorrected(double lambda, double Cs3)
+{
+    return 0.87 * sqrt(Cs3 * lambda);
+}
+double orc_scherzer_alphamax_uncorrected(double lambda, double Cs3)
+{
+    return 1.34 * pow(lambda / Cs3, 0.25);
+}
+double orc_scherzer_defocus_correctedtoCs5(double lambda, double Cs5)
+{
+    return -0.983 * pow(ref_pow2(lambda) * Cs5, 1.0 / 3.0);
+}
+double orc_scherzer_alphamax_correctedtoCs5(double lambda, double Cs5)
+{
+    return 1.513 * pow(lambda / Cs5, 1.0 / 6.0);
+}
+
+/* func.cpp:108-127 (assign_stem_raster_points) with the point count of
+ * tem.cpp:1357-1362 supplied by the caller. */
+void orc_raster(int n, double period, double qmin, double* q)
+{
+    const double d = period / n;
+    for (int i = 0; i < n; i++) q[i] = qmin + (i * d);
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix decimation in time.  Sub-transform of length n
+ * uses roots W_n^k = tw[k * (N/n)] from one table of the top length N. */
+typedef struct { double re, im; } cplx;
+
+static void fft_rec(const cplx* in, size_t istride, cplx* out, int n,
+                    const int* factors, const cplx* tw, int twstep, int N)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    const int p = factors[0];
+    const int m = n / p;
+    for (int r = 0; r < p; r++)
+        fft_rec(in + (size_t)r * istride, istride * p, out + (size_t)r * m, m,
+                factors + 1, tw, twstep * p, N);
+    cplx t[64];
+    cplx x[64];
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++) {
+            const cplx y = out[r * m + k];
+            const cplx w = tw[((long)r * k % n) * twstep];
+            t[r].re = y.re * w.re - y.im * w.im;
+            t[r].im = y.re * w.im + y.im * w.re;
+        }
+        for (int q = 0; q < p; q++) {
+            double sr = 0.0, si = 0.0;
+            for (int r = 0; r < p; r++) {
+                const cplx w = tw[((long)r * q % p) * (long)m * twstep];
+                sr += t[r].re * w.re - t[r].im * w.im;
+                si += t[r].re * w.im + t[r].im * w.re;
+            }
+            x[q].re = sr;
+            x[q].im = si;
+        }
+        for (int q = 0; q < p; q++) out[q * m + k] = x[q];
+    }
+    (void)N;
+}
+
+static int factorize(int n, int* f)
+{
+    int c = 0;
+    while (n > 1) {
+        int p = 2;
+        while (p * p <= n && n % p) p++;
+        if (n % p) p = n;
+        if (p > 64) return -1;
+        f[c++] = p;
+        n /= p;
+    }
+    return c;
+}
+
+static void make_twiddles(int n, int sign, cplx* tw)
+{
+    /* exact-pi roots; the reference's REF_PI applies only to its own physics */
+    for (int k = 0; k < n; k++) {
+        const double a = 2.0 * M_PI * (double)k / (double)n;
+        tw[k].re = cos(a);
+        tw[k].im = sign * sin(a);
+    }
+}
+
+int orc_fft1d(double* data, int n, int sign)
+{
+    int f[64];
+    if (n < 1 || factorize(n, f) < 0) return -1;
+    cplx* tw = (cplx*)malloc(sizeof(cplx) * n);
+    cplx* tmp = (cplx*)malloc(sizeof(cplx) * n);
+    make_twiddles(n, sign, tw);
+    memcpy(tmp, data, sizeof(cplx) * n);
+    fft_rec(tmp, 1, (cplx*)data, n, f, tw, 1, n);
+    free(tw);
+    free(tmp);
+    return 0;
+}
+
+int orc_fft2d(double* data, int nx, int ny, int sign)
+{
+    int fx[64], fy[64];
+    if (factorize(nx, fx) < 0 || factorize(ny, fy) < 0) return -1;
+    cplx* a = (cplx*)data;
+    cplx* twx = (cplx*)malloc(sizeof(cplx) * nx);
+    cplx* twy = (cplx*)malloc(sizeof(cplx) * ny);
+    const int nmax = nx > ny ? nx : ny;
+    cplx* t0 = (cplx*)malloc(sizeof(cplx) * nmax);
+    cplx* t1 = (cplx*)malloc(sizeof(cplx) * nmax);
+    make_twiddles(nx, sign, twx);
+    make_twiddles(ny, sign, twy);
+    for (int i = 0; i < nx; i++) { /* rows: transform along y */
+        memcpy(t0, a + (size_t)i * ny, sizeof(cplx) * ny);
+        fft_rec(t0, 1, a + (size_t)i * ny, ny, fy, twy, 1, ny);
+    }
+    for (int j = 0; j < ny; j++) { /* columns: transform along x */
+        for (int i = 0; i < nx; i++) t0[i] = a[(size_t)i * ny + j];
+        fft_rec(t0, 1, t1, nx, fx, twx, 1, nx);
+        for (int i = 0; i < nx; i++) a[(size_t)i * ny + j] = t1[i];
+    }
+    free(twx); free(twy); free(t0); free(t1);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* projected_atomic_potential.cpp:209-225 (f_ej_single_scatterer); parameter
+ * rows as assigned in scatterer_param_LUT.hpp:117-1560. */
+double orc_fe(int Z, double ksq)
+{
+    if (Z < 1 || Z > 103) return NAN;
+    const double* p = kirkland[Z - 1];
+    const double *a = p, *b = p + 3, *c = p + 6, *d = p + 9;
+    return a[0] / (ksq + b[0])
+         + a[1] / (ksq + b[1])
+         + a[2] / (ksq + b[2])
+         + c[0] * exp(-d[0] * ksq)
+         + c[1] * exp(-d[1] * ksq)
+         + c[2] * exp(-d[2] * ksq);
+}
+
+/* scatterer_pap.hpp:57-231 (constructor) with
+ * projected_atomic_potential.cpp:169-207 (sigmaVz_single_scatterer):
+ * k-space template inside |k|^2 < cutoff^2, BACKWARD FFT, / sqrt(Nx*Ny). */
+int orc_template(int Z, double lambda, double gamma, double ab_inv, double cutoff,
+                 const double* kx, int Nx, const double* ky, int Ny,
+                 double x, double y, double* out)
+{
+    const double cutoffsq = cutoff * cutoff;
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            double re = 0.0, im = 0.0;
+            if (ref_pow2(kx[i]) + ref_pow2(ky[j]) < cutoffsq) {
+                const double ksq = (kx[i] * kx[i]) + (ky[j] * ky[j]);
+                const double f = orc_fe(Z, ksq);
+                const double trig = 2 * REF_PI * (kx[i] * x + ky[j] * y);
+                re += lambda * gamma * ab_inv * f * cos(trig);
+                im += lambda * gamma * ab_inv * f * sin(trig);
+            }
+            out[2 * ((size_t)j + (size_t)i * Ny)] = re;
+            out[2 * ((size_t)j + (size_t)i * Ny) + 1] = im;
+        }
+    if (orc_fft2d(out, Nx, Ny, +1)) return -1;
+    const double s = sqrt((double)((long)Nx * Ny));
+    for (size_t k = 0; k < 2 * (size_t)Nx * Ny; k++) out[k] = out[k] / s;
+    return 0;
+}
+
+/* slice.cpp:469-740 (assign_atoms_to_slices_z_auto).  Ties in z are ordered
+ * by atom index (the reference's std::sort leaves them unordered; membership
+ * does not depend on it). */
+typedef struct { double z; int idx; } zkey;
+static int zkey_cmp(const void* a, const void* b)
+{
+    const zkey* x = (const zkey*)a;
+    const zkey* y = (const zkey*)b;
+    if (x->z < y->z) return -1;
+    if (x->z > y->z) return 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+int orc_autoslice(const double* q, int n, double zmin, double zperiod,
+                  double min_thickness, int32_t* slice_of_atom,
+                  double* lower_bound, double* thickness, int32_t* n_slices)
+{
+    if (n == 0) { /* slice.cpp:632-639 */
+        lower_bound[0] = zmin;
+        thickness[0] = zperiod;
+        *n_slices = 1;
+        return 0;
+    }
+    zkey* k = (zkey*)malloc(sizeof(zkey) * n);
+    for (int a = 0; a < n; a++) { k[a].z = q[3 * a + 2]; k[a].idx = a; }
+    qsort(k, n, sizeof(zkey), zkey_cmp);
+    int s = 0;
+    int* count = (int*)calloc(n, sizeof(int));
+    double* top = (double*)malloc(sizeof(double) * n);
+    lower_bound[0] = 0.0;
+    double tmp_lower = k[0].z;
+    for (int a = 0; a < n; a++) {
+        if (k[a].z - tmp_lower <= min_thickness) {
+            lower_bound[s] += k[a].z;
+        } else {
+            s++;
+            lower_bound[s] = k[a].z;
+            tmp_lower = k[a].z;
+        }
+        count[s]++;
+        top[s] = k[a].z; /* scatterers->back() */
+        slice_of_atom[k[a].idx] = s;
+    }
+    const int ns = s + 1;
+    for (int i = 0; i < ns; i++) lower_bound[i] = lower_bound[i] / count[i];
+    for (int i = 0; i < ns; i++) {
+        if (i + 1 == ns)
+            thickness[i] = top[i] + min_thickness - lower_bound[i];
+        else
+            thickness[i] = lower_bound[i + 1] - lower_bound[i];
+    }
+    *n_slices = ns;
+    free(k); free(count); free(top);
+    return 0;
+}
+
+/* slice.cpp:1293-1363 (update_propagator), one axis. */
+void orc_propagator(double lambda, double thickness, const double* k, int n,
+                    double bwcutoff_t, double* p)
+{
+    for (int i = 0; i < n; i++) {
+        if (k[i] < bwcutoff_t && k[i] > -1.0 * bwcutoff_t) {
+            const double trig = REF_PI * lambda * thickness * ref_pow2(k[i]);
+            p[2 * i] = cos(trig);
+            p[2 * i + 1] = -sin(trig);
+        } else {
+            p[2 * i] = 0.0;
+            p[2 * i + 1] = 0.0;
+        }
+    }
+}
+
+/* Nearest-pixel rule of slice.cpp:1554-1572 and adfstem.cpp:1590-1606: the
+ * first i with grid[i] - q >= half spacing, minus one; `previous` when no i
+ * qualifies.  Returns -1 for i == 0 (size_t underflow in the reference). */
+int orc_shift_index(const double* grid, int n, double q, int64_t previous)
+{
+    const double half = 0.5 * (grid[1] - grid[0]);
+    for (int i = 0; i < n; i++)
+        if (grid[i] - q >= half) return i - 1;
+    return (int)previous;
+}
+
+/* bw_limit.cpp:30-47 */
+static void bw_limit(double* a, const double* kx, int Nx, const double* ky, int Ny,
+                     double cutoff)
+{
+    const double cutoffsq = ref_pow2(cutoff);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++)
+            if (ref_pow2(kx[i]) + ref_pow2(ky[j]) >= cutoffsq) {
+                a[2 * ((size_t)j + (size_t)i * Ny)] = 0.0;
+                a[2 * ((size_t)j + (size_t)i * Ny) + 1] = 0.0;
+            }
+}
+
+/* slice.cpp:1451-1757 (update_transmission_function): shift-sum of species
+ * templates over the slice's atoms, exp(i sigma v), FFT, bw_limit, IFFT,
+ * / NxNy.  atoms[] lists the slice's atoms in scatterer-vector order. */
+int orc_transmission(const double* q, const int32_t* Z, const int32_t* atoms, int n_atoms,
+                     const int32_t* zlist, int nz, const double* templates,
+                     const double* kx, const double* ky, const double* xx, const double* yy,
+                     int Nx, int Ny, double bwcutoff_t, double* t)
+{
+    const size_t M = (size_t)Nx * Ny;
+    memset(t, 0, sizeof(double) * 2 * M);
+    for (int a = 0; a < n_atoms; a++) {
+        const int at = atoms[a];
+        int s = -1;
+        for (int z = 0; z < nz; z++)
+            if (zlist[z] == Z[at]) { s = z; break; }
+        if (s < 0) return -1;
+        const double* tmpl = templates + 2 * M * s;
+        const int sx = orc_shift_index(xx, Nx, q[3 * at], 0);
+        const int sy = orc_shift_index(yy, Ny, q[3 * at + 1], 0);
+        if (sx < 0 || sy < 0) return -2;
+        for (int i = 0; i < Nx; i++) {
+            const int ix = (i < sx) ? i + Nx - sx : i - sx;
+            for (int j = 0; j < Ny; j++) {
+                const int iy = (j < sy) ? j + Ny - sy : j - sy;
+                const size_t src = (size_t)iy + (size_t)ix * Ny;
+                const size_t dst = (size_t)j + (size_t)i * Ny;
+                t[2 * dst] += tmpl[2 * src];
+                t[2 * dst + 1] += tmpl[2 * src + 1];
+            }
+        }
+    }
+    for (size_t k = 0; k < M; k++) {
+        const double e = exp(-(t[2 * k + 1]));
+        const double v = t[2 * k];
+        t[2 * k] = e * cos(v);
+        t[2 * k + 1] = e * sin(v);
+    }
+    if (orc_fft2d(t, Nx, Ny, -1)) return -1;
+    bw_limit(t, kx, Nx, ky, Ny, bwcutoff_t);
+    if (orc_fft2d(t, Nx, Ny, +1)) return -1;
+    const double NxNy = (double)Nx * Ny;
+    for (size_t k = 0; k < 2 * M; k++) t[k] = t[k] / NxNy;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* lenses.cpp:249-261 and :263-290 */
+static double chi_uncorrected(double ksqr, double Cs3, double defocus,
+                              double lambda, double lambdasqr)
+{
+    return REF_PI * lambda * ksqr * (0.5 * Cs3 * lambdasqr * ksqr - defocus);
+}
+static double chi_correctedtoCs5(double ksqr, double Cs3, double Cs5, double defocus,
+                                 double lambda, double lambdasqr)
+{
+    const double kl = ksqr * lambdasqr;
+    return REF_PI * lambda * ksqr
+           * (-defocus + 0.5 * Cs3 * kl + (1.0 / 3.0) * Cs5 * ref_pow2(kl));
+}
+
+/* adfstem.cpp:586-817: probe_wavefunction_{uncorrected,correctedtoCs5}_
+ * unnormalized (probe_wavefunction.cpp:37-239) at (xx[0], yy[0]), BACKWARD
+ * FFT, / sqrtNxNy (adfstem.cpp:210-212), probe_wavefunction_norm
+ * (probe_wavefunction.cpp:287-322) over the whole grid (one rank), x A_p. */
+int orc_probe(const orc_job* job, const double* kx, const double* ky,
+              const double* xx, const double* yy, double* probe)
+{
+    const int Nx = job->nx, Ny = job->ny;
+    double lambda, gamma;
+    orc_wavelength(job->voltage, &lambda, &gamma);
+    const double lambda_sqr = lambda * lambda;
+    const double alpha_max_sqr = job->alpha_max * job->alpha_max;
+    const double x_p = xx[0], y_p = yy[0];
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const double ksqr = kx[i] * kx[i] + ky[j] * ky[j];
+            double re = 0.0, im = 0.0;
+            if (lambda_sqr * ksqr < alpha_max_sqr) {
+                const double chi = job->corrected
+                    ? chi_correctedtoCs5(ksqr, job->cs3, job->cs5, job->defocus, lambda, lambda_sqr)
+                    : chi_uncorrected(ksqr, job->cs3, job->defocus, lambda, lambda_sqr);
+                const double trig = chi + 2 * REF_PI * (((kx[i]) * x_p) + ((ky[j]) * y_p));
+                re = cos(trig);
+                im = -1.0 * sin(trig);
+            }
+            probe[2 * ((size_t)j + (size_t)i * Ny)] = re;
+            probe[2 * ((size_t)j + (size_t)i * Ny) + 1] = im;
+        }
+    if (orc_fft2d(probe, Nx, Ny, +1)) return -1;
+    const double NxNy = (double)Nx * Ny;
+    const double s = (Nx == Ny) ? (double)Nx : sqrt(NxNy);
+    const size_t M = (size_t)Nx * Ny;
+    for (size_t k = 0; k < 2 * M; k++) probe[k] = probe[k] / s;
+    double ap = 0.0;
+    for (size_t k = 0; k < M; k++)
+        ap += probe[2 * k] * probe[2 * k] + probe[2 * k + 1] * probe[2 * k + 1];
+    ap = 1.0 / sqrt(ap);
+    for (size_t k = 0; k < 2 * M; k++) probe[k] = ap * probe[k];
+    return 0;
+}
+
+/* adfstem.cpp:1460-1606: probe shift indices in raster order (x outer, y
+ * inner), persisting from the previous position when no index qualifies. */
+void orc_probe_shifts(const double* xx, int nx, const double* yy, int ny,
+                      const double* x_p, int nxp, const double* y_p, int nyp,
+                      int64_t* sx, int64_t* sy)
+{
+    int64_t px = 0, py = 0;
+    for (int a = 0; a < nxp; a++)
+        for (int b = 0; b < nyp; b++) {
+            px = orc_shift_index(xx, nx, x_p[a], px);
+            py = orc_shift_index(yy, ny, y_p[b], py);
+            sx[(size_t)a * nyp + b] = px;
+            sy[(size_t)a * nyp + b] = py;
+        }
+}
+
+/* adfstem.cpp:1632-1671 (circular shift of the real-space probe),
+ * :1831-1837 (FORWARD FFT, / sqrtNxNy), and the slice loop :1866-2132:
+ * BACKWARD FFT; t-multiply with the in-place update of :1922-1952 (the
+ * imaginary part reads the already-updated real part); FORWARD FFT;
+ * / sqrtNxNy; bw_limit (bw_limit.cpp:30-47); propagate (slice.cpp:1366-1421).
+ * transmit_mode 1 replaces the in-place update by the exact complex product. */
+int orc_propagate_position(const double* probe, int64_t sx, int64_t sy,
+                           int n_slices, const double* t_stack,
+                           const double* px, const double* py,
+                           const double* kx, const double* ky, int Nx, int Ny,
+                           double bwcutoff_t, int transmit_mode, double* psi)
+{
+    const size_t M = (size_t)Nx * Ny;
+    const double NxNy = (double)Nx * Ny;
+    const double s = (Nx == Ny) ? (double)Nx : sqrt(NxNy);
+    if (sx < 0 || sy < 0 || sx >= Nx || sy >= Ny) return -2;
+    for (int i = 0; i < Nx; i++) {
+        const int64_t ix = (i < sx) ? i + Nx - sx : i - sx;
+        for (int j = 0; j < Ny; j++) {
+            const int64_t iy = (j < sy) ? j + Ny - sy : j - sy;
+            psi[2 * ((size_t)j + (size_t)i * Ny)] = probe[2 * ((size_t)iy + (size_t)ix * Ny)];
+            psi[2 * ((size_t)j + (size_t)i * Ny) + 1] = probe[2 * ((size_t)iy + (size_t)ix * Ny) + 1];
+        }
+    }
+    if (orc_fft2d(psi, Nx, Ny, -1)) return -1;
+    for (size_t k = 0; k < 2 * M; k++) psi[k] = psi[k] / s;
+    const double bw_sqr = bwcutoff_t * bwcutoff_t;
+    for (int n = 0; n < n_slices; n++) {
+        const double* t = t_stack + 2 * M * n;
+        const double* pxn = px + 2 * (size_t)Nx * n;
+        const double* pyn = py + 2 * (size_t)Ny * n;
+        if (orc_fft2d(psi, Nx, Ny, +1)) return -1;
+        for (size_t k = 0; k < M; k++) {
+            const double tr = t[2 * k], ti = t[2 * k + 1];
+            if (transmit_mode == 0) {
+                psi[2 * k] = (tr * psi[2 * k] - ti * psi[2 * k + 1]) / s;
+                psi[2 * k + 1] = (tr * psi[2 * k + 1] + ti * psi[2 * k]) / s;
+            } else {
+                const double ur = psi[2 * k], ui = psi[2 * k + 1];
+                psi[2 * k] = (tr * ur - ti * ui) / s;
+                psi[2 * k + 1] = (tr * ui + ti * ur) / s;
+            }
+        }
+        if (orc_fft2d(psi, Nx, Ny, -1)) return -1;
+        for (size_t k = 0; k < 2 * M; k++) psi[k] = psi[k] / s;
+        bw_limit(psi, kx, Nx, ky, Ny, bwcutoff_t);
+        for (int i = 0; i < Nx; i++)
+            for (int j = 0; j < Ny; j++) {
+                const size_t k = (size_t)j + (size_t)i * Ny;
+                const double k_sqr = ref_pow2(kx[i]) + ref_pow2(ky[j]);
+                if (k_sqr < bw_sqr) {
+                    const double pr = pxn[2 * i] * pyn[2 * j] - pxn[2 * i + 1] * pyn[2 * j + 1];
+                    const double pi = pxn[2 * i] * pyn[2 * j + 1] + pxn[2 * i + 1] * pyn[2 * j];
+                    const double fr = psi[2 * k], fi = psi[2 * k + 1];
+                    psi[2 * k] = (pr * fr - pi * fi);
+                    psi[2 * k + 1] = (pr * fi + pi * fr);
+                } else {
+                    psi[2 * k] = 0.0;
+                    psi[2 * k + 1] = 0.0;
+                }
+            }
+    }
+    return 0;
+}
+
+/* func.cpp:2593-2641 (sum_detected_intensity): (k1, k2] against k^2. */
+double orc_adf(const double* psi, const double* kx, const double* ky,
+               int Nx, int Ny, double k1, double k2)
+{
+    double sum = 0.0;
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const double k_sqr = kx[i] * kx[i] + ky[j] * ky[j];
+            if (k_sqr > k1 && k_sqr <= k2) {
+                const size_t k = (size_t)j + (size_t)i * Ny;
+                sum += psi[2 * k] * psi[2 * k] + psi[2 * k + 1] * psi[2 * k + 1];
+            }
+        }
+    return sum;
+}
+
+/* adfstem.cpp:947-972: delta_k and the number of k bins. */
+int orc_fem_nbins(const double* kx, const double* ky, double dr, double bwcutoff_t,
+                  double* delta_k)
+{
+    const double dk = dr * sqrt((kx[1] - kx[0]) * (kx[1] - kx[0])
+                                + (ky[1] - ky[0]) * (ky[1] - ky[0]));
+    *delta_k = dk;
+    const unsigned int nb = (unsigned int)(bwcutoff_t / dk);
+    return (int)nb;
+}
+
+/* adfstem.cpp:1043-1062 (sorted indexed magnitudes), variance.cpp:64-300
+ * (integrate_out_phi_fftw: sum of |psi| per ring, counts) and the direct-beam
+ * zeroing of adfstem.cpp:2518-2522. */
+typedef struct { double v; long idx; } kmag;
+static int kmag_cmp(const void* a, const void* b)
+{
+    const kmag* x = (const kmag*)a;
+    const kmag* y = (const kmag*)b;
+    if (x->v < y->v) return -1;
+    if (x->v > y->v) return 1;
+    return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+int orc_fem_radial(const double* psi, const double* kx, const double* ky, int Nx, int Ny,
+                   double delta_k, int nbins, double lambda, double alpha_max,
+                   double* radial, int32_t* counts)
+{
+    const size_t M = (size_t)Nx * Ny;
+    kmag* m = (kmag*)malloc(sizeof(kmag) * M);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const size_t k = (size_t)j + (size_t)i * Ny;
+            m[k].v = kx[i] * kx[i] + ky[j] * ky[j];
+            m[k].idx = (long)k;
+        }
+    qsort(m, M, sizeof(kmag), kmag_cmp);
+    size_t it = 0;
+    for (int ii = 0; ii < nbins; ii++) {
+        radial[ii] = 0.0;
+        counts[ii] = 0;
+        const double upper = (unsigned int)(ii + 1) * delta_k;
+        for (; it < M && m[it].v <= (upper * upper); ++it) {
+            const double re = psi[2 * m[it].idx], im = psi[2 * m[it].idx + 1];
+            radial[ii] += sqrt((re * re) + (im * im));
+            counts[ii] += 1;
+        }
+    }
+    for (int ii = 0; ii < nbins; ii++) {
+        const double upper = (unsigned int)(ii + 1) * delta_k;
+        if (lambda * upper <= alpha_max) radial[ii] = 0.0;
+    }
+    free(m);
+    return 0;
+}
+
+/* adfstem.cpp:2822-2866 (ftem_d1) */
+void orc_fem_finalize(const double* sum_r, const double* sum_r2, int nbins,
+                      int64_t n_positions, double* v)
+{
+    for (int i = 0; i < nbins; i++) {
+        if (sum_r[i] == 0.0)
+            v[i] = 0.0;
+        else
+            v[i] = ((double)n_positions * sum_r2[i] / (sum_r[i] * sum_r[i])) - 1.0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* tem.cpp:944-1276: unique species, templates (scatterer_pap_LUT),
+ * auto-slicing, per-slice propagator and transmission function. */
+int orc_specimen_build(const orc_job* job, const double* q, const int32_t* Z, int n_atoms,
+                       orc_specimen* out)
+{
+    const int Nx = job->nx, Ny = job->ny;
+    const size_t M = (size_t)Nx * Ny;
+    double lambda, gamma, pap, bwt;
+    orc_wavelength(job->voltage, &lambda, &gamma);
+    orc_bwcutoffs(Nx, Ny, job->xperiod, job->yperiod, &pap, &bwt);
+    double* kx = (double*)malloc(sizeof(double) * Nx);
+    double* ky = (double*)malloc(sizeof(double) * Ny);
+    double* xx = (double*)malloc(sizeof(double) * Nx);
+    double* yy = (double*)malloc(sizeof(double) * Ny);
+    orc_domains(Nx, Ny, job->xperiod, job->yperiod, job->xmin, job->ymin, kx, ky, xx, yy);
+
+    int32_t zlist[103];
+    int nz = 0;
+    for (int z = 1; z <= 103; z++)
+        for (int a = 0; a < n_atoms; a++)
+            if (Z[a] == z) { zlist[nz++] = z; break; }
+    double* templates = (double*)malloc(sizeof(double) * 2 * M * (nz ? nz : 1));
+    const double ab_inv = 1 / (job->xperiod * job->yperiod);
+    for (int z = 0; z < nz; z++)
+        orc_template(zlist[z], lambda, gamma, ab_inv, pap, kx, Nx, ky, Ny,
+                     job->xmin, job->ymin, templates + 2 * M * z);
+
+    int32_t* slice_of = (int32_t*)malloc(sizeof(int32_t) * (n_atoms ? n_atoms : 1));
+    double* lb = (double*)malloc(sizeof(double) * (n_atoms ? n_atoms : 1));
+    double* th = (double*)malloc(sizeof(double) * (n_atoms ? n_atoms : 1));
+    int32_t ns = 0;
+    orc_autoslice(q, n_atoms, job->zmin, job->zperiod, job->min_slice, slice_of, lb, th, &ns);
+
+    /* atoms of each slice in z-sorted order (the slice's scatterer vector) */
+    zkey* k = (zkey*)malloc(sizeof(zkey) * (n_atoms ? n_atoms : 1));
+    for (int a = 0; a < n_atoms; a++) { k[a].z = q[3 * a + 2]; k[a].idx = a; }
+    if (n_atoms) qsort(k, n_atoms, sizeof(zkey), zkey_cmp);
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (n_atoms ? n_atoms : 1));
+    for (int a = 0; a < n_atoms; a++) order[a] = k[a].idx;
+
+    out->n_slices = ns;
+    out->t_stack = (double*)malloc(sizeof(double) * 2 * M * ns);
+    out->px = (double*)malloc(sizeof(double) * 2 * (size_t)Nx * ns);
+    out->py = (double*)malloc(sizeof(double) * 2 * (size_t)Ny * ns);
+    out->lower_bound = (double*)malloc(sizeof(double) * ns);
+    out->thickness = (double*)malloc(sizeof(double) * ns);
+    int rc = 0;
+    int first = 0;
+    for (int s = 0; s < ns; s++) {
+        int cnt = 0;
+        while (first + cnt < n_atoms && slice_of[order[first + cnt]] == s) cnt++;
+        out->lower_bound[s] = lb[s];
+        out->thickness[s] = th[s];
+        orc_propagator(lambda, th[s], kx, Nx, bwt, out->px + 2 * (size_t)Nx * s);
+        orc_propagator(lambda, th[s], ky, Ny, bwt, out->py + 2 * (size_t)Ny * s);
+        rc |= orc_transmission(q, Z, order + first, cnt, zlist, nz, templates,
+                               kx, ky, xx, yy, Nx, Ny, bwt, out->t_stack + 2 * M * s);
+        first += cnt;
+    }
+    free(kx); free(ky); free(xx); free(yy); free(templates);
+    free(slice_of); free(lb); free(th); free(k); free(order);
+    return rc;
+}
+
+void orc_specimen_free(orc_specimen* s)
+{
+    free(s->t_stack); free(s->px); free(s->py); free(s->lower_bound); free(s->thickness);
+    memset(s, 0, sizeof(*s));
+}
+
+/* adfstem.cpp:1556-2793 (raster loop: ADF :2245-2267, FEM d1 accumulate
+ * :2495-2601) and :2822-2866 (V_omega).  Positions run in parallel over
+ * n_threads; the FEM sums are then taken in raster order. */
+int orc_adfstem(const orc_job* job, const orc_specimen* spec,
+                const double* x_p, int nxp, const double* y_p, int nyp,
+                int n_threads, double* adf_image, double* fem_v, double* exit_psi)
+{
+    const int Nx = job->nx, Ny = job->ny;
+    const size_t M = (size_t)Nx * Ny;
+    const long P = (long)nxp * nyp;
+    double lambda, gamma, pap, bwt;
+    orc_wavelength(job->voltage, &lambda, &gamma);
+    orc_bwcutoffs(Nx, Ny, job->xperiod, job->yperiod, &pap, &bwt);
+    double* kx = (double*)malloc(sizeof(double) * Nx);
+    double* ky = (double*)malloc(sizeof(double) * Ny);
+    double* xx = (double*)malloc(sizeof(double) * Nx);
+    double* yy = (double*)malloc(sizeof(double) * Ny);
+    orc_domains(Nx, Ny, job->xperiod, job->yperiod, job->xmin, job->ymin, kx, ky, xx, yy);
+    double* probe = (double*)malloc(sizeof(double) * 2 * M);
+    orc_probe(job, kx, ky, xx, yy, probe);
+    int64_t* sx = (int64_t*)malloc(sizeof(int64_t) * P);
+    int64_t* sy = (int64_t*)malloc(sizeof(int64_t) * P);
+    orc_probe_shifts(xx, Nx, yy, Ny, x_p, nxp, y_p, nyp, sx, sy);
+
+    double delta_k = 0.0;
+    int nbins = 0;
+    if (job->fem) nbins = orc_fem_nbins(kx, ky, job->dr, bwt, &delta_k);
+    double* radial = (double*)malloc(sizeof(double) * (size_t)P * (nbins ? nbins : 1));
+    int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)P * (nbins ? nbins : 1));
+    double k1 = job->det_inner / lambda, k2 = job->det_outer / lambda;
+    if (job->adf_mode == 1) { k1 = k1 * k1; k2 = k2 * k2; }
+    int rc = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel
+#endif
+    {
+        double* psi = (double*)malloc(sizeof(double) * 2 * M);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (long p = 0; p < P; p++) {
+            int r = orc_propagate_position(probe, sx[p], sy[p], spec->n_slices, spec->t_stack,
+                                           spec->px, spec->py, kx, ky, Nx, Ny, bwt,
+                                           job->transmit_mode, psi);
+            if (r) rc = r;
+            if (job->adf && adf_image) adf_image[p] = orc_adf(psi, kx, ky, Nx, Ny, k1, k2);
+            if (job->fem)
+                orc_fem_radial(psi, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max,
+                               radial + (size_t)p * nbins, counts + (size_t)p * nbins);
+            if (exit_psi) memcpy(exit_psi + 2 * M * p, psi, sizeof(double) * 2 * M);
+        }
+        free(psi);
+    }
+    if (job->fem && fem_v) {
+        double* s1 = (double*)calloc(nbins ? nbins : 1, sizeof(double));
+        double* s2 = (double*)calloc(nbins ? nbins : 1, sizeof(double));
+        for (long p = 0; p < P; p++)
+            for (int ii = 0; ii < nbins; ii++) {
+                const double r = radial[(size_t)p * nbins + ii];
+                const int c = counts[(size_t)p * nbins + ii];
+                s2[ii] += (r * r) / (c * c);
+                s1[ii] += r / c;
+            }
+        orc_fem_finalize(s1, s2, nbins, P, fem_v);
+        free(s1); free(s2);
+    }
+    free(kx); free(ky); free(xx); free(yy); free(probe); free(sx); free(sy);
+    free(radial); free(counts);
+    return rc;
+}
