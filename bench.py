@@ -1,0 +1,190 @@
+"""Benchmark: probe-slice propagations/s of the multislice FEM job at 512^2.
+
+Workload (one "step" = one whole job on resident inputs): 3,500 Cu/Zr atoms
+(seeded coin flip, 2.2 A minimum separation, seed 2001) in a 40 A cube, 512^2
+grid, 200 kV, Cs3 = 1.2 mm at Scherzer defocus, 2 mrad aperture, minslice 2.0
+(auto-slicer), 32 x 32 raster at 1.25 A, FEM V_omega (dr = 0.1414213562).
+The specimen's transmission functions are built before the timed region and
+stay resident in HBM; each step runs probe set-up, the slice loop over every
+position and the FP64 FEM statistics.
+
+value  = positions x slices x steps / wall time of the timed steps (batched).
+plain  = the same kernels one position at a time (batch 1) on a sample.
+roofline: the dominant slice-pass kernel, compulsory HBM bytes per launch
+          (read + write of the band-limited wave) over its average duration,
+          measured with HIP events on the library's stream in a profiled step.
+cpu_baseline: the FP64 CPU restatement (oracle, OpenMP over positions) on a
+          bounded sample of positions of the same job.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ms-stem-fem_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "probe-slice propagations/s at 512² (batched FP32 vs plain); p99 job latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(job, q, Z, xp, yp, threads, target_s):
+    """Oracle (FP64, own FFT) on a bounded sample of the raster."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    oj = O.make_job(nx=job["nx"], ny=job["ny"], xperiod=job["xperiod"], yperiod=job["yperiod"],
+                    zperiod=job["zperiod"], voltage=job["voltage"], min_slice=job["min_slice"],
+                    cs3=job["cs3"], defocus=job["defocus"], alpha_max=job["alpha_max"], adf=0,
+                    fem=1, dr=job["dr"])
+    t0 = time.perf_counter()
+    spec = O.Specimen(oj, q, Z.astype(np.int32))
+    setup_s = time.perf_counter() - t0
+    # one position to size the sample
+    t0 = time.perf_counter()
+    O.adfstem(oj, spec, xp[:1], yp[:1], n_threads=1)
+    one = time.perf_counter() - t0
+    n_pos = int(max(threads, min(xp.size * yp.size, round(target_s * threads / max(one, 1e-3)))))
+    n_pos = max(threads, (n_pos // threads) * threads)
+    ys = np.resize(yp, n_pos)
+    t0 = time.perf_counter()
+    for k in range(0, n_pos, yp.size):
+        O.adfstem(oj, spec, xp[:1], ys[k:k + yp.size], n_threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n_pos * spec.n_slices / dt, "unit": "probe-slice propagations/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{n_pos} positions x {spec.n_slices} slices of the same job "
+                      f"(FP64 restatement, OpenMP over positions, {dt:.1f} s; "
+                      f"specimen set-up {setup_s:.1f} s not included)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--plain-positions", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the multislice path has no multi-GPU mode in this "
+              "build (positions would shard across GPUs with no exchange); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import msf
+    import specimens as SP
+
+    log("generating specimen (seed 2001)")
+    job, q, Z, xp, yp = SP.config_cuzr_512(seed=2001)
+    lam, _ = msf.wavelength(job["voltage"])
+    job["defocus"] = msf.scherzer_defocus(lam, job["cs3"])
+    ctx = msf.Context(0)
+    grid = msf.make_grid(job["nx"], job["ny"], job["xperiod"], job["yperiod"], job["zperiod"],
+                         job["voltage"])
+    t0 = time.perf_counter()
+    spec = ctx.specimen(grid, q, Z, job["min_slice"])
+    spec_ms = (time.perf_counter() - t0) * 1e3
+    S = spec.n_slices
+    P = xp.size * yp.size
+    optics = msf.Optics(0, job["cs3"], 0.0, job["defocus"], job["alpha_max"])
+    fem = msf.Fem(1, job["dr"])
+    log(f"specimen: {S} slices, build {spec_ms:.1f} ms")
+
+    ctx.configure(batch=args.batch)
+    for _ in range(args.warmup):
+        ctx.adfstem(spec, optics, xp, yp, fem=fem)
+    times = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, v = ctx.adfstem(spec, optics, xp, yp, fem=fem)
+        times.append(time.perf_counter() - t0)
+    elapsed = time.perf_counter() - t_all
+    st = ctx.stats()
+    value = P * S * args.steps / elapsed
+    log(f"batched: {value:.4g} props/s, batch {st['batch']}, loop {st['slice_loop_ms']:.2f} ms")
+
+    # profiled step: per-kernel durations with HIP events on the library stream
+    ctx.configure(batch=args.batch, profile_kernels=True)
+    ctx.adfstem(spec, optics, xp, yp, fem=fem)
+    pst = ctx.stats()
+    ctx.configure(batch=args.batch)
+    kern = {"row_pass": (pst["row_pass_us"], pst["row_pass_bytes"]),
+            "col_pass": (pst["col_pass_us"], pst["col_pass_bytes"])}
+    dom = max(kern, key=lambda k: kern[k][0])
+    us, nbytes = kern[dom]
+    achieved = nbytes / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"{dom}_cuzr512_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "bytes_per_launch": nbytes, "avg_us": round(us, 2),
+                "other_kernels_us": {k: round(v[0], 2) for k, v in kern.items() if k != dom}
+                | {"first_row_pass": round(pst["first_row_pass_us"], 2),
+                   "last_col_pass": round(pst["last_col_pass_us"], 2),
+                   "reduce": round(pst["reduce_us"], 2)}}
+
+    plain = None
+    if not args.no_plain:
+        n = min(args.plain_positions, P)
+        ctx.configure(batch=1)
+        xs = np.resize(xp, n)
+        ctx.adfstem(spec, optics, xs[:1], yp[:1], fem=fem)
+        t0 = time.perf_counter()
+        ctx.adfstem(spec, optics, xs, yp[:1], fem=fem)
+        dt = time.perf_counter() - t0
+        ctx.configure(batch=args.batch)
+        plain = {"value": n * S / dt, "positions": n, "batch": 1,
+                 "speedup_batched_vs_plain": round(value / (n * S / dt), 2)}
+        log(f"plain: {plain['value']:.4g} props/s")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        cpu = cpu_baseline(job, q, Z, xp, yp, threads, args.cpu_seconds)
+
+    lat = np.array(times) * 1e3
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "probe-slice propagations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(lat.mean()), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cuzr_fem_512: 3500 Cu/Zr atoms, 40 A cube, 512x512, 200 kV, "
+                               "Cs3 1.2 mm Scherzer, 2 mrad, 32x32 raster, V_omega",
+                   "grid": 512, "atoms": int(Z.size), "slices": int(S), "positions": int(P),
+                   "batch": int(st["batch"]), "seed": 2001},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3)},
+        "slice_loop_ms": round(st["slice_loop_ms"], 3),
+        "specimen_build_ms": round(spec_ms, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
