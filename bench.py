@@ -64,6 +64,23 @@ def cpu_baseline(job, q, Z, xp, yp, threads, target_s):
                       f"specimen set-up {setup_s:.1f} s not included)"}
 
 
+def copy_calibration(nbytes):
+    """A plain copy of known size (the runtime's blit kernel), so that memory
+    counters of the slice passes can be scaled against it."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 x {nbytes} B device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +91,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run hipMemcpy device-to-device of this size first (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the multislice path has no multi-GPU mode in this "
@@ -83,6 +102,9 @@ def main():
 
     import msf
     import specimens as SP
+
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
 
     log("generating specimen (seed 2001)")
     job, q, Z, xp, yp = SP.config_cuzr_512(seed=2001)

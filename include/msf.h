@@ -85,6 +85,7 @@ typedef struct msf_config {
     uint32_t batch;                    /* probe positions per launch; 0 = automatic */
     int32_t transmit_mode;             /* MSF_TRANSMIT_* */
     int32_t profile_kernels;           /* 1: time every hot launch with HIP events */
+    int32_t col_tile;                  /* columns per column-pass workgroup: 8, 16; 0 = automatic */
 } msf_config;
 
 /* Timings of the last msf_run_stem() on the context. */

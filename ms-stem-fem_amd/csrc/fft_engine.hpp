@@ -143,8 +143,23 @@ template <bool INV, typename C> struct Dft<6, INV, C> {
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 3); }
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3) + 1; }
 
+// ------------------------------------------------------------ sync
+// WAVE = true when all lanes of a transform (and its LDS region) belong to one
+// wavefront: LDS instructions of a wavefront execute in order, so only the
+// compiler must be kept from reordering the exchange.
+template <bool WAVE> __device__ __forceinline__ void lds_sync()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ one stage
-template <typename C, int N, int E, int R, int NS, bool INV, bool LAST>
+template <typename C, int N, int E, int R, int NS, bool INV, bool LAST, bool WAVE>
 __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 {
     constexpr int T = N / E;
@@ -174,7 +189,7 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 #pragma unroll
             for (int q = 0; q < R; ++q) x[m + q * BPT] = y[m * R + q];
     } else {
-        __syncthreads(); // previous readers of this buffer are done
+        lds_sync<WAVE>(); // previous readers of this buffer are done
 #pragma unroll
         for (int m = 0; m < BPT; ++m) {
             const int j = t + T * m;
@@ -182,17 +197,18 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 #pragma unroll
             for (int q = 0; q < R; ++q) lds[lds_pad(base + q * NS)] = y[m * R + q];
         }
-        __syncthreads();
+        lds_sync<WAVE>();
 #pragma unroll
         for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
     }
 }
 
-template <typename C, int N, int E, bool INV, int NS, int R, int... Rest>
+template <typename C, int N, int E, bool INV, bool WAVE, int NS, int R, int... Rest>
 __device__ __forceinline__ void run_stages(C (&x)[E], C* lds, const C* tw, int t)
 {
-    fft_stage<C, N, E, R, NS, INV, sizeof...(Rest) == 0>(x, lds, tw, t);
-    if constexpr (sizeof...(Rest) > 0) run_stages<C, N, E, INV, NS * R, Rest...>(x, lds, tw, t);
+    fft_stage<C, N, E, R, NS, INV, sizeof...(Rest) == 0, WAVE>(x, lds, tw, t);
+    if constexpr (sizeof...(Rest) > 0)
+        run_stages<C, N, E, INV, WAVE, NS * R, Rest...>(x, lds, tw, t);
 }
 
 // ------------------------------------------------------------ plans
@@ -201,10 +217,10 @@ template <int N> struct Plan;
 #define MSF_PLAN(N_, E_, ...)                                                                  \
     template <> struct Plan<N_> {                                                              \
         static constexpr int N = N_, E = E_, T = N_ / E_;                                      \
-        template <bool INV, typename C>                                                        \
+        template <bool INV, typename C, bool WAVE = false>                                     \
         static __device__ __forceinline__ void run(C (&x)[E_], C* lds, const C* tw, int t)     \
         {                                                                                      \
-            run_stages<C, N_, E_, INV, 1, __VA_ARGS__>(x, lds, tw, t);                         \
+            run_stages<C, N_, E_, INV, WAVE, 1, __VA_ARGS__>(x, lds, tw, t);                   \
         }                                                                                      \
     };
 MSF_PLAN(16, 8, 8, 2)

@@ -43,20 +43,26 @@ struct ColPassArgs {
     const int2* shift;      // [B] probe shifts (last pass only)
     const float2* twy;      // [ny] forward roots (last pass only)
     int ny, ncp, ka, kb, batch;
+    int tile;               // columns per workgroup (8 or 16; 0 = default)
 };
 
 bool slice_lengths_ok(int n);
-// tile width (columns per workgroup) of the column pass for length nx
+// the column count of the compact wave is padded to a multiple of this
 int col_tile(int nx);
 
 hipError_t launch_row_pass(int ny, bool first, int transmit_mode, const RowPassArgs& a,
                            hipStream_t s);
 hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s);
 
-// per (position, group) FP64 sums over pixel lists: kind 0 -> |psi|^2, 1 -> |psi|
-hipError_t launch_reduce_groups(const float2* psi, int64_t plane, const int32_t* offsets,
-                                const int32_t* pixels, const int32_t* kind, int groups,
-                                int batch, double* out, hipStream_t s);
+// per (position, segment) FP64 sums over interleaved pixel tiles:
+// kind 0 -> |psi|^2, 1 -> |psi|; then per (position, group) the fixed-order
+// sum of the group's segments
+hipError_t launch_reduce_segments(const float2* psi, int64_t plane, const int32_t* pix,
+                                  const int32_t* tile_off, const int32_t* seg_kind, int n_tiles,
+                                  int batch, double* seg_out, hipStream_t s);
+hipError_t launch_combine_segments(const double* seg_in, int n_seg_stride,
+                                   const int32_t* seg_of_group, int groups, int batch,
+                                   double* out, hipStream_t s);
 // ADF column of the partial sums -> adf[pos0 + b]
 hipError_t launch_collect_adf(const double* partial, int groups, int batch, double* adf,
                               hipStream_t s);

@@ -22,6 +22,27 @@ namespace msf {
 //   a  = t're u_re - ti u_im,            b  = t're u_im + ti a
 //   Da = tau u0_re - ti u0_im,           Db = tau u0_im + ti (u0_re/s + Da)
 // and TMODE 1 (exact product) uses b = t're u_im + ti u_re, Db = tau u0_im + ti u0_re.
+template <int N> __device__ __forceinline__ int wrap_sub(int j, int s)
+{
+    if constexpr ((N & (N - 1)) == 0) {
+        return (j - s) & (N - 1);
+    } else {
+        const int d = j - s;
+        return d < 0 ? d + N : d;
+    }
+}
+
+// Workgroup -> (row i, group of RPW positions).  Consecutive launch indices go
+// to different XCDs round-robin; the permutation gives each XCD a contiguous
+// band of rows so that the t and u0 rows a band needs are fetched into that
+// XCD's L2 once.
+__device__ __forceinline__ int xcd_band(int L, int G)
+{
+    constexpr int kXcd = 8;
+    if (G % kXcd) return L;
+    return (L % kXcd) * (G / kXcd) + L / kXcd;
+}
+
 template <int NY, bool FIRST, int TMODE>
 __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
 {
@@ -30,17 +51,27 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
     constexpr int RPW = 256 / T;
     constexpr int LEN = lds_len(NY);
     __shared__ float2 s_tw[NY];
+    __shared__ float2 s_t[NY];
     __shared__ float2 s_buf[RPW * LEN];
 
-    for (int e = threadIdx.x; e < NY; e += blockDim.x) s_tw[e] = a.tw[e];
+    // workgroup-uniform decode (scalar unit)
+    const int groups = (a.batch + RPW - 1) / RPW;
+    const int g = xcd_band((int)blockIdx.x, groups * a.nx);
+    const int i = g / groups;
+    const int b0 = (g - i * groups) * RPW;
+
+    // twiddles and the transmission row i, shared by the RPW rows of the group
+    const float2* trow = a.t + (int64_t)i * NY;
+    for (int e = threadIdx.x; e < NY; e += blockDim.x) {
+        s_tw[e] = a.tw[e];
+        s_t[e] = trow[e];
+    }
     __syncthreads();
 
     const int lr = threadIdx.x / T;
     const int t = threadIdx.x % T;
-    const int64_t r = (int64_t)blockIdx.x * RPW + lr;
-    const bool active = r < (int64_t)a.batch * a.nx;
-    const int b = active ? (int)(r % a.batch) : 0;
-    const int i = active ? (int)(r / a.batch) : 0;
+    const bool active = b0 + lr < a.batch;
+    const int b = active ? b0 + lr : a.batch - 1;
     float2* row = a.psi + ((int64_t)b * a.nx + i) * a.ncp;
     float2* lds = s_buf + lr * LEN;
     const int hi = NY - a.kb;           // first kept column of the negative-ky band
@@ -51,29 +82,24 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
 #pragma unroll
         for (int p = 0; p < E; ++p) {
             const int j = t + T * p;
-            float2 v{0.f, 0.f};
-            if (active) {
-                if (j < a.ka) v = row[j];
-                else if (j >= hi) v = row[j + off];
-            }
-            x[p] = v;
+            const bool in = (j < a.ka) || (j >= hi);
+            // lanes outside the kept band all read element 0 (one shared line)
+            const float2 v = row[in ? (j < a.ka ? j : j + off) : 0];
+            x[p] = in ? v : float2{0.f, 0.f};
         }
-        P::template run<true>(x, lds, s_tw, t);
+        P::template run<true, float2, true>(x, lds, s_tw, t);
     }
 
-    const int2 sh = active ? a.shift[b] : int2{0, 0};
+    const int2 sh = a.shift[b];
     int ii = i - sh.x;
     if (ii < 0) ii += a.nx;
     const float2* urow = a.u0 + (int64_t)ii * NY;
-    const float2* trow = a.t + (int64_t)i * NY;
     const float is = a.inv_s;
 #pragma unroll
     for (int p = 0; p < E; ++p) {
         const int j = t + T * p;
-        int jj = j - sh.y;
-        if (jj < 0) jj += NY;
-        const float2 tv = trow[j];
-        const float2 u0 = urow[jj];
+        const float2 tv = s_t[j];
+        const float2 u0 = urow[wrap_sub<NY>(j, sh.y)];
         const float da = tv.x * u0.x - tv.y * u0.y;
         const float db = (TMODE == 0) ? tv.x * u0.y + tv.y * (u0.x * is + da) : tv.x * u0.y + tv.y * u0.x;
         if constexpr (FIRST) {
@@ -87,7 +113,7 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
         }
     }
 
-    P::template run<false>(x, lds, s_tw, t);
+    P::template run<false, float2, true>(x, lds, s_tw, t);
 
     if (active) {
 #pragma unroll
@@ -100,42 +126,75 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
 }
 
 // --------------------------------------------------------------- column pass
+// A workgroup owns W adjacent kept columns of one position.  The tile is
+// loaded with lanes running along the columns (coalesced W*8-byte segments
+// per row) into a column-major LDS tile, each column is then transformed by
+// T lanes of one wavefront (wave-local exchanges, no workgroup barrier
+// between radix stages), and the tile is written back the same coalesced way.
 template <int NX> struct ColTile {
     static constexpr int T = Plan<NX>::T;
-    static constexpr int W0 = 1024 / T < 16 ? 1024 / T : 16;
-    // keep the tile's LDS (W * lds_len * 8 B + twiddles) well inside 160 KiB
-    static constexpr int W = (W0 * lds_len(NX) * 8 > 96 * 1024) ? W0 / 2 : W0;
+    static constexpr int WMAX = 16;  // ncp is a multiple of this
+    // default tile: keep LDS (W * lds_len * 8 B) near 40 KiB so that several
+    // workgroups share a CU
+    static constexpr int W = (16 * lds_len(NX) * 8 > 48 * 1024) ? 8 : 16;
 };
 
-template <int NX, bool LAST>
+// column stride of the LDS tile: the coalesced load/store phases touch W
+// columns x (64/W) rows per wavefront; a stride congruent to 64/W modulo 16
+// (in 8-byte words) spreads each 16-lane group over all banks
+template <int NX, int W> __host__ __device__ constexpr int col_len()
+{
+    constexpr int base = lds_len(NX);
+    constexpr int want = (64 / W) % 16;
+    return base + ((want - base % 16) + 16) % 16;
+}
+
+template <int NX, bool LAST, int W>
 __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 {
     using P = Plan<NX>;
     constexpr int E = P::E, T = P::T;
-    constexpr int W = ColTile<NX>::W;
-    constexpr int LEN = lds_len(NX);
+    constexpr int LEN = col_len<NX, W>();
+    static_assert(64 % T == 0, "a column transform must fit in one wavefront");
     __shared__ float2 s_tw[NX];
+    __shared__ float2 s_px[NX];
+    __shared__ int2 s_keep[NX];
     __shared__ float2 s_buf[W * LEN];
 
-    for (int e = threadIdx.x; e < NX; e += blockDim.x) s_tw[e] = a.tw[e];
-    __syncthreads();
+    for (int e = threadIdx.x; e < NX; e += blockDim.x) {
+        s_tw[e] = a.tw[e];
+        s_px[e] = a.px[e];
+        s_keep[e] = a.keep[e];
+    }
 
-    const int c = threadIdx.x % W;
-    const int t = threadIdx.x / W;
     const int ntiles = a.ncp / W;
     const int b = blockIdx.x / ntiles;
-    const int jc = (blockIdx.x % ntiles) * W + c;
+    const int j0 = (blockIdx.x % ntiles) * W;
     const int nc = a.ka + a.kb;
+    float2* tile = a.psi + (int64_t)b * NX * a.ncp + j0;
+
+    {   // coalesced load of the tile
+        const int c = threadIdx.x % W, r0 = threadIdx.x / W;
+        const bool ok = (b < a.batch) && (j0 + c < nc);
+#pragma unroll
+        for (int p = 0; p < E; ++p) {
+            const int i = r0 + T * p;
+            s_buf[c * LEN + lds_pad(i)] = ok ? tile[(int64_t)i * a.ncp + c] : float2{0.f, 0.f};
+        }
+    }
+    __syncthreads();
+
+    const int c = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int jc = j0 + c;
     const bool active = (b < a.batch) && (jc < nc);
     const int j = (jc < a.ka) ? jc : jc - nc + a.ny;
-    float2* col = a.psi + (int64_t)b * NX * a.ncp + jc;
     float2* lds = s_buf + c * LEN;
-
     float2 x[E];
 #pragma unroll
-    for (int p = 0; p < E; ++p) x[p] = active ? col[(int64_t)(t + T * p) * a.ncp] : float2{0.f, 0.f};
+    for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
 
-    P::template run<false>(x, lds, s_tw, t);
+    P::template run<false, float2, true>(x, lds, s_tw, t);
 
     // band limit (bw_limit + propagate's k^2 test, decided on the host) and
     // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
@@ -143,16 +202,16 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 #pragma unroll
     for (int p = 0; p < E; ++p) {
         const int i = t + T * p;
-        const int2 kr = a.keep[i];
+        const int2 kr = s_keep[i];
         const bool keep = (j < kr.x) || (j >= a.ny - kr.y);
-        const float2 pxi = a.px[i];
+        const float2 pxi = s_px[i];
         const float2 pr = float2{pxi.x * pyj.x - pxi.y * pyj.y, pxi.x * pyj.y + pxi.y * pyj.x};
         const float2 f = x[p];
         x[p] = keep ? float2{pr.x * f.x - pr.y * f.y, pr.x * f.y + pr.y * f.x} : float2{0.f, 0.f};
     }
 
     if constexpr (!LAST) {
-        P::template run<true>(x, lds, s_tw, t);
+        P::template run<true, float2, true>(x, lds, s_tw, t);
     } else if (active) {
         // add the vacuum exit wave, shifted: psi0(k) e^{-2 pi i (i sx / nx + j sy / ny)}
         const int2 sh = a.shift[b];
@@ -168,9 +227,21 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
         }
     }
 
-    if (active) {
+    lds_sync<true>();
 #pragma unroll
-        for (int p = 0; p < E; ++p) col[(int64_t)(t + T * p) * a.ncp] = x[p];
+    for (int p = 0; p < E; ++p) lds[lds_pad(t + T * p)] = x[p];
+    __syncthreads();
+
+    {   // coalesced store of the tile
+        const int c2 = threadIdx.x % W, r0 = threadIdx.x / W;
+        const bool ok = (b < a.batch) && (j0 + c2 < nc);
+        if (ok) {
+#pragma unroll
+            for (int p = 0; p < E; ++p) {
+                const int i = r0 + T * p;
+                tile[(int64_t)i * a.ncp + c2] = s_buf[c2 * LEN + lds_pad(i)];
+            }
+        }
     }
 }
 
@@ -189,7 +260,7 @@ bool slice_lengths_ok(int n)
 int col_tile(int nx)
 {
     switch (nx) {
-#define MSF_CASE(N) case N: return ColTile<N>::W;
+#define MSF_CASE(N) case N: return ColTile<N>::WMAX;
         MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
         MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
 #undef MSF_CASE
@@ -202,8 +273,8 @@ template <int NY>
 static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStream_t s)
 {
     constexpr int RPW = 256 / Plan<NY>::T;
-    const int64_t rows = (int64_t)a.batch * a.nx;
-    const dim3 grid((unsigned)((rows + RPW - 1) / RPW));
+    const int64_t groups = (a.batch + RPW - 1) / RPW;
+    const dim3 grid((unsigned)(groups * a.nx));
     const dim3 block(RPW * Plan<NY>::T);
     if (first) {
         if (tmode == 0) hipLaunchKernelGGL((row_pass_kernel<NY, true, 0>), grid, block, 0, s, a);
@@ -227,16 +298,32 @@ hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, 
     }
 }
 
-template <int NX>
-static hipError_t col_pass_n(bool last, const ColPassArgs& a, hipStream_t s)
+template <int NX, int W>
+static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s)
 {
-    constexpr int W = ColTile<NX>::W;
     if (a.ncp % W) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((int64_t)a.batch * (a.ncp / W)));
     const dim3 block(W * Plan<NX>::T);
-    if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((col_pass_kernel<NX, false>), grid, block, 0, s, a);
+    if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true, W>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((col_pass_kernel<NX, false, W>), grid, block, 0, s, a);
     return hipGetLastError();
+}
+
+template <int NX, int W> constexpr bool col_fits()
+{
+    return (size_t)W * col_len<NX, W>() * 8 + (size_t)NX * 24 <= 160 * 1024 &&
+           W * Plan<NX>::T <= 1024;
+}
+
+template <int NX>
+static hipError_t col_pass_n(bool last, const ColPassArgs& a, hipStream_t s)
+{
+    int w = ColTile<NX>::W;
+    if (a.tile == 8 || a.tile == 16) w = a.tile;
+    if constexpr (col_fits<NX, 16>()) {
+        if (w == 16) return col_pass_w<NX, 16>(last, a, s);
+    }
+    return col_pass_w<NX, 8>(last, a, s);
 }
 
 hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s)
@@ -252,41 +339,69 @@ hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t 
 }
 
 // ------------------------------------------------------------- reductions
-// One wavefront per (group, position): lanes stride the group's pixel list,
-// accumulate in FP64, then a fixed shuffle tree; the result does not depend
-// on the batch composition.
-__global__ void __launch_bounds__(256) reduce_groups_kernel(const float2* psi, int64_t plane,
-                                                             const int32_t* offsets,
-                                                             const int32_t* pixels,
-                                                             const int32_t* kind, int groups,
-                                                             double* out)
+// Each lane sums one segment of a ring's pixel list in FP64, in a fixed order;
+// the segments of a ring are then added in segment order.  Neither depends on
+// the batch composition.
+__global__ void __launch_bounds__(256) reduce_segments_kernel(const float2* psi, int64_t plane,
+                                                               const int32_t* pix,
+                                                               const int32_t* tile_off,
+                                                               const int32_t* seg_kind,
+                                                               int n_tiles, double* seg_out)
 {
     const int lane = threadIdx.x & 63;
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int b = blockIdx.y;
-    if (g >= groups) return;
-    const float2* w = psi + (int64_t)b * plane;
-    const int k0 = offsets[g], k1 = offsets[g + 1];
-    const bool amp = kind[g] != 0;
+    if (w >= n_tiles) return;
+    const float2* wave = psi + (int64_t)b * plane;
+    const int k0 = tile_off[w], k1 = tile_off[w + 1];
+    const bool amp = seg_kind[w * 64 + lane] != 0;
     double acc = 0.0;
-    for (int k = k0 + lane; k < k1; k += 64) {
-        const float2 v = w[pixels[k]];
-        const double re = v.x, im = v.y;
-        const double i2 = re * re + im * im;
-        acc += amp ? sqrt(i2) : i2;
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
+        const int o = pix[(int64_t)k * 64 + lane];
+        if (o >= 0) {
+            const float2 v = wave[o];
+            const double re = v.x, im = v.y;
+            const double i2 = re * re + im * im;
+            acc += amp ? sqrt(i2) : i2;
+        }
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
-    if (lane == 0) out[(int64_t)b * groups + g] = acc;
+    seg_out[(int64_t)b * n_tiles * 64 + w * 64 + lane] = acc;
 }
 
-hipError_t launch_reduce_groups(const float2* psi, int64_t plane, const int32_t* offsets,
-                                const int32_t* pixels, const int32_t* kind, int groups,
-                                int batch, double* out, hipStream_t s)
+hipError_t launch_reduce_segments(const float2* psi, int64_t plane, const int32_t* pix,
+                                  const int32_t* tile_off, const int32_t* seg_kind, int n_tiles,
+                                  int batch, double* seg_out, hipStream_t s)
 {
-    const dim3 grid((unsigned)((groups + 3) / 4), (unsigned)batch);
-    hipLaunchKernelGGL(reduce_groups_kernel, grid, dim3(256), 0, s, psi, plane, offsets, pixels,
-                       kind, groups, out);
+    if (n_tiles <= 0 || batch <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n_tiles + 3) / 4), (unsigned)batch);
+    hipLaunchKernelGGL(reduce_segments_kernel, grid, dim3(256), 0, s, psi, plane, pix, tile_off,
+                       seg_kind, n_tiles, seg_out);
+    return hipGetLastError();
+}
+
+__global__ void combine_segments_kernel(const double* seg_in, int n_seg_stride,
+                                        const int32_t* seg_of_group, int groups, int batch,
+                                        double* out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)groups * batch) return;
+    const int g = (int)(idx % groups);
+    const int b = (int)(idx / groups);
+    const double* in = seg_in + (int64_t)b * n_seg_stride;
+    double acc = 0.0;
+    for (int k = seg_of_group[g]; k < seg_of_group[g + 1]; ++k) acc += in[k];
+    out[(int64_t)b * groups + g] = acc;
+}
+
+hipError_t launch_combine_segments(const double* seg_in, int n_seg_stride,
+                                   const int32_t* seg_of_group, int groups, int batch,
+                                   double* out, hipStream_t s)
+{
+    const int64_t n = (int64_t)groups * batch;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(combine_segments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       seg_in, n_seg_stride, seg_of_group, groups, batch, out);
     return hipGetLastError();
 }
 

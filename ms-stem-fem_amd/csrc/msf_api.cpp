@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -49,6 +50,8 @@ struct DevBuf {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+std::atomic<uint64_t> g_specimen_ids{1};
+
 double now_ms()
 {
     using namespace std::chrono;
@@ -57,13 +60,30 @@ double now_ms()
 
 } // namespace
 
+// Everything a raster run needs besides the positions, cached on the context
+// and rebuilt only when the specimen, optics, detector or FEM settings change.
+struct RunPlan {
+    uint64_t spec_id = 0;
+    msf_optics optics{};
+    msf_detector det{};
+    msf_fem fem{};
+    bool valid = false;
+    int ncp = 0, G = 0, nbins = 0;
+    DevBuf U0, Psi0;                          // vacuum waves
+    DevBuf seg_pix, tile_off, seg_kind, seg_of_group, cnt, zero;  // reduction groups
+    int n_tiles = 0;
+    int64_t cap_b = 0, cap_p = 0;
+    DevBuf psi, part, segp, state, femv, adf, shift, exit;
+};
+
 struct msf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0};
+    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0};
     msf_stats stats{};
     std::map<int, DevBuf> tw32, tw64;
+    RunPlan plan;
 };
 
 struct msf_specimen {
@@ -79,6 +99,9 @@ struct msf_specimen {
     DevBuf py;   // [S][ny] float2, P_y / sqrtNxNy
     DevBuf px64; // [S][nx] double2
     DevBuf py64; // [S][ny] double2
+    uint64_t id = 0;           // unique per build, keys the run-plan cache
+    msf::RowRanges keep;       // columns the slice step keeps (bw_limit + propagate)
+    DevBuf dkeep;              // keep as int2 [nx]
 };
 
 namespace {
@@ -220,56 +243,82 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             }
     }
 
-    // band-limited column set of the slice step
-    const msf::RowRanges keep = msf::slice_keep_ranges(d);
-    if (!keep.ok) return fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order");
+    const msf::RowRanges& keep = sp->keep;
     const int W = msf::col_tile(nx);
     const int ka = keep.amax, kb = keep.bmax;
     const int ncp = std::max(W, ((ka + kb + W - 1) / W) * W);
-
-    double k1 = 0.0, k2 = 0.0;
-    if (want_adf) {
-        k1 = det->inner / d.lambda;
-        k2 = det->outer / d.lambda;
-        if (det->threshold_mode == MSF_ADF_PHYSICAL) {
-            k1 = k1 * k1;
-            k2 = k2 * k2;
-        }
-    }
-    const msf::Groups g = msf::build_groups(d, keep, ncp, want_adf, k1, k2, want_fem,
-                                            want_fem ? fem->dr : 1.0, o->alpha_max);
-    const int G = (int)g.kind.size();
-    const int nbins = g.nbins;
-
-    // vacuum wave (t = 1) of every slice, FP64, from the probe of
-    // adfstem.cpp:586-817: u0_0 = s * A_p * IFFT(H)/s; then per slice
-    // w = FFT(u0_n) / s^2 * M P_n (adfstem.cpp:1986-2043), u0_{n+1} = IFFT(w)
     const int64_t M = (int64_t)nx * ny;
-    DevBuf h, norm2, dU0, dPsi0;
-    if (int r = probe_device(c, d, o, h, norm2)) return r;
-    const double2 *twx64, *twy64;
-    if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
-    if (int r = twiddles(c, ny, nullptr, &twy64)) return r;
-    HIPCHK(c, dU0.alloc((size_t)S * M * sizeof(float2)));
-    HIPCHK(c, dPsi0.alloc((size_t)M * sizeof(float2)));
-    {
-        DevBuf dkeep64;
-        HIPCHK(c, upload(dkeep64, to_int2(keep), c->stream));
+    const int64_t plane = (int64_t)nx * ncp;
+
+    RunPlan& pl = c->plan;
+    msf_detector dkey{};
+    msf_fem fkey{};
+    if (want_adf) dkey = *det;
+    if (want_fem) fkey = *fem;
+    const bool same_optics = pl.valid && pl.spec_id == sp->id &&
+                             std::memcmp(&pl.optics, o, sizeof(msf_optics)) == 0;
+    const bool same_groups = same_optics && std::memcmp(&pl.det, &dkey, sizeof dkey) == 0 &&
+                             std::memcmp(&pl.fem, &fkey, sizeof fkey) == 0;
+    if (!same_optics || !same_groups) pl.valid = false;
+
+    if (!same_groups) {
+        double k1 = 0.0, k2 = 0.0;
+        if (want_adf) {
+            k1 = det->inner / d.lambda;
+            k2 = det->outer / d.lambda;
+            if (det->threshold_mode == MSF_ADF_PHYSICAL) {
+                k1 = k1 * k1;
+                k2 = k2 * k2;
+            }
+        }
+        const msf::Groups g = msf::build_groups(d, keep, ncp, want_adf, k1, k2, want_fem,
+                                                want_fem ? fem->dr : 1.0, o->alpha_max);
+        pl.G = (int)g.kind.size();
+        pl.nbins = g.nbins;
+        const msf::SegmentTiles st = msf::build_segments(g, 32);
+        pl.n_tiles = st.n_tiles;
+        HIPCHK(c, upload(pl.seg_pix, st.pix, c->stream));
+        HIPCHK(c, upload(pl.tile_off, st.tile_off, c->stream));
+        HIPCHK(c, upload(pl.seg_kind, st.seg_kind, c->stream));
+        HIPCHK(c, upload(pl.seg_of_group, st.seg_of_group, c->stream));
+        HIPCHK(c, upload(pl.cnt, g.counts, c->stream));
+        HIPCHK(c, upload(pl.zero, g.zeroed, c->stream));
+        pl.cap_b = 0; // partial-sum buffers depend on G
+    }
+    const int G = pl.G;
+    const int nbins = pl.nbins;
+
+    if (!same_optics) {
+        // vacuum wave (t = 1) of every slice, FP64, from the probe of
+        // adfstem.cpp:586-817: u0_0 = s * A_p * IFFT(H)/s; then per slice
+        // w = FFT(u0_n) / s^2 * M P_n (adfstem.cpp:1986-2043), u0_{n+1} = IFFT(w)
+        DevBuf h, norm2;
+        if (int r = probe_device(c, d, o, h, norm2)) return r;
+        const double2 *twx64, *twy64;
+        if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
+        if (int r = twiddles(c, ny, nullptr, &twy64)) return r;
+        HIPCHK(c, pl.U0.alloc((size_t)S * M * sizeof(float2)));
+        HIPCHK(c, pl.Psi0.alloc((size_t)M * sizeof(float2)));
         double2* v = h.as<double2>();
         HIPCHK(c, msf::launch_scale_f64(v, M, norm2.as<double>(), d.sqrt_nxny, c->stream));
         for (int n = 0; n < S; ++n) {
-            HIPCHK(c, msf::launch_to_f32(v, M, dU0.as<float2>() + (int64_t)n * M, c->stream));
+            HIPCHK(c, msf::launch_to_f32(v, M, pl.U0.as<float2>() + (int64_t)n * M, c->stream));
             HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, false, twx64, twy64, c->stream));
             HIPCHK(c, msf::launch_vacuum_step(v, nx, ny, sp->px64.as<double2>() + (int64_t)n * nx,
                                               sp->py64.as<double2>() + (int64_t)n * ny,
-                                              dkeep64.as<int2>(), 1.0 / (d.sqrt_nxny * d.sqrt_nxny),
+                                              sp->dkeep.as<int2>(), 1.0 / (d.sqrt_nxny * d.sqrt_nxny),
                                               c->stream));
             if (n + 1 < S) HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, true, twx64, twy64, c->stream));
         }
-        HIPCHK(c, msf::launch_to_f32(v, M, dPsi0.as<float2>(), c->stream));
+        HIPCHK(c, msf::launch_to_f32(v, M, pl.Psi0.as<float2>(), c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    h.release();
+    pl.spec_id = sp->id;
+    pl.optics = *o;
+    pl.det = dkey;
+    pl.fem = fkey;
+    pl.ncp = ncp;
+    pl.valid = true;
 
     const float2 *twx, *twy;
     if (int r = twiddles(c, nx, &twx, nullptr)) return r;
@@ -278,23 +327,33 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     int64_t B = c->cfg.batch ? (int64_t)c->cfg.batch : std::min<int64_t>(P, 128);
     B = std::max<int64_t>(1, std::min<int64_t>(B, P));
 
-    DevBuf dshift, dkeep, doff, dpix, dkind, dcnt, dzero, dpsi, dpart, dadf, dstate, dfemv, dexit;
-    HIPCHK(c, upload(dshift, shifts, c->stream));
-    HIPCHK(c, upload(dkeep, to_int2(keep), c->stream));
-    HIPCHK(c, upload(doff, g.offsets, c->stream));
-    HIPCHK(c, upload(dpix, g.pixels, c->stream));
-    HIPCHK(c, upload(dkind, g.kind, c->stream));
-    HIPCHK(c, upload(dcnt, g.counts, c->stream));
-    HIPCHK(c, upload(dzero, g.zeroed, c->stream));
-    const int64_t plane = (int64_t)nx * ncp;
-    HIPCHK(c, dpsi.alloc((size_t)B * plane * sizeof(float2)));
-    HIPCHK(c, hipMemsetAsync(dpsi.p, 0, dpsi.bytes, c->stream));
-    HIPCHK(c, dpart.alloc((size_t)B * G * sizeof(double)));
-    HIPCHK(c, dadf.alloc((size_t)P * sizeof(double)));
-    HIPCHK(c, dstate.alloc((size_t)std::max(nbins, 1) * 3 * sizeof(double)));
-    HIPCHK(c, hipMemsetAsync(dstate.p, 0, dstate.bytes, c->stream));
-    HIPCHK(c, dfemv.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
-    if (out.exit) HIPCHK(c, dexit.alloc((size_t)B * nx * ny * sizeof(float2)));
+    // work buffers, grown as needed and kept for the next run
+    if (B > pl.cap_b || pl.psi.bytes < (size_t)B * plane * sizeof(float2)) {
+        HIPCHK(c, pl.psi.alloc((size_t)B * plane * sizeof(float2)));
+        HIPCHK(c, pl.part.alloc((size_t)B * G * sizeof(double)));
+        HIPCHK(c, pl.segp.alloc((size_t)B * std::max(pl.n_tiles, 1) * 64 * sizeof(double)));
+        HIPCHK(c, hipMemsetAsync(pl.psi.p, 0, pl.psi.bytes, c->stream));
+        pl.cap_b = B;
+        pl.exit.release();
+    }
+    if (P > pl.cap_p) {
+        HIPCHK(c, pl.adf.alloc((size_t)P * sizeof(double)));
+        HIPCHK(c, pl.shift.alloc((size_t)P * sizeof(int2)));
+        pl.cap_p = P;
+    }
+    if (pl.state.bytes < (size_t)std::max(nbins, 1) * 3 * sizeof(double)) {
+        HIPCHK(c, pl.state.alloc((size_t)std::max(nbins, 1) * 3 * sizeof(double)));
+        HIPCHK(c, pl.femv.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
+    }
+    HIPCHK(c, hipMemsetAsync(pl.state.p, 0, (size_t)std::max(nbins, 1) * 3 * sizeof(double), c->stream));
+    HIPCHK(c, hipMemcpyAsync(pl.shift.p, shifts.data(), (size_t)P * sizeof(int2),
+                             hipMemcpyHostToDevice, c->stream));
+    if (out.exit && pl.exit.bytes < (size_t)B * M * sizeof(float2))
+        HIPCHK(c, pl.exit.alloc((size_t)B * M * sizeof(float2)));
+    const DevBuf& dkeep = sp->dkeep;
+    DevBuf &dshift = pl.shift, &dcnt = pl.cnt, &dzero = pl.zero, &dpsi = pl.psi, &dpart = pl.part,
+           &dadf = pl.adf, &dstate = pl.state, &dfemv = pl.femv, &dexit = pl.exit,
+           &dU0 = pl.U0, &dPsi0 = pl.Psi0;
 
     // events for per-kernel timing
     const bool prof = c->cfg.profile_kernels != 0;
@@ -333,6 +392,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ca.tw = twx;
     ca.psi0 = dPsi0.as<float2>();
     ca.twy = twy;
+    ca.tile = c->cfg.col_tile;
     const float2* t32 = sp->t32.as<float2>();
     const float2* px = sp->px.as<float2>();
     const float2* py = sp->py.as<float2>();
@@ -357,9 +417,12 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             HIPCHK(c, ev_end());
         }
         HIPCHK(c, ev_begin(K_RED));
-        HIPCHK(c, msf::launch_reduce_groups(dpsi.as<float2>(), plane, doff.as<int32_t>(),
-                                            dpix.as<int32_t>(), dkind.as<int32_t>(), G, nb,
-                                            dpart.as<double>(), c->stream));
+        HIPCHK(c, msf::launch_reduce_segments(dpsi.as<float2>(), plane, pl.seg_pix.as<int32_t>(),
+                                              pl.tile_off.as<int32_t>(), pl.seg_kind.as<int32_t>(),
+                                              pl.n_tiles, nb, pl.segp.as<double>(), c->stream));
+        HIPCHK(c, msf::launch_combine_segments(pl.segp.as<double>(), pl.n_tiles * 64,
+                                               pl.seg_of_group.as<int32_t>(), G, nb,
+                                               dpart.as<double>(), c->stream));
         HIPCHK(c, ev_end());
         if (want_adf)
             HIPCHK(c, msf::launch_collect_adf(dpart.as<double>(), G, nb, dadf.as<double>() + b0,
@@ -459,6 +522,8 @@ int msf_set_config(msf_ctx* c, const msf_config* cfg)
     if (!c || !cfg) return MSF_ERR_INVALID;
     if (cfg->transmit_mode != MSF_TRANSMIT_REFERENCE && cfg->transmit_mode != MSF_TRANSMIT_EXACT)
         return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
+    if (cfg->col_tile != 0 && cfg->col_tile != 8 && cfg->col_tile != 16)
+        return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 8 or 16");
     c->cfg = *cfg;
     return MSF_OK;
 }
@@ -507,6 +572,16 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
     sp->dom = msf::make_domain(nx, ny, grid->xperiod, grid->yperiod, grid->xmin, grid->ymin,
                                grid->voltage);
     const msf::Domain& d = sp->dom;
+    sp->id = g_specimen_ids.fetch_add(1);
+    sp->keep = msf::slice_keep_ranges(d);
+    if (!sp->keep.ok) {
+        delete sp;
+        return fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order");
+    }
+    if (hipError_t e = upload(sp->dkeep, to_int2(sp->keep), c->stream)) {
+        delete sp;
+        return fail(c, MSF_ERR_DEVICE, hipGetErrorString(e));
+    }
 
     // species present, ascending Z (tem.cpp:944-972)
     std::vector<int> species_of(104, -1), zlist;
@@ -640,7 +715,11 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
 int msf_specimen_free(msf_specimen* sp)
 {
     if (!sp) return MSF_ERR_INVALID;
-    if (sp->ctx) (void)hipSetDevice(sp->ctx->device);
+    if (sp->ctx) {
+        (void)hipSetDevice(sp->ctx->device);
+        (void)hipStreamSynchronize(sp->ctx->stream);
+        if (sp->ctx->plan.spec_id == sp->id) sp->ctx->plan.valid = false;
+    }
     delete sp;
     return MSF_OK;
 }

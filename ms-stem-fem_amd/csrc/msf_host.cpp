@@ -223,4 +223,44 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
     return g;
 }
 
+SegmentTiles build_segments(const Groups& g, int seg)
+{
+    SegmentTiles t;
+    std::vector<std::pair<int32_t, int32_t>> segs; // [begin, end) into g.pixels
+    const int G = (int)g.kind.size();
+    for (int k = 0; k < G; ++k) {
+        t.seg_of_group.push_back((int32_t)segs.size());
+        for (int32_t b = g.offsets[k]; b < g.offsets[k + 1]; b += seg) {
+            const int32_t e = std::min(b + seg, g.offsets[k + 1]);
+            // gather order within a segment: memory order
+            segs.emplace_back(b, e);
+            t.seg_kind.push_back(g.kind[k]);
+        }
+    }
+    t.seg_of_group.push_back((int32_t)segs.size());
+    t.n_segments = (int)segs.size();
+    t.n_tiles = (t.n_segments + 63) / 64;
+    t.tile_off.push_back(0);
+    std::vector<int32_t> sorted;
+    for (int w = 0; w < t.n_tiles; ++w) {
+        int len = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int s = w * 64 + l;
+            if (s < t.n_segments) len = std::max(len, segs[s].second - segs[s].first);
+        }
+        const size_t base = t.pix.size();
+        t.pix.resize(base + (size_t)len * 64, -1);
+        for (int l = 0; l < 64; ++l) {
+            const int s = w * 64 + l;
+            if (s >= t.n_segments) continue;
+            sorted.assign(g.pixels.begin() + segs[s].first, g.pixels.begin() + segs[s].second);
+            std::sort(sorted.begin(), sorted.end());
+            for (size_t k = 0; k < sorted.size(); ++k) t.pix[base + k * 64 + l] = sorted[k];
+        }
+        t.tile_off.push_back(t.tile_off.back() + len);
+    }
+    t.seg_kind.resize((size_t)t.n_tiles * 64, 0);
+    return t;
+}
+
 } // namespace msf

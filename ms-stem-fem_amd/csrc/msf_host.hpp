@@ -108,4 +108,17 @@ struct Groups {
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
                     double k2, bool fem, double dr, double alpha_max);
 
+// Groups cut into segments of at most `seg` pixels, 64 segments per wavefront
+// tile, pixel offsets interleaved [tile][k][lane] (-1 pads short segments) so
+// that a wavefront's index loads are coalesced.  Segments of one group are
+// consecutive; group g owns segments [seg_of_group[g], seg_of_group[g+1]).
+struct SegmentTiles {
+    std::vector<int32_t> pix;         // interleaved pixel offsets
+    std::vector<int32_t> tile_off;    // n_tiles + 1, into pix (units of 64 entries)
+    std::vector<int32_t> seg_kind;    // per segment: 0 |psi|^2, 1 |psi|
+    std::vector<int32_t> seg_of_group;
+    int n_segments = 0, n_tiles = 0;
+};
+SegmentTiles build_segments(const Groups& g, int seg);
+
 } // namespace msf

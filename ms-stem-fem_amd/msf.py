@@ -54,7 +54,7 @@ class Fem(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
-                ("profile_kernels", C.c_int32)]
+                ("profile_kernels", C.c_int32), ("col_tile", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -181,8 +181,9 @@ class Context:
         if rc:
             raise MsfError(rc, f"{what}: {lib().msf_last_error(self._h).decode()}")
 
-    def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False):
-        cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0)
+    def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False,
+                  col_tile=0):
+        cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0, col_tile)
         self._check(lib().msf_set_config(self._h, C.byref(cfg)), "msf_set_config")
 
     def stats(self):
