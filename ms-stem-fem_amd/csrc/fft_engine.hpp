@@ -12,7 +12,8 @@
 // for butterfly j = t + T*m, m = 0..E/R-1.
 //
 // Twiddles come from a table of W_N^e = exp(-2 pi i e / N), e in [0, N),
-// built on the host in FP64 and rounded once to the working precision.
+// built on the host in FP64 and rounded once to the working precision, held
+// in LDS at padded positions tw_pad(e).
 // FFT sign conventions match FFTW: forward = exp(-2 pi i), inverse =
 // exp(+2 pi i), both unnormalised.
 #pragma once
@@ -142,6 +143,10 @@ template <bool INV, typename C> struct Dft<6, INV, C> {
 // Stockham write pattern across LDS banks
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 3); }
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3) + 1; }
+// twiddle tables in LDS: stage reads W^(q k N/(NS R)) step through the table
+// with power-of-two strides; one slot every 16 entries spreads them over banks
+__host__ __device__ constexpr int tw_pad(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int tw_len(int n) { return n + (n >> 4) + 1; }
 
 // ------------------------------------------------------------ sync
 // WAVE = true when all lanes of a transform (and its LDS region) belong to one
@@ -159,7 +164,7 @@ template <bool WAVE> __device__ __forceinline__ void lds_sync()
 }
 
 // ------------------------------------------------------------ one stage
-template <typename C, int N, int E, int R, int NS, bool INV, bool LAST, bool WAVE>
+template <typename C, int N, int E, int R, int NS, bool INV, bool LAST, bool WAVE, bool TWP>
 __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 {
     constexpr int T = N / E;
@@ -175,7 +180,8 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
             const int k = (t + T * m) % NS;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                const C w = tw[q * k * (N / (NS * R))];
+                const int e = q * k * (N / (NS * R));
+                const C w = tw[TWP ? tw_pad(e) : e];
                 v[q] = INV ? cmulc(v[q], w) : cmul(v[q], w);
             }
         }
@@ -203,12 +209,12 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
     }
 }
 
-template <typename C, int N, int E, bool INV, bool WAVE, int NS, int R, int... Rest>
+template <typename C, int N, int E, bool INV, bool WAVE, bool TWP, int NS, int R, int... Rest>
 __device__ __forceinline__ void run_stages(C (&x)[E], C* lds, const C* tw, int t)
 {
-    fft_stage<C, N, E, R, NS, INV, sizeof...(Rest) == 0, WAVE>(x, lds, tw, t);
+    fft_stage<C, N, E, R, NS, INV, sizeof...(Rest) == 0, WAVE, TWP>(x, lds, tw, t);
     if constexpr (sizeof...(Rest) > 0)
-        run_stages<C, N, E, INV, WAVE, NS * R, Rest...>(x, lds, tw, t);
+        run_stages<C, N, E, INV, WAVE, TWP, NS * R, Rest...>(x, lds, tw, t);
 }
 
 // ------------------------------------------------------------ plans
@@ -217,10 +223,10 @@ template <int N> struct Plan;
 #define MSF_PLAN(N_, E_, ...)                                                                  \
     template <> struct Plan<N_> {                                                              \
         static constexpr int N = N_, E = E_, T = N_ / E_;                                      \
-        template <bool INV, typename C, bool WAVE = false>                                     \
+        template <bool INV, typename C, bool WAVE = false, bool TWP = true>                    \
         static __device__ __forceinline__ void run(C (&x)[E_], C* lds, const C* tw, int t)     \
         {                                                                                      \
-            run_stages<C, N_, E_, INV, WAVE, 1, __VA_ARGS__>(x, lds, tw, t);                   \
+            run_stages<C, N_, E_, INV, WAVE, TWP, 1, __VA_ARGS__>(x, lds, tw, t);              \
         }                                                                                      \
     };
 MSF_PLAN(16, 8, 8, 2)

@@ -26,11 +26,14 @@ namespace msf {
 struct RowPassArgs {
     float2* psi;            // [B][nx][ncp] scattered wave phi
     const float2* t;        // [nx][ny] (t_re/s - 1/s, t_im/s) of this slice
-    const float2* u0;       // [nx][ny] vacuum wave entering this slice, s * real space
+    const float2* u0;       // [nx][2*ny] vacuum wave entering this slice, s * real space,
+                            // each row stored twice in a row
+
     const int2* shift;      // [B] probe shifts of the batch
     const float2* tw;       // [ny] forward roots
     float inv_s;            // 1/sqrtNxNy
     int nx, ncp, ka, kb, batch;
+    int prune;              // 1: columns [3ny/8, 5ny/8) are outside the kept band
 };
 
 struct ColPassArgs {
@@ -44,11 +47,16 @@ struct ColPassArgs {
     const float2* twy;      // [ny] forward roots (last pass only)
     int ny, ncp, ka, kb, batch;
     int tile;               // columns per workgroup (8 or 16; 0 = default)
+    int prune;              // 1: rows [3nx/8, 5nx/8) are masked for every column
 };
 
 bool slice_lengths_ok(int n);
 // the column count of the compact wave is padded to a multiple of this
 int col_tile(int nx);
+// true when the kept set leaves [3n/8, 5n/8) empty along the row (ka, kb) and
+// every row i in [3nx/8, 5nx/8) fully masked (keep ranges)
+bool row_prunable(int ny, int ka, int kb);
+bool col_prunable(int nx, const int32_t* keep_ab);
 
 hipError_t launch_row_pass(int ny, bool first, int transmit_mode, const RowPassArgs& a,
                            hipStream_t s);
@@ -112,6 +120,9 @@ hipError_t launch_vacuum_step(double2* a, int nx, int ny, const double2* px, con
                               const int2* keep, double scale, hipStream_t s);
 // out = (float2) a
 hipError_t launch_to_f32(const double2* a, int64_t n, float2* out, hipStream_t s);
+// out[i][j'] = (float2) a[i][j' mod ny], j' < 2*ny
+hipError_t launch_to_f32_doubled_rows(const double2* a, int nx, int ny, float2* out,
+                                      hipStream_t s);
 // a *= f / sqrt(*norm2)
 hipError_t launch_scale_f64(double2* a, int64_t n, const double* norm2, double f, hipStream_t s);
 

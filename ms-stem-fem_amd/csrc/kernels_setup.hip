@@ -25,9 +25,9 @@ __global__ void __launch_bounds__(256) fft64_rows_kernel(double2* data, int64_t 
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = 256 / T;
     constexpr int LEN = lds_len(NY);
-    __shared__ double2 s_tw[NY];
+    __shared__ double2 s_tw[tw_len(NY)];
     __shared__ double2 s_buf[RPW * LEN];
-    for (int e = threadIdx.x; e < NY; e += blockDim.x) s_tw[e] = tw[e];
+    for (int e = threadIdx.x; e < NY; e += blockDim.x) s_tw[tw_pad(e)] = tw[e];
     __syncthreads();
     const int lr = threadIdx.x / T, t = threadIdx.x % T;
     const int64_t r = (int64_t)blockIdx.x * RPW + lr;
@@ -51,9 +51,9 @@ __global__ void __launch_bounds__(256) fft64_cols_kernel(double2* data, int ny, 
     constexpr int E = P::E, T = P::T;
     constexpr int W = (256 / T) < 4 ? (256 / T) : 4;
     constexpr int LEN = lds_len(NX);
-    __shared__ double2 s_tw[NX];
+    __shared__ double2 s_tw[tw_len(NX)];
     __shared__ double2 s_buf[W * LEN];
-    for (int e = threadIdx.x; e < NX; e += blockDim.x) s_tw[e] = tw[e];
+    for (int e = threadIdx.x; e < NX; e += blockDim.x) s_tw[tw_pad(e)] = tw[e];
     __syncthreads();
     const int c = threadIdx.x % W, t = threadIdx.x / W;
     const int ntiles = ny / W;
@@ -372,6 +372,25 @@ __global__ void to_f32_kernel(const double2* a, int64_t n, float2* out)
 hipError_t launch_to_f32(const double2* a, int64_t n, float2* out, hipStream_t s)
 {
     hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, out);
+    return hipGetLastError();
+}
+
+__global__ void to_f32_doubled_kernel(const double2* a, int nx, int ny, float2* out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)nx * 2 * ny) return;
+    const int64_t i = k / (2 * ny);
+    const int j = (int)(k % (2 * ny));
+    const double2 v = a[i * ny + (j < ny ? j : j - ny)];
+    out[k] = float2{(float)v.x, (float)v.y};
+}
+
+hipError_t launch_to_f32_doubled_rows(const double2* a, int nx, int ny, float2* out,
+                                      hipStream_t s)
+{
+    const int64_t n = (int64_t)nx * 2 * ny;
+    hipLaunchKernelGGL(to_f32_doubled_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       a, nx, ny, out);
     return hipGetLastError();
 }
 

@@ -15,6 +15,14 @@
 
 namespace msf {
 
+// twiddle-table padding per pass (measured: it slows the row pass)
+#ifndef ROW_TWP
+#define ROW_TWP false
+#endif
+#ifndef COL_TWP
+#define COL_TWP true
+#endif
+
 // ------------------------------------------------------------------ row pass
 // Transmission of one row.  With t' = t/s = (tau + 1/s, ti) and the
 // reference's in-place update (TMODE 0: Im uses the updated Re), for the
@@ -22,6 +30,28 @@ namespace msf {
 //   a  = t're u_re - ti u_im,            b  = t're u_im + ti a
 //   Da = tau u0_re - ti u0_im,           Db = tau u0_im + ti (u0_re/s + Da)
 // and TMODE 1 (exact product) uses b = t're u_im + ti u_re, Db = tau u0_im + ti u0_re.
+// Lane element p (indices t + T*p) lies entirely inside the middle band
+// [3N/8, 5N/8), which the 2/3 band limit leaves empty in k-space (checked on
+// the host before a pruned variant is launched).
+template <int N, int E> __host__ __device__ constexpr bool in_zero_band(int p)
+{
+    constexpr int T = N / E;
+    return T * p >= (3 * N + 7) / 8 && T * (p + 1) <= (5 * N) / 8;
+}
+
+// With PRUNE the host also guarantees ka >= N/4 and kb >= N/4: lane elements
+// entirely below N/4 are always in the low kept band, those entirely at or
+// above 3N/4 always in the high band; only the elements straddling a band edge
+// need a run-time test.
+template <int N, int E> __host__ __device__ constexpr bool always_low(int p)
+{
+    return (N / E) * (p + 1) <= N / 4;
+}
+template <int N, int E> __host__ __device__ constexpr bool always_high(int p)
+{
+    return (N / E) * p >= N - N / 4;
+}
+
 template <int N> __device__ __forceinline__ int wrap_sub(int j, int s)
 {
     if constexpr ((N & (N - 1)) == 0) {
@@ -43,14 +73,15 @@ __device__ __forceinline__ int xcd_band(int L, int G)
     return (L % kXcd) * (G / kXcd) + L / kXcd;
 }
 
-template <int NY, bool FIRST, int TMODE>
-__global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
+template <int NY, bool FIRST, int TMODE, bool PRUNE, bool TWP = ROW_TWP>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+row_pass_kernel(RowPassArgs a)
 {
     using P = Plan<NY>;
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = 256 / T;
     constexpr int LEN = lds_len(NY);
-    __shared__ float2 s_tw[NY];
+    __shared__ float2 s_tw[TWP ? tw_len(NY) : NY];
     __shared__ float2 s_t[NY];
     __shared__ float2 s_buf[RPW * LEN];
 
@@ -63,12 +94,13 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
     // twiddles and the transmission row i, shared by the RPW rows of the group
     const float2* trow = a.t + (int64_t)i * NY;
     for (int e = threadIdx.x; e < NY; e += blockDim.x) {
-        s_tw[e] = a.tw[e];
+        s_tw[TWP ? tw_pad(e) : e] = a.tw[e];
         s_t[e] = trow[e];
     }
     __syncthreads();
 
-    const int lr = threadIdx.x / T;
+    // with T == 64 each row is one wavefront: keep its index in a scalar register
+    const int lr = (T == 64) ? __builtin_amdgcn_readfirstlane(threadIdx.x / T) : threadIdx.x / T;
     const int t = threadIdx.x % T;
     const bool active = b0 + lr < a.batch;
     const int b = active ? b0 + lr : a.batch - 1;
@@ -81,25 +113,35 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
     if constexpr (!FIRST) {
 #pragma unroll
         for (int p = 0; p < E; ++p) {
-            const int j = t + T * p;
-            const bool in = (j < a.ka) || (j >= hi);
-            // lanes outside the kept band all read element 0 (one shared line)
-            const float2 v = row[in ? (j < a.ka ? j : j + off) : 0];
-            x[p] = in ? v : float2{0.f, 0.f};
+            const uint32_t j = (uint32_t)(t + T * p);
+            if (PRUNE && in_zero_band<NY, E>(p)) {
+                x[p] = float2{0.f, 0.f};
+            } else if (PRUNE && always_low<NY, E>(p)) {
+                x[p] = row[j];
+            } else if (PRUNE && always_high<NY, E>(p)) {
+                x[p] = row[j + (uint32_t)off];
+            } else {
+                const bool lo = (int)j < a.ka, up = (int)j >= hi;
+                // lanes outside the kept band all read element 0 (one shared line)
+                const float2 v = row[lo ? j : (up ? j + (uint32_t)off : 0u)];
+                x[p] = (lo || up) ? v : float2{0.f, 0.f};
+            }
         }
-        P::template run<true, float2, true>(x, lds, s_tw, t);
+        P::template run<true, float2, true, TWP>(x, lds, s_tw, t);
     }
 
     const int2 sh = a.shift[b];
     int ii = i - sh.x;
     if (ii < 0) ii += a.nx;
-    const float2* urow = a.u0 + (int64_t)ii * NY;
+    // u0 rows are stored twice over (length 2*NY): the circular shift by sy is
+    // a plain offset
+    const float2* urow = a.u0 + (int64_t)ii * (2 * NY) + (NY - sh.y) + t;
     const float is = a.inv_s;
 #pragma unroll
     for (int p = 0; p < E; ++p) {
         const int j = t + T * p;
         const float2 tv = s_t[j];
-        const float2 u0 = urow[wrap_sub<NY>(j, sh.y)];
+        const float2 u0 = urow[T * p];
         const float da = tv.x * u0.x - tv.y * u0.y;
         const float db = (TMODE == 0) ? tv.x * u0.y + tv.y * (u0.x * is + da) : tv.x * u0.y + tv.y * u0.x;
         if constexpr (FIRST) {
@@ -113,14 +155,21 @@ __global__ void __launch_bounds__(256) row_pass_kernel(RowPassArgs a)
         }
     }
 
-    P::template run<false, float2, true>(x, lds, s_tw, t);
+    P::template run<false, float2, true, TWP>(x, lds, s_tw, t);
 
     if (active) {
 #pragma unroll
         for (int p = 0; p < E; ++p) {
-            const int j = t + T * p;
-            if (j < a.ka) row[j] = x[p];
-            else if (j >= hi) row[j + off] = x[p];
+            const uint32_t j = (uint32_t)(t + T * p);
+            if (PRUNE && in_zero_band<NY, E>(p)) continue;
+            if (PRUNE && always_low<NY, E>(p)) {
+                row[j] = x[p];
+            } else if (PRUNE && always_high<NY, E>(p)) {
+                row[j + (uint32_t)off] = x[p];
+            } else {
+                if ((int)j < a.ka) row[j] = x[p];
+                else if ((int)j >= hi) row[j + (uint32_t)off] = x[p];
+            }
         }
     }
 }
@@ -139,48 +188,57 @@ template <int NX> struct ColTile {
     static constexpr int W = (16 * lds_len(NX) * 8 > 48 * 1024) ? 8 : 16;
 };
 
-// column stride of the LDS tile: the coalesced load/store phases touch W
-// columns x (64/W) rows per wavefront; a stride congruent to 64/W modulo 16
-// (in 8-byte words) spreads each 16-lane group over all banks
+// column stride of the LDS tile: a 16-lane group of the coalesced load/store
+// phases covers W' = min(W, 16) columns x 16/W' rows; a stride congruent to
+// 16/W' modulo 16 (8-byte words) puts those 16 lanes on distinct bank pairs
 template <int NX, int W> __host__ __device__ constexpr int col_len()
 {
     constexpr int base = lds_len(NX);
-    constexpr int want = (64 / W) % 16;
+    constexpr int want = (16 / (W < 16 ? W : 16)) % 16;
     return base + ((want - base % 16) + 16) % 16;
 }
 
-template <int NX, bool LAST, int W>
+template <int NX, bool LAST, int W, bool PRUNE, bool TWP = COL_TWP>
 __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 {
     using P = Plan<NX>;
     constexpr int E = P::E, T = P::T;
     constexpr int LEN = col_len<NX, W>();
     static_assert(64 % T == 0, "a column transform must fit in one wavefront");
-    __shared__ float2 s_tw[NX];
+    __shared__ float2 s_tw[TWP ? tw_len(NX) : NX];
     __shared__ float2 s_px[NX];
     __shared__ int2 s_keep[NX];
     __shared__ float2 s_buf[W * LEN];
 
     for (int e = threadIdx.x; e < NX; e += blockDim.x) {
-        s_tw[e] = a.tw[e];
+        s_tw[TWP ? tw_pad(e) : e] = a.tw[e];
         s_px[e] = a.px[e];
         s_keep[e] = a.keep[e];
     }
 
+    // neighbouring tiles share 128-byte lines (W = 8 reads 64 B per row): keep
+    // them on one XCD, next to each other in dispatch order
     const int ntiles = a.ncp / W;
-    const int b = blockIdx.x / ntiles;
-    const int j0 = (blockIdx.x % ntiles) * W;
+    const int g = xcd_band((int)blockIdx.x, (int)gridDim.x);
+    const int b = g / ntiles;
+    const int j0 = (g - b * ntiles) * W;
     const int nc = a.ka + a.kb;
     float2* tile = a.psi + (int64_t)b * NX * a.ncp + j0;
 
-    {   // coalesced load of the tile
+    {   // coalesced load of the tile (32-bit offsets from the uniform tile base).
+        // Every load is in bounds (b < batch by the grid, padding columns
+        // jc >= nc exist in the buffer), so all E loads are issued before the
+        // first LDS write and overlap; padding columns are zeroed after.
         const int c = threadIdx.x % W, r0 = threadIdx.x / W;
-        const bool ok = (b < a.batch) && (j0 + c < nc);
+        const bool ok = j0 + c < nc;
+        const uint32_t o0 = (uint32_t)(r0 * a.ncp + c);
+        const uint32_t step = (uint32_t)(T * a.ncp);
+        float2 v[E];
 #pragma unroll
-        for (int p = 0; p < E; ++p) {
-            const int i = r0 + T * p;
-            s_buf[c * LEN + lds_pad(i)] = ok ? tile[(int64_t)i * a.ncp + c] : float2{0.f, 0.f};
-        }
+        for (int p = 0; p < E; ++p) v[p] = tile[o0 + (uint32_t)p * step];
+#pragma unroll
+        for (int p = 0; p < E; ++p)
+            s_buf[c * LEN + lds_pad(r0 + T * p)] = ok ? v[p] : float2{0.f, 0.f};
     }
     __syncthreads();
 
@@ -194,13 +252,17 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 #pragma unroll
     for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
 
-    P::template run<false, float2, true>(x, lds, s_tw, t);
+    P::template run<false, float2, true, TWP>(x, lds, s_tw, t);
 
     // band limit (bw_limit + propagate's k^2 test, decided on the host) and
     // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
     const float2 pyj = active ? a.py[j] : float2{0.f, 0.f};
 #pragma unroll
     for (int p = 0; p < E; ++p) {
+        if (PRUNE && in_zero_band<NX, E>(p)) {
+            x[p] = float2{0.f, 0.f};
+            continue;
+        }
         const int i = t + T * p;
         const int2 kr = s_keep[i];
         const bool keep = (j < kr.x) || (j >= a.ny - kr.y);
@@ -211,15 +273,17 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
     }
 
     if constexpr (!LAST) {
-        P::template run<true, float2, true>(x, lds, s_tw, t);
+        P::template run<true, float2, true, TWP>(x, lds, s_tw, t);
     } else if (active) {
         // add the vacuum exit wave, shifted: psi0(k) e^{-2 pi i (i sx / nx + j sy / ny)}
         const int2 sh = a.shift[b];
-        const float2 wy = a.twy[(int)(((int64_t)j * sh.y) % a.ny)];
+        const float2 wy = a.twy[((uint32_t)j * (uint32_t)sh.y) % (uint32_t)a.ny];
 #pragma unroll
         for (int p = 0; p < E; ++p) {
-            const int i = t + T * p;
-            const float2 wx = s_tw[(int)(((int64_t)i * sh.x) % NX)];
+            if (PRUNE && in_zero_band<NX, E>(p)) continue;
+            const uint32_t i = (uint32_t)(t + T * p);
+            const int e = (int)((i * (uint32_t)sh.x) % (uint32_t)NX);
+            const float2 wx = s_tw[TWP ? tw_pad(e) : e];
             const float2 w = float2{wx.x * wy.x - wx.y * wy.y, wx.x * wy.y + wx.y * wy.x};
             const float2 v0 = a.psi0[(int64_t)i * a.ny + j];
             x[p].x += v0.x * w.x - v0.y * w.y;
@@ -235,11 +299,13 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
     {   // coalesced store of the tile
         const int c2 = threadIdx.x % W, r0 = threadIdx.x / W;
         const bool ok = (b < a.batch) && (j0 + c2 < nc);
+        const uint32_t o0 = (uint32_t)(r0 * a.ncp + c2);
+        const uint32_t step = (uint32_t)(T * a.ncp);
         if (ok) {
 #pragma unroll
             for (int p = 0; p < E; ++p) {
                 const int i = r0 + T * p;
-                tile[(int64_t)i * a.ncp + c2] = s_buf[c2 * LEN + lds_pad(i)];
+                tile[o0 + (uint32_t)p * step] = s_buf[c2 * LEN + lds_pad(i)];
             }
         }
     }
@@ -255,6 +321,20 @@ bool slice_lengths_ok(int n)
     default:
         return false;
     }
+}
+
+bool row_prunable(int ny, int ka, int kb)
+{
+    const int lo = (3 * ny + 7) / 8, hi = (5 * ny) / 8;
+    return ka <= lo && ny - kb >= hi && ka >= ny / 4 && kb >= ny / 4;
+}
+
+bool col_prunable(int nx, const int32_t* ab)
+{
+    const int lo = (3 * nx + 7) / 8, hi = (5 * nx) / 8;
+    for (int i = lo; i < hi; ++i)
+        if (ab[2 * i] != 0 || ab[2 * i + 1] != 0) return false;
+    return true;
 }
 
 int col_tile(int nx)
@@ -276,13 +356,15 @@ static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStr
     const int64_t groups = (a.batch + RPW - 1) / RPW;
     const dim3 grid((unsigned)(groups * a.nx));
     const dim3 block(RPW * Plan<NY>::T);
-    if (first) {
-        if (tmode == 0) hipLaunchKernelGGL((row_pass_kernel<NY, true, 0>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((row_pass_kernel<NY, true, 1>), grid, block, 0, s, a);
+#define MSF_ROW(F, M, PR) hipLaunchKernelGGL((row_pass_kernel<NY, F, M, PR>), grid, block, 0, s, a)
+    if (a.prune) {
+        if (first) { if (tmode == 0) MSF_ROW(true, 0, true); else MSF_ROW(true, 1, true); }
+        else { if (tmode == 0) MSF_ROW(false, 0, true); else MSF_ROW(false, 1, true); }
     } else {
-        if (tmode == 0) hipLaunchKernelGGL((row_pass_kernel<NY, false, 0>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((row_pass_kernel<NY, false, 1>), grid, block, 0, s, a);
+        if (first) { if (tmode == 0) MSF_ROW(true, 0, false); else MSF_ROW(true, 1, false); }
+        else { if (tmode == 0) MSF_ROW(false, 0, false); else MSF_ROW(false, 1, false); }
     }
+#undef MSF_ROW
     return hipGetLastError();
 }
 
@@ -304,8 +386,13 @@ static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s)
     if (a.ncp % W) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((int64_t)a.batch * (a.ncp / W)));
     const dim3 block(W * Plan<NX>::T);
-    if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true, W>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((col_pass_kernel<NX, false, W>), grid, block, 0, s, a);
+    if (a.prune) {
+        if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true, W, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((col_pass_kernel<NX, false, W, true>), grid, block, 0, s, a);
+    } else {
+        if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true, W, false>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((col_pass_kernel<NX, false, W, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

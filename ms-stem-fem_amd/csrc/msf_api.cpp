@@ -297,12 +297,13 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         const double2 *twx64, *twy64;
         if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
         if (int r = twiddles(c, ny, nullptr, &twy64)) return r;
-        HIPCHK(c, pl.U0.alloc((size_t)S * M * sizeof(float2)));
+        HIPCHK(c, pl.U0.alloc((size_t)S * 2 * M * sizeof(float2)));
         HIPCHK(c, pl.Psi0.alloc((size_t)M * sizeof(float2)));
         double2* v = h.as<double2>();
         HIPCHK(c, msf::launch_scale_f64(v, M, norm2.as<double>(), d.sqrt_nxny, c->stream));
         for (int n = 0; n < S; ++n) {
-            HIPCHK(c, msf::launch_to_f32(v, M, pl.U0.as<float2>() + (int64_t)n * M, c->stream));
+            HIPCHK(c, msf::launch_to_f32_doubled_rows(v, nx, ny, pl.U0.as<float2>() + (int64_t)n * 2 * M,
+                                                      c->stream));
             HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, false, twx64, twy64, c->stream));
             HIPCHK(c, msf::launch_vacuum_step(v, nx, ny, sp->px64.as<double2>() + (int64_t)n * nx,
                                               sp->py64.as<double2>() + (int64_t)n * ny,
@@ -383,6 +384,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ra.kb = kb;
     ra.tw = twy;
     ra.inv_s = (float)(1.0 / d.sqrt_nxny);
+    ra.prune = msf::row_prunable(ny, ka, kb) ? 1 : 0;
     msf::ColPassArgs ca{};
     ca.ny = ny;
     ca.ncp = ncp;
@@ -393,6 +395,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ca.psi0 = dPsi0.as<float2>();
     ca.twy = twy;
     ca.tile = c->cfg.col_tile;
+    ca.prune = msf::col_prunable(nx, keep.ab.data()) ? 1 : 0;
     const float2* t32 = sp->t32.as<float2>();
     const float2* px = sp->px.as<float2>();
     const float2* py = sp->py.as<float2>();
@@ -406,7 +409,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         ca.shift = dshift.as<int2>() + b0;
         for (int n = 0; n < S; ++n) {
             ra.t = t32 + (int64_t)n * M;
-            ra.u0 = dU0.as<float2>() + (int64_t)n * M;
+            ra.u0 = dU0.as<float2>() + (int64_t)n * 2 * M;
             HIPCHK(c, ev_begin(n == 0 ? K_ROW0 : K_ROW));
             HIPCHK(c, msf::launch_row_pass(ny, n == 0, tmode, ra, c->stream));
             HIPCHK(c, ev_end());
