@@ -11,9 +11,8 @@
 //     y[(j/NS)*NS*R + j%NS + q*NS] = sum_r x[j + r*N/R] W_{NS R}^{r (j%NS)} W_R^{rq}
 // for butterfly j = t + T*m, m = 0..E/R-1.
 //
-// Twiddles come from a table of W_N^e = exp(-2 pi i e / N), e in [0, N),
-// built on the host in FP64 and rounded once to the working precision, held
-// in LDS at padded positions tw_pad(e).
+// Twiddles come from per-stage tables (see fft_stage) built on the host in
+// FP64 and rounded once to the working precision, held in LDS.
 // FFT sign conventions match FFTW: forward = exp(-2 pi i), inverse =
 // exp(+2 pi i), both unnormalised.
 #pragma once
@@ -164,7 +163,11 @@ template <bool WAVE> __device__ __forceinline__ void lds_sync()
 }
 
 // ------------------------------------------------------------ one stage
-template <typename C, int N, int E, int R, int NS, bool INV, bool LAST, bool WAVE, bool TWP>
+// Twiddles of stage s (radix R, span NS) come from a per-stage table laid out
+// [q-1][k], entry W_{NS R}^{q k}, q = 1..R-1, k = 0..NS-1, at offset TWO in
+// the concatenated table of all stages (stage_twiddles): lanes with
+// consecutive k read consecutive words, and the q offsets are immediates.
+template <typename C, int N, int E, int R, int NS, int TWO, bool INV, bool LAST, bool WAVE>
 __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 {
     constexpr int T = N / E;
@@ -177,11 +180,10 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 #pragma unroll
         for (int q = 0; q < R; ++q) v[q] = x[m + q * BPT];
         if constexpr (NS > 1) {
-            const int k = (t + T * m) % NS;
+            const C* twk = tw + TWO + (t + T * m) % NS;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
-                const int e = q * k * (N / (NS * R));
-                const C w = tw[TWP ? tw_pad(e) : e];
+                const C w = twk[(q - 1) * NS];
                 v[q] = INV ? cmulc(v[q], w) : cmul(v[q], w);
             }
         }
@@ -209,24 +211,30 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
     }
 }
 
-template <typename C, int N, int E, bool INV, bool WAVE, bool TWP, int NS, int R, int... Rest>
+// stage offsets in the concatenated twiddle table: stage s with span NS > 1
+// adds (R-1)*NS entries; the first stage (NS = 1) needs none
+template <typename C, int N, int E, bool INV, bool WAVE, int NS, int TWO, int R, int... Rest>
 __device__ __forceinline__ void run_stages(C (&x)[E], C* lds, const C* tw, int t)
 {
-    fft_stage<C, N, E, R, NS, INV, sizeof...(Rest) == 0, WAVE, TWP>(x, lds, tw, t);
+    fft_stage<C, N, E, R, NS, TWO, INV, sizeof...(Rest) == 0, WAVE>(x, lds, tw, t);
     if constexpr (sizeof...(Rest) > 0)
-        run_stages<C, N, E, INV, WAVE, TWP, NS * R, Rest...>(x, lds, tw, t);
+        run_stages<C, N, E, INV, WAVE, NS * R, (NS > 1 ? TWO + (R - 1) * NS : 0), Rest...>(
+            x, lds, tw, t);
 }
 
 // ------------------------------------------------------------ plans
-// E = values per lane; radices applied first to last.
+// E = values per lane; radices applied first to last.  kRadix lists them for
+// the host code that builds the twiddle tables.
 template <int N> struct Plan;
 #define MSF_PLAN(N_, E_, ...)                                                                  \
     template <> struct Plan<N_> {                                                              \
         static constexpr int N = N_, E = E_, T = N_ / E_;                                      \
-        template <bool INV, typename C, bool WAVE = false, bool TWP = true>                    \
+        static constexpr int kRadix[] = {__VA_ARGS__};                                         \
+        static constexpr int kStages = sizeof(kRadix) / sizeof(int);                           \
+        template <bool INV, typename C, bool WAVE = false>                                     \
         static __device__ __forceinline__ void run(C (&x)[E_], C* lds, const C* tw, int t)     \
         {                                                                                      \
-            run_stages<C, N_, E_, INV, WAVE, TWP, 1, __VA_ARGS__>(x, lds, tw, t);              \
+            run_stages<C, N_, E_, INV, WAVE, 1, 0, __VA_ARGS__>(x, lds, tw, t);                \
         }                                                                                      \
     };
 MSF_PLAN(16, 8, 8, 2)
@@ -241,5 +249,8 @@ MSF_PLAN(512, 8, 8, 8, 8)
 MSF_PLAN(768, 24, 6, 8, 4, 4)
 MSF_PLAN(1024, 16, 8, 8, 4, 4)
 #undef MSF_PLAN
+
+// entries of the concatenated stage tables: sum (R-1) NS over stages 1.. = N - R_0
+template <int N> constexpr int stage_tw_len() { return N - Plan<N>::kRadix[0]; }
 
 } // namespace msf

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace msf {
 
@@ -30,7 +31,7 @@ struct RowPassArgs {
                             // each row stored twice in a row
 
     const int2* shift;      // [B] probe shifts of the batch
-    const float2* tw;       // [ny] forward roots
+    const float2* tw;       // stage twiddle table of length ny (stage_twiddles)
     float inv_s;            // 1/sqrtNxNy
     int nx, ncp, ka, kb, batch;
     int prune;              // 1: columns [3ny/8, 5ny/8) are outside the kept band
@@ -41,16 +42,19 @@ struct ColPassArgs {
     const float2* px;       // [nx] propagator along x of this slice
     const float2* py;       // [ny] propagator along y of this slice, scaled by 1/s
     const int2* keep;       // [nx] (a_i, b_i) kept-column ranges of row i
-    const float2* tw;       // [nx] forward roots
+    const float2* tw;       // stage twiddle table of length nx (stage_twiddles)
     const float2* psi0;     // [nx][ny] vacuum exit wave in k-space (last pass only)
     const int2* shift;      // [B] probe shifts (last pass only)
-    const float2* twy;      // [ny] forward roots (last pass only)
+    const float2* twx_nat;  // [nx] roots W_nx^e (last pass only)
+    const float2* twy;      // [ny] roots W_ny^e (last pass only)
     int ny, ncp, ka, kb, batch;
     int tile;               // columns per workgroup (8 or 16; 0 = default)
     int prune;              // 1: rows [3nx/8, 5nx/8) are masked for every column
 };
 
 bool slice_lengths_ok(int n);
+// concatenated per-stage twiddle tables of the length-n plan, (re, im) FP64
+bool stage_twiddles(int n, std::vector<double>& re_im);
 // the column count of the compact wave is padded to a multiple of this
 int col_tile(int nx);
 // true when the kept set leaves [3n/8, 5n/8) empty along the row (ka, kb) and
@@ -85,7 +89,8 @@ hipError_t launch_expand_exit(const float2* psi, int nx, int ny, int ncp, int ka
 
 // ---------------------------------------------------------------- FP64 set-up
 bool setup_lengths_ok(int n);
-// in-place 2-D FFT of `count` planes of nx x ny complex doubles
+// in-place 2-D FFT of `count` planes of nx x ny complex doubles (twx, twy:
+// stage twiddle tables)
 hipError_t fft2d_f64(double2* data, int nx, int ny, int count, bool inverse, const double2* twx,
                      const double2* twy, hipStream_t s);
 // probe H(k) of probe_wavefunction.cpp:37-239 inside the aperture ranges

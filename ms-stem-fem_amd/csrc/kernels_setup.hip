@@ -25,9 +25,9 @@ __global__ void __launch_bounds__(256) fft64_rows_kernel(double2* data, int64_t 
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = 256 / T;
     constexpr int LEN = lds_len(NY);
-    __shared__ double2 s_tw[tw_len(NY)];
+    __shared__ double2 s_tw[stage_tw_len<NY>()];
     __shared__ double2 s_buf[RPW * LEN];
-    for (int e = threadIdx.x; e < NY; e += blockDim.x) s_tw[tw_pad(e)] = tw[e];
+    for (int e = threadIdx.x; e < stage_tw_len<NY>(); e += blockDim.x) s_tw[e] = tw[e];
     __syncthreads();
     const int lr = threadIdx.x / T, t = threadIdx.x % T;
     const int64_t r = (int64_t)blockIdx.x * RPW + lr;
@@ -51,9 +51,9 @@ __global__ void __launch_bounds__(256) fft64_cols_kernel(double2* data, int ny, 
     constexpr int E = P::E, T = P::T;
     constexpr int W = (256 / T) < 4 ? (256 / T) : 4;
     constexpr int LEN = lds_len(NX);
-    __shared__ double2 s_tw[tw_len(NX)];
+    __shared__ double2 s_tw[stage_tw_len<NX>()];
     __shared__ double2 s_buf[W * LEN];
-    for (int e = threadIdx.x; e < NX; e += blockDim.x) s_tw[tw_pad(e)] = tw[e];
+    for (int e = threadIdx.x; e < stage_tw_len<NX>(); e += blockDim.x) s_tw[e] = tw[e];
     __syncthreads();
     const int c = threadIdx.x % W, t = threadIdx.x / W;
     const int ntiles = ny / W;

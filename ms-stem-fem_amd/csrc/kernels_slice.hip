@@ -10,18 +10,14 @@
 //                 (slice.cpp:1366-1421) + the next slice's BACKWARD FFT (x half).
 //   reductions  = sum_detected_intensity (func.cpp:2593-2641) and
 //                 integrate_out_phi_fftw (variance.cpp:64-300), FP64.
+#include <cmath>
+#include <vector>
+
 #include "fft_engine.hpp"
 #include "kernels.hpp"
 
 namespace msf {
 
-// twiddle-table padding per pass (measured: it slows the row pass)
-#ifndef ROW_TWP
-#define ROW_TWP false
-#endif
-#ifndef COL_TWP
-#define COL_TWP true
-#endif
 
 // ------------------------------------------------------------------ row pass
 // Transmission of one row.  With t' = t/s = (tau + 1/s, ti) and the
@@ -68,12 +64,17 @@ template <int N> __device__ __forceinline__ int wrap_sub(int j, int s)
 // XCD's L2 once.
 __device__ __forceinline__ int xcd_band(int L, int G)
 {
+    // Workgroups are dispatched round-robin over the 8 XCDs (L % 8).  Give
+    // XCD x the contiguous logical range [x*q + min(x, r), ... + q + (x < r))
+    // with G = 8q + r, so neighbouring tiles (which share cache lines) meet
+    // in one L2.  A bijection on [0, G) for every G.
     constexpr int kXcd = 8;
-    if (G % kXcd) return L;
-    return (L % kXcd) * (G / kXcd) + L / kXcd;
+    const int q = G / kXcd, r = G % kXcd;
+    const int x = L % kXcd, k = L / kXcd;
+    return x * q + min(x, r) + k;
 }
 
-template <int NY, bool FIRST, int TMODE, bool PRUNE, bool TWP = ROW_TWP>
+template <int NY, bool FIRST, int TMODE, bool PRUNE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 row_pass_kernel(RowPassArgs a)
 {
@@ -81,7 +82,7 @@ row_pass_kernel(RowPassArgs a)
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = 256 / T;
     constexpr int LEN = lds_len(NY);
-    __shared__ float2 s_tw[TWP ? tw_len(NY) : NY];
+    __shared__ float2 s_tw[stage_tw_len<NY>()];
     __shared__ float2 s_t[NY];
     __shared__ float2 s_buf[RPW * LEN];
 
@@ -94,7 +95,7 @@ row_pass_kernel(RowPassArgs a)
     // twiddles and the transmission row i, shared by the RPW rows of the group
     const float2* trow = a.t + (int64_t)i * NY;
     for (int e = threadIdx.x; e < NY; e += blockDim.x) {
-        s_tw[TWP ? tw_pad(e) : e] = a.tw[e];
+        if (e < stage_tw_len<NY>()) s_tw[e] = a.tw[e];
         s_t[e] = trow[e];
     }
     __syncthreads();
@@ -127,7 +128,7 @@ row_pass_kernel(RowPassArgs a)
                 x[p] = (lo || up) ? v : float2{0.f, 0.f};
             }
         }
-        P::template run<true, float2, true, TWP>(x, lds, s_tw, t);
+        P::template run<true, float2, true>(x, lds, s_tw, t);
     }
 
     const int2 sh = a.shift[b];
@@ -155,7 +156,7 @@ row_pass_kernel(RowPassArgs a)
         }
     }
 
-    P::template run<false, float2, true, TWP>(x, lds, s_tw, t);
+    P::template run<false, float2, true>(x, lds, s_tw, t);
 
     if (active) {
 #pragma unroll
@@ -198,20 +199,22 @@ template <int NX, int W> __host__ __device__ constexpr int col_len()
     return base + ((want - base % 16) + 16) % 16;
 }
 
-template <int NX, bool LAST, int W, bool PRUNE, bool TWP = COL_TWP>
+template <int NX, bool LAST, int W, bool PRUNE>
 __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 {
     using P = Plan<NX>;
     constexpr int E = P::E, T = P::T;
     constexpr int LEN = col_len<NX, W>();
     static_assert(64 % T == 0, "a column transform must fit in one wavefront");
-    __shared__ float2 s_tw[TWP ? tw_len(NX) : NX];
+    __shared__ float2 s_tw[stage_tw_len<NX>()];
+    __shared__ float2 s_nat[LAST ? NX : 1]; // roots W_nx^e for the exit phase ramp
     __shared__ float2 s_px[NX];
     __shared__ int2 s_keep[NX];
     __shared__ float2 s_buf[W * LEN];
 
     for (int e = threadIdx.x; e < NX; e += blockDim.x) {
-        s_tw[TWP ? tw_pad(e) : e] = a.tw[e];
+        if (e < stage_tw_len<NX>()) s_tw[e] = a.tw[e];
+        if constexpr (LAST) s_nat[e] = a.twx_nat[e];
         s_px[e] = a.px[e];
         s_keep[e] = a.keep[e];
     }
@@ -252,7 +255,7 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 #pragma unroll
     for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
 
-    P::template run<false, float2, true, TWP>(x, lds, s_tw, t);
+    P::template run<false, float2, true>(x, lds, s_tw, t);
 
     // band limit (bw_limit + propagate's k^2 test, decided on the host) and
     // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
@@ -273,7 +276,7 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
     }
 
     if constexpr (!LAST) {
-        P::template run<true, float2, true, TWP>(x, lds, s_tw, t);
+        P::template run<true, float2, true>(x, lds, s_tw, t);
     } else if (active) {
         // add the vacuum exit wave, shifted: psi0(k) e^{-2 pi i (i sx / nx + j sy / ny)}
         const int2 sh = a.shift[b];
@@ -282,8 +285,7 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
         for (int p = 0; p < E; ++p) {
             if (PRUNE && in_zero_band<NX, E>(p)) continue;
             const uint32_t i = (uint32_t)(t + T * p);
-            const int e = (int)((i * (uint32_t)sh.x) % (uint32_t)NX);
-            const float2 wx = s_tw[TWP ? tw_pad(e) : e];
+            const float2 wx = s_nat[(i * (uint32_t)sh.x) % (uint32_t)NX];
             const float2 w = float2{wx.x * wy.x - wx.y * wy.y, wx.x * wy.y + wx.y * wy.x};
             const float2 v0 = a.psi0[(int64_t)i * a.ny + j];
             x[p].x += v0.x * w.x - v0.y * w.y;
@@ -318,6 +320,36 @@ bool slice_lengths_ok(int n)
     case 16: case 32: case 64: case 96: case 128: case 192: case 256: case 384: case 512:
     case 768: case 1024:
         return true;
+    default:
+        return false;
+    }
+}
+
+template <int N> static void stage_table(std::vector<double>& out)
+{
+    out.clear();
+    int ns = 1;
+    for (int s = 0; s < Plan<N>::kStages; ++s) {
+        const int r = Plan<N>::kRadix[s];
+        if (ns > 1)
+            for (int q = 1; q < r; ++q)
+                for (int k = 0; k < ns; ++k) {
+                    const long m = (long)ns * r;
+                    const double a = 2.0 * M_PI * (double)(((long)q * k) % m) / (double)m;
+                    out.push_back(std::cos(a));
+                    out.push_back(-std::sin(a));
+                }
+        ns *= r;
+    }
+}
+
+bool stage_twiddles(int n, std::vector<double>& re_im)
+{
+    switch (n) {
+#define MSF_CASE(N) case N: stage_table<N>(re_im); return (int)re_im.size() == 2 * stage_tw_len<N>();
+        MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
+        MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
+#undef MSF_CASE
     default:
         return false;
     }

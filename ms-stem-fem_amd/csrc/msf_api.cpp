@@ -82,7 +82,7 @@ struct msf_ctx {
     std::string err;
     msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0};
     msf_stats stats{};
-    std::map<int, DevBuf> tw32, tw64;
+    std::map<int, DevBuf> tw32, tw64, twnat;
     RunPlan plan;
 };
 
@@ -127,23 +127,33 @@ template <typename T> hipError_t upload(DevBuf& b, const std::vector<T>& v, hipS
     return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
 }
 
-// forward roots W_N^e = exp(-2 pi i e / N), FP64 then rounded
-int twiddles(msf_ctx* c, int n, const float2** f32, const double2** f64)
+// Per-stage twiddle tables of the length-n transform plan (FP64 and FP32)
+// and the natural roots W_n^e = exp(-2 pi i e / n) in FP32, built in FP64 on
+// the host and rounded once.
+int twiddles(msf_ctx* c, int n, const float2** f32, const double2** f64,
+             const float2** nat32 = nullptr)
 {
     if (!c->tw32.count(n)) {
-        std::vector<double2> d(n);
-        std::vector<float2> f(n);
+        std::vector<double> st;
+        if (!msf::stage_twiddles(n, st)) return fail(c, MSF_ERR_UNSUPPORTED, "no plan for this length");
+        std::vector<double2> d(st.size() / 2);
+        std::vector<float2> f(st.size() / 2), nat(n);
+        for (size_t e = 0; e < d.size(); ++e) {
+            d[e] = double2{st[2 * e], st[2 * e + 1]};
+            f[e] = float2{(float)st[2 * e], (float)st[2 * e + 1]};
+        }
         for (int e = 0; e < n; ++e) {
             const double a = 2.0 * M_PI * (double)e / (double)n;
-            d[e] = double2{std::cos(a), -std::sin(a)};
-            f[e] = float2{(float)d[e].x, (float)d[e].y};
+            nat[e] = float2{(float)std::cos(a), (float)-std::sin(a)};
         }
         HIPCHK(c, upload(c->tw64[n], d, c->stream));
         HIPCHK(c, upload(c->tw32[n], f, c->stream));
+        HIPCHK(c, upload(c->twnat[n], nat, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     if (f32) *f32 = c->tw32[n].as<float2>();
     if (f64) *f64 = c->tw64[n].as<double2>();
+    if (nat32) *nat32 = c->twnat[n].as<float2>();
     return MSF_OK;
 }
 
@@ -321,11 +331,19 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     pl.ncp = ncp;
     pl.valid = true;
 
-    const float2 *twx, *twy;
-    if (int r = twiddles(c, nx, &twx, nullptr)) return r;
-    if (int r = twiddles(c, ny, &twy, nullptr)) return r;
+    const float2 *twx, *twy, *twx_nat, *twy_nat;
+    if (int r = twiddles(c, nx, &twx, nullptr, &twx_nat)) return r;
+    if (int r = twiddles(c, ny, &twy, nullptr, &twy_nat)) return r;
 
-    int64_t B = c->cfg.batch ? (int64_t)c->cfg.batch : std::min<int64_t>(P, 128);
+    // batch depth: the fewest batches whose wave buffer stays within ~256 MB
+    // (the Infinity Cache), spread evenly over the raster
+    int64_t B = c->cfg.batch;
+    if (B == 0) {
+        const int64_t per_pos = plane * (int64_t)sizeof(float2);
+        const int64_t bmax = std::max<int64_t>(1, (256ll << 20) / per_pos);
+        const int64_t nbat = (P + bmax - 1) / bmax;
+        B = (P + nbat - 1) / nbat;
+    }
     B = std::max<int64_t>(1, std::min<int64_t>(B, P));
 
     // work buffers, grown as needed and kept for the next run
@@ -393,7 +411,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ca.keep = dkeep.as<int2>();
     ca.tw = twx;
     ca.psi0 = dPsi0.as<float2>();
-    ca.twy = twy;
+    ca.twx_nat = twx_nat;
+    ca.twy = twy_nat;
     ca.tile = c->cfg.col_tile;
     ca.prune = msf::col_prunable(nx, keep.ab.data()) ? 1 : 0;
     const float2* t32 = sp->t32.as<float2>();
@@ -513,6 +532,7 @@ int msf_destroy(msf_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     c->tw32.clear();
     c->tw64.clear();
+    c->twnat.clear();
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MSF_OK;
