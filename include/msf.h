@@ -86,6 +86,8 @@ typedef struct msf_config {
     int32_t transmit_mode;             /* MSF_TRANSMIT_* */
     int32_t profile_kernels;           /* 1: time every hot launch with HIP events */
     int32_t col_tile;                  /* columns per column-pass workgroup: 8, 16; 0 = automatic */
+    int32_t no_graph;                  /* 1: launch on the stream; 0: capture the job's launch
+                                          sequence once per shape as a HIP graph and replay it */
 } msf_config;
 
 /* Timings of the last msf_run_stem() on the context. */
@@ -100,6 +102,7 @@ typedef struct msf_stats {
     double row_pass_us, col_pass_us, first_row_pass_us, last_col_pass_us, reduce_us;
     uint64_t row_pass_count, col_pass_count;
     double row_pass_bytes, col_pass_bytes; /* compulsory HBM bytes of one launch */
+    int32_t graph;                     /* 0: stream launches; 1: graph captured; 2: graph replayed */
 } msf_stats;
 
 /* ------------------------------------------------------------ context */
@@ -114,9 +117,11 @@ int msf_supported_length(uint32_t n);
 /* ------------------------------------------------------------ scalars */
 /* tem.cpp:424-438 */
 int msf_wavelength(double voltage, double* lambda, double* gamma);
-/* scherzer.cpp:61-79 (uncorrected) and :131-168 (corrected to Cs5) */
+/* scherzer.cpp:61-79 (uncorrected) and :141-157 (corrected to Cs5) */
 double msf_scherzer_defocus(int32_t corrected, double lambda, double cs);
 double msf_scherzer_alphamax(int32_t corrected, double lambda, double cs);
+/* scherzer.cpp:131-139: Cs3 balancing a given Cs5 (adfstemcorr with --scherzer_cs3) */
+double msf_scherzer_cs3(double lambda, double cs5);
 
 /* ------------------------------------------------------------ specimen */
 /* Replaces tem.cpp:944-1276: species templates (scatterer_pap_LUT,
@@ -158,12 +163,24 @@ int msf_run_stem(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optic
                  const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
                  double* adf_image, double* fem_v);
 
+/* Probe shift indices of every raster position, (sx, sy) pairs in raster
+ * order (x outer, y inner): the nearest-pixel rule of adfstem.cpp:1590-1606
+ * with indices persisting from the previous position when no grid point
+ * qualifies (:1476-1477).  Host-only; MSF_ERR_DOMAIN below the domain. */
+int msf_probe_shifts(const msf_grid* grid, const double* x_p, uint32_t n_xp, const double* y_p,
+                     uint32_t n_yp, int32_t* shifts);
+
 /* Exit waves psi(k) after the last slice for the same raster, FP32 complex,
  * n_xp*n_yp*nx*ny*2 floats in the reference layout (zeros outside the band
  * limit).  Parity entry point: the reference keeps these only in memory. */
 int msf_exit_waves(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
                    const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
                    float* psi);
+/* The same for n explicit (sx, sy) shift pairs, e.g. single positions of a
+ * raster whose shifts msf_probe_shifts gave: the reference's first- and
+ * final-position diffraction images (adfstem.cpp:2140-2230, :3780-3797). */
+int msf_exit_waves_shifted(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
+                           const int32_t* shifts, uint32_t n, float* psi);
 
 #ifdef __cplusplus
 }

@@ -74,13 +74,27 @@ struct RunPlan {
     int n_tiles = 0;
     int64_t cap_b = 0, cap_p = 0;
     DevBuf psi, part, segp, state, femv, adf, shift, exit;
+    // the job's launch sequence as an instantiated HIP graph, replayed while
+    // every pointer and size it was captured with is unchanged
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_sig;
+    RunPlan() = default;
+    RunPlan(const RunPlan&) = delete;
+    RunPlan& operator=(const RunPlan&) = delete;
+    ~RunPlan() { drop_graph(); }
+    void drop_graph()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_sig.clear();
+    }
 };
 
 struct msf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0};
+    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0};
     msf_stats stats{};
     std::map<int, DevBuf> tw32, tw64, twnat;
     RunPlan plan;
@@ -216,14 +230,33 @@ struct RunOut {
     float* exit = nullptr;
 };
 
+// Probe shifts in raster order (x outer, y inner), each index persisting
+// from the previous position when no grid point qualifies
+// (adfstem.cpp:1476-1477, :1590-1606).  False if a position lies below the
+// domain, where the reference's size_t index would wrap.
+bool raster_shifts(const msf::Domain& d, const double* x_p, uint32_t n_xp, const double* y_p,
+                   uint32_t n_yp, std::vector<int2>& shifts)
+{
+    shifts.resize((size_t)n_xp * n_yp);
+    int64_t sx = 0, sy = 0;
+    for (uint32_t a = 0; a < n_xp; ++a)
+        for (uint32_t b = 0; b < n_yp; ++b) {
+            sx = msf::shift_index(d.xx, x_p[a], sx);
+            sy = msf::shift_index(d.yy, y_p[b], sy);
+            if (sx < 0 || sy < 0) return false;
+            shifts[(size_t)a * n_yp + b] = int2{(int)sx, (int)sy};
+        }
+    return true;
+}
+
 int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
-                 const msf_detector* det, const msf_fem* fem, const double* x_p, uint32_t n_xp,
-                 const double* y_p, uint32_t n_yp, RunOut out)
+                 const msf_detector* det, const msf_fem* fem, const std::vector<int2>& shifts,
+                 RunOut out)
 {
     const double t_start = now_ms();
     if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
     if (int r = check_optics(c, o)) return r;
-    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+    if (shifts.empty()) return fail(c, MSF_ERR_INVALID, "empty raster");
     const msf::Domain& d = sp->dom;
     const int nx = d.nx, ny = d.ny, S = sp->n_slices;
     const bool want_adf = det && det->enabled;
@@ -236,22 +269,11 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                         "detector inner angle falls beyond bandwidth limits (tem.cpp:721-732)");
     }
     if (want_fem && !(fem->dr > 0.0)) return fail(c, MSF_ERR_INVALID, "dr must be positive");
-    const int64_t P = (int64_t)n_xp * n_yp;
+    const int64_t P = (int64_t)shifts.size();
     const int tmode = c->cfg.transmit_mode;
-
-    // probe shifts in raster order, persisting when no index qualifies
-    std::vector<int2> shifts(P);
-    {
-        int64_t sx = 0, sy = 0;
-        for (uint32_t a = 0; a < n_xp; ++a)
-            for (uint32_t b = 0; b < n_yp; ++b) {
-                sx = msf::shift_index(d.xx, x_p[a], sx);
-                sy = msf::shift_index(d.yy, y_p[b], sy);
-                if (sx < 0 || sy < 0)
-                    return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
-                shifts[(size_t)a * n_yp + b] = int2{(int)sx, (int)sy};
-            }
-    }
+    for (const int2& sh : shifts)
+        if (sh.x < 0 || sh.x >= nx || sh.y < 0 || sh.y >= ny)
+            return fail(c, MSF_ERR_DOMAIN, "probe shift outside the grid");
 
     const msf::RowRanges& keep = sp->keep;
     const int W = msf::col_tile(nx);
@@ -418,6 +440,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const float2* t32 = sp->t32.as<float2>();
     const float2* px = sp->px.as<float2>();
     const float2* py = sp->py.as<float2>();
+    // every launch of the job (all batches, reductions, FEM statistics);
+    // issued directly or captured once into a graph
+    auto enqueue = [&]() -> int {
     for (int64_t b0 = 0; b0 < P; b0 += B) {
         const int nb = (int)std::min<int64_t>(B, P - b0);
         ra.psi = dpsi.as<float2>();
@@ -463,6 +488,50 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     }
     if (want_fem)
         HIPCHK(c, msf::launch_fem_finalize(dstate.as<double>(), nbins, dfemv.as<double>(), c->stream));
+    return MSF_OK;
+    };
+
+    // Graph replay: the signature lists everything the captured kernel
+    // arguments depend on (job shape, device addresses); any change
+    // re-captures.  Profiled runs and exit-wave runs (host copies per batch)
+    // launch directly.
+    int graph_state = 0;
+    if (!prof && !out.exit && !c->cfg.no_graph) {
+        const std::vector<uint64_t> sig = {
+            sp->id, (uint64_t)P, (uint64_t)B, (uint64_t)tmode, (uint64_t)want_adf,
+            (uint64_t)want_fem, (uint64_t)(uintptr_t)dpsi.p, (uint64_t)(uintptr_t)dpart.p,
+            (uint64_t)(uintptr_t)pl.segp.p, (uint64_t)(uintptr_t)dadf.p,
+            (uint64_t)(uintptr_t)dshift.p, (uint64_t)(uintptr_t)dstate.p,
+            (uint64_t)(uintptr_t)dfemv.p, (uint64_t)(uintptr_t)dU0.p, (uint64_t)(uintptr_t)dPsi0.p,
+            (uint64_t)(uintptr_t)pl.seg_pix.p, (uint64_t)(uintptr_t)pl.tile_off.p,
+            (uint64_t)(uintptr_t)pl.seg_kind.p, (uint64_t)(uintptr_t)pl.seg_of_group.p,
+            (uint64_t)(uintptr_t)dcnt.p, (uint64_t)(uintptr_t)dzero.p,
+            (uint64_t)(uintptr_t)twx, (uint64_t)(uintptr_t)twy, (uint64_t)(uintptr_t)twx_nat,
+            (uint64_t)(uintptr_t)twy_nat, (uint64_t)ncp, (uint64_t)G, (uint64_t)nbins,
+            (uint64_t)pl.n_tiles, (uint64_t)c->cfg.col_tile};
+        graph_state = 2;
+        if (!pl.graph || pl.graph_sig != sig) {
+            pl.drop_graph();
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const int r = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (r != MSF_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            HIPCHK(c, e);
+            const hipError_t ei = hipGraphInstantiate(&pl.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) pl.graph = nullptr;
+            HIPCHK(c, ei);
+            pl.graph_sig = sig;
+            graph_state = 1;
+        }
+        HIPCHK(c, hipGraphLaunch(pl.graph, c->stream));
+    } else {
+        if (int r = enqueue()) return r;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const double t_end = now_ms();
 
@@ -481,6 +550,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     // compulsory traffic of a full-batch pass: read + write of the compact wave
     st.row_pass_bytes = 2.0 * (double)B * nx * (ka + kb) * sizeof(float2);
     st.col_pass_bytes = st.row_pass_bytes;
+    st.graph = graph_state;
     if (prof) {
         double sum[K_NUM] = {0}, cnt[K_NUM] = {0};
         for (size_t k = 0; k < ev_kind.size(); ++k) {
@@ -569,14 +639,19 @@ int msf_wavelength(double voltage, double* lambda, double* gamma)
 
 double msf_scherzer_defocus(int32_t corrected, double lambda, double cs)
 {
-    if (corrected) return -0.983 * std::pow(msf::ref_pow2(lambda) * cs, 1.0 / 3.0); // scherzer.cpp:152-158
+    if (corrected) return -0.983 * std::pow(msf::ref_pow2(lambda) * cs, 1.0 / 3.0); // scherzer.cpp:141-148
     return 0.87 * std::sqrt(cs * lambda);                                          // scherzer.cpp:61-69
 }
 
 double msf_scherzer_alphamax(int32_t corrected, double lambda, double cs)
 {
-    if (corrected) return 1.513 * std::pow(lambda / cs, 1.0 / 6.0); // scherzer.cpp:160-166
+    if (corrected) return 1.513 * std::pow(lambda / cs, 1.0 / 6.0); // scherzer.cpp:150-157
     return 1.34 * std::pow(lambda / cs, 0.25);                      // scherzer.cpp:71-79
+}
+
+double msf_scherzer_cs3(double lambda, double cs5)
+{
+    return -2.289 * std::pow(lambda * msf::ref_pow2(cs5), 1.0 / 3.0); // scherzer.cpp:131-139
 }
 
 int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const uint32_t* Z,
@@ -813,10 +888,43 @@ int msf_run_stem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const 
     HIPCHK(c, hipSetDevice(c->device));
     if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
     if (fem && fem->enabled && !fem_v) return fail(c, MSF_ERR_INVALID, "fem_v is NULL");
+    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+    std::vector<int2> shifts;
+    if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
+        return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
     RunOut out;
     out.adf = adf_image;
     out.fem = fem_v;
-    return run_internal(c, sp, o, det, fem, x_p, n_xp, y_p, n_yp, out);
+    return run_internal(c, sp, o, det, fem, shifts, out);
+}
+
+int msf_probe_shifts(const msf_grid* g, const double* x_p, uint32_t n_xp, const double* y_p,
+                     uint32_t n_yp, int32_t* shifts)
+{
+    if (!g || !x_p || !y_p || !shifts || g->nx < 2 || g->ny < 2) return MSF_ERR_INVALID;
+    const msf::Domain d =
+        msf::make_domain((int)g->nx, (int)g->ny, g->xperiod, g->yperiod, g->xmin, g->ymin, g->voltage);
+    std::vector<int2> sh;
+    if (!raster_shifts(d, x_p, n_xp, y_p, n_yp, sh)) return MSF_ERR_DOMAIN;
+    for (size_t k = 0; k < sh.size(); ++k) {
+        shifts[2 * k] = sh[k].x;
+        shifts[2 * k + 1] = sh[k].y;
+    }
+    return MSF_OK;
+}
+
+int msf_exit_waves_shifted(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
+                           const int32_t* shifts, uint32_t n, float* psi)
+{
+    if (!c || !psi || !shifts) return MSF_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) return fail(c, MSF_ERR_INVALID, "empty shift list");
+    std::vector<int2> sh(n);
+    for (uint32_t k = 0; k < n; ++k) sh[k] = int2{shifts[2 * k], shifts[2 * k + 1]};
+    RunOut out;
+    out.exit = psi;
+    return run_internal(c, sp, o, nullptr, nullptr, sh, out);
 }
 
 int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const double* x_p,
@@ -824,9 +932,14 @@ int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, cons
 {
     if (!c || !psi) return MSF_ERR_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
+    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+    std::vector<int2> shifts;
+    if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
+        return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
     RunOut out;
     out.exit = psi;
-    return run_internal(c, sp, o, nullptr, nullptr, x_p, n_xp, y_p, n_yp, out);
+    return run_internal(c, sp, o, nullptr, nullptr, shifts, out);
 }
 
 } // extern "C"

@@ -52,9 +52,17 @@ class Fem(C.Structure):
     _fields_ = [("enabled", C.c_int32), ("dr", C.c_double)]
 
 
+class Model(C.Structure):
+    _fields_ = [("n_atoms", C.c_uint32), ("q", C.POINTER(C.c_double)),
+                ("Z", C.POINTER(C.c_uint32)), ("xperiod", C.c_double), ("yperiod", C.c_double),
+                ("zperiod", C.c_double), ("xmin", C.c_double), ("ymin", C.c_double),
+                ("zmin", C.c_double)]
+
+
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
-                ("profile_kernels", C.c_int32), ("col_tile", C.c_int32)]
+                ("profile_kernels", C.c_int32), ("col_tile", C.c_int32),
+                ("no_graph", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -65,17 +73,22 @@ class Stats(C.Structure):
                 ("first_row_pass_us", C.c_double), ("last_col_pass_us", C.c_double),
                 ("reduce_us", C.c_double), ("row_pass_count", C.c_uint64),
                 ("col_pass_count", C.c_uint64), ("row_pass_bytes", C.c_double),
-                ("col_pass_bytes", C.c_double)]
+                ("col_pass_bytes", C.c_double), ("graph", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+IO_EXPORTS = ["msf_io_last_error", "msf_element_z", "msf_guess_z", "msf_read_xyz",
+              "msf_read_lammps_data", "msf_model_free", "msf_write_tiff16",
+              "msf_write_variance_txt", "msf_write_f64"]
+
 EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf_get_stats",
            "msf_supported_length", "msf_wavelength", "msf_scherzer_defocus",
            "msf_scherzer_alphamax", "msf_specimen_build", "msf_specimen_free",
            "msf_specimen_slices", "msf_specimen_slice_info", "msf_specimen_transmission",
-           "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves"]
+           "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves",
+           "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -115,6 +128,25 @@ def lib():
                                    C.POINTER(Fem), _dp, C.c_uint32, _dp, C.c_uint32, _dp, _dp]
         L.msf_exit_waves.argtypes = [vp, vp, C.POINTER(Optics), _dp, C.c_uint32, _dp,
                                      C.c_uint32, C.POINTER(C.c_float)]
+        L.msf_probe_shifts.argtypes = [C.POINTER(Grid), _dp, C.c_uint32, _dp, C.c_uint32,
+                                       C.POINTER(C.c_int32)]
+        L.msf_exit_waves_shifted.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(C.c_int32),
+                                             C.c_uint32, C.POINTER(C.c_float)]
+        L.msf_scherzer_cs3.argtypes = [C.c_double, C.c_double]
+        L.msf_scherzer_cs3.restype = C.c_double
+        # host-side readers and writers (include/msf_io.h)
+        L.msf_io_last_error.restype = C.c_char_p
+        L.msf_element_z.argtypes = [C.c_char_p]
+        L.msf_element_z.restype = C.c_uint32
+        L.msf_guess_z.argtypes = [C.c_double, C.POINTER(C.c_uint32)]
+        L.msf_read_xyz.argtypes = [C.c_char_p, C.POINTER(Model)]
+        L.msf_read_lammps_data.argtypes = [C.c_char_p, C.POINTER(Model)]
+        L.msf_model_free.argtypes = [C.POINTER(Model)]
+        L.msf_model_free.restype = None
+        L.msf_write_tiff16.argtypes = [C.c_char_p, _dp, C.c_uint32, C.c_uint32, C.c_double,
+                                       C.c_double, C.c_int, C.c_uint32, C.c_double, C.c_double]
+        L.msf_write_variance_txt.argtypes = [C.c_char_p, _dp, _dp, C.c_uint32]
+        L.msf_write_f64.argtypes = [C.c_char_p, _dp, C.c_uint32, C.c_uint32]
         _lib = L
     return _lib
 
@@ -137,6 +169,78 @@ def scherzer_defocus(lam, cs, corrected=0):
 
 def scherzer_alphamax(lam, cs, corrected=0):
     return lib().msf_scherzer_alphamax(corrected, lam, cs)
+
+
+def scherzer_cs3(lam, cs5):
+    return lib().msf_scherzer_cs3(lam, cs5)
+
+
+def _io_check(rc, what):
+    if rc:
+        raise MsfError(rc, f"{what}: {lib().msf_io_last_error().decode()}")
+
+
+def probe_shifts(grid, x_p, y_p):
+    """(sx, sy) of every raster position, shape [n_xp * n_yp, 2] (host only)."""
+    x_p = np.ascontiguousarray(x_p, np.float64)
+    y_p = np.ascontiguousarray(y_p, np.float64)
+    out = np.zeros((x_p.size * y_p.size, 2), np.int32)
+    rc = lib().msf_probe_shifts(C.byref(grid), _p(x_p), x_p.size, _p(y_p), y_p.size,
+                                _p(out, C.c_int32))
+    if rc:
+        raise MsfError(rc, "msf_probe_shifts")
+    return out
+
+
+def element_z(name):
+    return int(lib().msf_element_z(name.encode()))
+
+
+def guess_z(mass):
+    z = C.c_uint32()
+    _io_check(lib().msf_guess_z(mass, C.byref(z)), "msf_guess_z")
+    return z.value
+
+
+def read_model(path):
+    """Reads an xyz file (extension "xyz") or a LAMMPS data file (any other
+    extension), as the reference's main does (tem.cpp:270-330).  Returns
+    (q [n, 3], Z [n], (xperiod, yperiod, zperiod))."""
+    m = Model()
+    xyz = path[path.rfind(".") + 1:] == "xyz"   # find_last_of(".") + 1, npos + 1 == 0
+    fn = lib().msf_read_xyz if xyz else lib().msf_read_lammps_data
+    _io_check(fn(os.fsencode(path), C.byref(m)), "read " + path)
+    try:
+        n = m.n_atoms
+        q = np.ctypeslib.as_array(m.q, shape=(3 * n,)).reshape(n, 3).copy() if n else np.zeros((0, 3))
+        Z = np.ctypeslib.as_array(m.Z, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+        return q, Z, (m.xperiod, m.yperiod, m.zperiod)
+    finally:
+        lib().msf_model_free(C.byref(m))
+
+
+def write_tiff16(path, data, vmin=None, vmax=None, logscale=False, resolution_unit=3,
+                 x_resolution=1.0, y_resolution=1.0):
+    d = np.ascontiguousarray(data, np.float64)
+    vmin = float(d.min()) if vmin is None else vmin
+    vmax = float(d.max()) if vmax is None else vmax
+    _io_check(lib().msf_write_tiff16(os.fsencode(path), _p(d), d.shape[0], d.shape[1], vmin, vmax,
+                                     1 if logscale else 0, resolution_unit, x_resolution,
+                                     y_resolution), "msf_write_tiff16")
+
+
+def write_variance_txt(path, values, lower_edges):
+    v = np.ascontiguousarray(values, np.float64)
+    e = np.ascontiguousarray(lower_edges, np.float64)
+    _io_check(lib().msf_write_variance_txt(os.fsencode(path), _p(v), _p(e), v.size),
+              "msf_write_variance_txt")
+
+
+def write_f64(path, data):
+    d = np.ascontiguousarray(data, np.float64)
+    d2 = d.reshape(d.shape[0], -1) if d.ndim > 1 else d.reshape(1, -1)
+    _io_check(lib().msf_write_f64(os.fsencode(path), _p(d2), d2.shape[0], d2.shape[1]),
+              "msf_write_f64")
 
 
 def supported_length(n):
@@ -182,8 +286,9 @@ class Context:
             raise MsfError(rc, f"{what}: {lib().msf_last_error(self._h).decode()}")
 
     def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False,
-                  col_tile=0):
-        cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0, col_tile)
+                  col_tile=0, graph=True):
+        cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0, col_tile,
+                     0 if graph else 1)
         self._check(lib().msf_set_config(self._h, C.byref(cfg)), "msf_set_config")
 
     def stats(self):
@@ -218,6 +323,15 @@ class Context:
         if v is not None:
             v = v[:fem_bins(spec.grid, fm.dr).size]
         return img, v
+
+    def exit_waves_shifted(self, spec, optics, shifts):
+        sh = np.ascontiguousarray(shifts, np.int32).reshape(-1, 2)
+        out = np.zeros((sh.shape[0], spec.grid.nx, spec.grid.ny), np.complex64)
+        self._check(lib().msf_exit_waves_shifted(self._h, spec._h, C.byref(optics),
+                                                 _p(sh, C.c_int32), sh.shape[0],
+                                                 _p(out.view(np.float32), C.c_float)),
+                    "msf_exit_waves_shifted")
+        return out
 
     def exit_waves(self, spec, optics, x_p, y_p):
         x_p = np.ascontiguousarray(x_p, np.float64)

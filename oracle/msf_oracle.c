@@ -82,7 +82,7 @@ void orc_domains(int Nx, int Ny, double xperiod, double yperiod,
     for (int i = 0; i < Ny; i++) yy[i] = ymin + (i * dy);
 }
 
-/* scherzer.cpp:61-79 and :131-168 */
+/* scherzer.cpp:61-79 and :131-157 */
 double orc_scherzer_defocus_uncorrected(double lambda, double Cs3)
 {
     return 0.87 * sqrt(Cs3 * lambda);
@@ -98,6 +98,10 @@ double orc_scherzer_defocus_correctedtoCs5(double lambda, double Cs5)
 double orc_scherzer_alphamax_correctedtoCs5(double lambda, double Cs5)
 {
     return 1.513 * pow(lambda / Cs5, 1.0 / 6.0);
+}
+double orc_scherzer_cs3_correctedtoCs5(double lambda, double Cs5)
+{
+    return -2.289 * pow(lambda * ref_pow2(Cs5), 1.0 / 3.0);
 }
 
 /* func.cpp:108-127 (assign_stem_raster_points) with the point count of

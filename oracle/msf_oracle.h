@@ -47,6 +47,7 @@ double orc_scherzer_defocus_uncorrected(double lambda, double cs3);
 double orc_scherzer_alphamax_uncorrected(double lambda, double cs3);
 double orc_scherzer_defocus_correctedtoCs5(double lambda, double cs5);
 double orc_scherzer_alphamax_correctedtoCs5(double lambda, double cs5);
+double orc_scherzer_cs3_correctedtoCs5(double lambda, double cs5);
 void orc_raster(int n, double period, double qmin, double* q);
 
 /* ---- FFT (own mixed-radix, FP64) ---- */

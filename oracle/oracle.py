@@ -64,7 +64,8 @@ def lib():
         L.orc_domains.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, _dp, _dp, _dp, _dp]
         for f in ("orc_scherzer_defocus_uncorrected", "orc_scherzer_alphamax_uncorrected",
-                  "orc_scherzer_defocus_correctedtoCs5", "orc_scherzer_alphamax_correctedtoCs5"):
+                  "orc_scherzer_defocus_correctedtoCs5", "orc_scherzer_alphamax_correctedtoCs5",
+                  "orc_scherzer_cs3_correctedtoCs5"):
             getattr(L, f).argtypes = [C.c_double, C.c_double]
             getattr(L, f).restype = C.c_double
         L.orc_raster.argtypes = [C.c_int, C.c_double, C.c_double, _dp]

@@ -10,8 +10,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    txt = open(os.path.join(ROOT, "include", "msf.h")).read()
+def _declared(header="msf.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(msf_\w+)\(", txt, re.M)))
 
 
@@ -29,6 +29,11 @@ def test_library_exports_every_declared_symbol(msf):
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(msf.EXPORTS) == names
+    io = _declared("msf_io.h")
+    assert len(io) == 9
+    for n in io:
+        assert hasattr(lib, n), n
+    assert sorted(msf.IO_EXPORTS) == io
 
 
 def test_wavelength_and_scherzer_bitwise(msf, O):
@@ -40,6 +45,9 @@ def test_wavelength_and_scherzer_bitwise(msf, O):
     L = O.lib()
     assert msf.scherzer_defocus(lam, 5e7, 1) == L.orc_scherzer_defocus_correctedtoCs5(lam, 5e7)
     assert msf.scherzer_alphamax(lam, 5e7, 1) == L.orc_scherzer_alphamax_correctedtoCs5(lam, 5e7)
+    assert msf.scherzer_cs3(lam, 5e7) == L.orc_scherzer_cs3_correctedtoCs5(lam, 5e7)
+    # Kirkland's table value: C_S3 = -2.289 (lambda C_S5^2)^(1/3)
+    assert abs(msf.scherzer_cs3(lam, 5e7) / (-2.289 * (lam * 5e7 ** 2) ** (1 / 3)) - 1) < 1e-15
 
 
 @pytest.mark.parametrize("n,L,dr", [(512, 40.0, 0.1414213562), (256, 20.0, 1 / np.sqrt(2)),
@@ -66,3 +74,18 @@ def test_errors_are_reported_not_crashing(msf):
         msf.wavelength(-1.0)
     with pytest.raises(msf.MsfError):
         msf.fem_bins(msf.make_grid(64, 64, 8.0, 8.0, 8.0, 200e3), -1.0)
+
+
+def test_probe_shifts_match_oracle(msf, O):
+    """msf_probe_shifts (host rule, no GPU) against the oracle's raster loop,
+    including positions where no index qualifies and the shift persists."""
+    g = msf.make_grid(64, 48, 12.0, 9.0, 5.0, 300e3)
+    xx, yy = O.domains(64, 48, 12.0, 9.0)[2:4]
+    x_p = np.array([0.0, 3.3, 11.99, 11.95, 6.0, 12.0 - 12.0 / 128])
+    y_p = np.array([8.99, 0.05, 4.4, 9.0])
+    got = msf.probe_shifts(g, x_p, y_p)
+    want = np.stack(O.probe_shifts(xx, yy, x_p, y_p), axis=1)
+    np.testing.assert_array_equal(got, want)
+    with pytest.raises(msf.MsfError) as e:
+        msf.probe_shifts(g, np.array([-0.5]), np.array([1.0]))
+    assert e.value.code == -5

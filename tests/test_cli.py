@@ -1,0 +1,140 @@
+"""The ms-stem-fem command line: flags, parameter-file keys, required
+parameters, defaults and derived settings (tem_io.cpp:4063-4500,
+:5277-5700, :5858-6060; tem.cpp:1320-1480), checked through --plan_only,
+which stops before any device work."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import msf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "ms-stem-fem_amd", "ms-stem-fem")
+
+MODEL = "4\n20 20 10\nSi 1 1 1\nSi 5 5 5\nCu 9 2 3\nZr 3 8 9\n"
+
+
+@pytest.fixture
+def model(tmp_path):
+    p = tmp_path / "m.xyz"
+    p.write_text(MODEL)
+    return str(p)
+
+
+def run(args, cwd):
+    r = subprocess.run([CLI] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+    plan = {}
+    for line in r.stdout.splitlines():
+        k, _, v = line.partition(" ")
+        plan.setdefault(k, []).append(v)
+    return r.returncode, plan, r.stdout + r.stderr
+
+
+def test_uncorrected_fem_plan(tmp_path, model):
+    rc, plan, out = run(["-m", "128", "128", "200000", "-a", model, "--output_prefix", "o",
+                         "--adfstemuncorrfem", "--cs3", "1.2e7", "--scherzer_defocus",
+                         "--scherzer_alphamax", "--rasterspacing", "1.25", "--minslice", "2",
+                         "--dr", "0.1414213562", "--plan_only"], tmp_path)
+    assert rc == 0, out
+    lam = msf.wavelength(200e3)[0]
+    mode = plan["mode"][0].split()
+    assert mode[0] == "adfstem_uncorr"
+    assert float(mode[6]) == msf.scherzer_defocus(lam, 1.2e7)
+    assert float(mode[8]) == msf.scherzer_alphamax(lam, 1.2e7)
+    assert plan["raster"][0].split()[:2] == ["16", "16"]          # floor(20 / 1.25)
+    assert [float(x) for x in plan["detector"][0].split()] == [0.040, 0.200]  # >= 200 kV
+    assert plan["fem"] == ["1"] and plan["images"] == ["0"]
+    assert float(plan["minslice"][0]) == 2.0 and float(plan["dr"][0]) == 0.1414213562
+
+
+def test_parameter_file_keys_and_precedence(tmp_path, model):
+    pf = tmp_path / "p.txt"
+    pf.write_text(f"SAMPLES_X 256\nsamples_y 192\nmicroscope_voltage 100000\nmodel_file {model}\n"
+                  "output_prefix pf\n\nadfstemcorr\ncs5 5e7\nscherzer_cs3\nscherzer_defocus\n"
+                  "alphamax 0.02\ndetectorangles 0.3 0.05\nraster_spacing 2.5\nimages\n")
+    rc, plan, out = run(["--parameter_file", str(pf), "--alphamax", "0.015", "--plan_only"], tmp_path)
+    assert rc == 0, out
+    lam = msf.wavelength(100e3)[0]
+    mode = plan["mode"][0].split()
+    assert mode[0] == "adfstem_corr"
+    assert float(mode[2]) == msf.scherzer_cs3(lam, 5e7)
+    assert float(mode[6]) == msf.scherzer_defocus(lam, 5e7, 1)
+    assert float(mode[8]) == 0.015                     # the later command-line value wins
+    assert [float(x) for x in plan["detector"][0].split()] == [0.05, 0.3]  # swapped
+    assert plan["grid"][0].split()[:2] == ["256", "192"]
+    assert plan["raster"][0].split()[:2] == ["8", "8"]
+    assert plan["images"] == ["1"] and plan["fem"] == ["0"]
+    # a command-line value given before the parameter file is overridden by it
+    rc, plan, _ = run(["--alphamax", "0.015", "--parameter_file", str(pf), "--plan_only"], tmp_path)
+    assert float(plan["mode"][0].split()[8]) == 0.02
+
+
+def test_defaults(tmp_path, model):
+    base = ["-m", "64", "64", "100000", "-a", model, "--output_prefix", "o", "--cs3", "1e6",
+            "--defocus", "10", "--alphamax", "0.01", "--plan_only"]
+    rc, plan, out = run(base + ["--adfstemuncorr"], tmp_path)
+    assert rc == 0, out
+    assert [float(x) for x in plan["detector"][0].split()] == [0.045, 0.200]  # below 200 kV
+    assert float(plan["raster"][0].split()[2]) == 1.5
+    assert float(plan["minslice"][0]) == 1.0
+    assert float(plan["dr"][0]) == 0.70710678118654746
+    rc, plan, _ = run(base + ["--adfstemuncorrfem"], tmp_path)
+    assert float(plan["raster"][0].split()[2]) == 10.0   # FEM default spacing
+    assert plan["raster"][0].split()[:2] == ["2", "2"]
+
+
+def test_dupe_replicates_for_fem_only(tmp_path, model):
+    base = ["-m", "64", "64", "300000", "-a", model, "--output_prefix", "o", "--cs3", "0",
+            "--defocus", "0", "--alphamax", "0.01", "--dupe", "2", "1", "3", "--rasterspacing",
+            "5", "--plan_only"]
+    rc, plan, out = run(base + ["--adfstemuncorrfem"], tmp_path)
+    assert rc == 0, out
+    assert plan["atoms"] == ["24"]
+    g = plan["grid"][0].split()
+    assert [float(x) for x in g[2:]] == [40.0, 20.0, 30.0]
+    assert plan["raster"][0].split()[:2] == ["4", "4"]     # raster of the undupled cell
+    rc, plan, _ = run(base + ["--adfstemuncorr"], tmp_path)
+    assert plan["atoms"] == ["4"]
+
+
+@pytest.mark.parametrize("args, why", [
+    (["-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr", "--cs3", "0", "--defocus", "0",
+      "--alphamax", "0.01"], "no -m"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--adfstemuncorr", "--cs3", "0", "--defocus", "0",
+      "--alphamax", "0.01"], "no output prefix"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
+      "--defocus", "0", "--alphamax", "0.01"], "uncorrected without cs3"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemcorr",
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01"], "corrected without cs5"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
+      "--cs3", "0", "--alphamax", "0.01"], "no defocus"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--bogus"], "unknown flag"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorrfem",
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--V_re"], "mode outside the build"),
+    (["-m", "64", "64", "3e5", "-a", "missing.xyz", "--output_prefix", "o", "--adfstemuncorr",
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01"], "unreadable model"),
+    (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--detectorangles", "2.0", "3.0"],
+     "inner angle beyond the band limit"),
+])
+def test_failures(tmp_path, model, args, why):
+    args = [model if a == "MODEL" else a for a in args] + ["--plan_only"]
+    rc, _, out = run(args, tmp_path)
+    assert rc == 1, (why, out)
+
+
+def test_parameter_file_rejects_unknown_key_and_drops_unterminated_last_line(tmp_path, model):
+    pf = tmp_path / "p.txt"
+    pf.write_text(f"samples_x 64\nsamples_y 64\nmicroscope_voltage 3e5\nmodel_file {model}\n"
+                  "output_prefix o\nadfstemuncorr\ncs3 0\ndefocus 0\nalphamax 0.01\nfrobnicate 1\n")
+    rc, _, out = run(["--parameter_file", str(pf), "--plan_only"], tmp_path)
+    assert rc == 1 and "unexpected argument in parameter file" in out
+    # the reference's read loop stops at a line without a newline: "alphamax"
+    # is never read, so the run lacks a required parameter
+    pf.write_text(f"samples_x 64\nsamples_y 64\nmicroscope_voltage 3e5\nmodel_file {model}\n"
+                  "output_prefix o\nadfstemuncorr\ncs3 0\ndefocus 0\nalphamax 0.01")
+    rc, _, out = run(["--parameter_file", str(pf), "--plan_only"], tmp_path)
+    assert rc == 1 and "lacking required parameters" in out

@@ -1,0 +1,83 @@
+"""End-to-end run of the ms-stem-fem command line on a small xyz model: the
+float64 ADF sidecar and the V_omega text file match the FP64 oracle
+(1e-4 relative L2), the 16-bit TIFFs are the min-max quantisation of the
+sidecar, and the diffraction images are the log-scaled exit waves."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import Case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "ms-stem-fem_amd", "ms-stem-fem")
+
+
+def _tiff(path):
+    b = open(path, "rb").read()
+    ifd = struct.unpack_from("<I", b, 4)[0]
+    tags = {}
+    for k in range(struct.unpack_from("<H", b, ifd)[0]):
+        tid, typ, cnt, val = struct.unpack_from("<HHII", b, ifd + 2 + 12 * k)
+        tags[tid] = (cnt, val & 0xFFFF if typ == 3 and cnt == 1 else val)
+    w, h = tags[256][1], tags[257][1]
+    n = tags[273][0]
+    offs = [tags[273][1]] if n == 1 else struct.unpack_from(f"<{n}I", b, tags[273][1])
+    cnts = [tags[279][1]] if n == 1 else struct.unpack_from(f"<{n}I", b, tags[279][1])
+    raw = b"".join(b[o:o + c] for o, c in zip(offs, cnts))
+    return np.frombuffer(raw, "<u2").reshape(h, w)[::-1]   # rows are stored bottom-up
+
+
+def test_cli_adfstem_fem_run(tmp_path, O):
+    rng = np.random.default_rng(31)
+    n_at = 45
+    q = rng.random((n_at, 3)) * [10.0, 10.0, 6.0]
+    names = rng.choice(["Si", "Cu"], n_at)
+    lines = [f"{n_at}", "10 10 6"] + [f"{e} {x:.6f} {y:.6f} {z:.6f}" for e, (x, y, z) in zip(names, q)]
+    (tmp_path / "m.xyz").write_text("\n".join(lines) + "\n")
+    args = [CLI, "-m", "128", "128", "300000", "-a", "m.xyz", "--output_prefix", "run",
+            "--adfstemuncorrfem", "--cs3", "1.0e6", "--defocus", "25", "--alphamax", "0.02",
+            "--rasterspacing", "2.5", "--minslice", "1.5", "--dr", "0.5", "--images"]
+    print("cli:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    # the same job through the oracle: positions 0, 2.5, 5, 7.5 on both axes
+    qq = np.array([[float(v) for v in l.split()[1:]] for l in lines[2:]])
+    Z = np.array([14 if e == "Si" else 29 for e in names], np.uint32)
+    c = Case(nx=128, ny=128, cell=(10.0, 10.0, 6.0), voltage=300e3, q=qq, Z=Z, min_slice=1.5,
+             cs3=1.0e6, defocus=25.0, alpha_max=0.02, inner=0.040, outer=0.200, fem=True, dr=0.5)
+    xp = np.arange(4) * 2.5
+    _, (img_o, v_o, ex_o) = c.run_oracle(O, xp, xp, exit_waves=True)
+
+    pre = tmp_path / "run_adfstem_uncorr"
+    raw = open(f"{pre}_stem_image_output.f64", "rb").read()
+    head, body = raw.split(b"\n", 1)
+    assert head == b"msf-f64 4 4"
+    img = np.frombuffer(body, "<f8").reshape(4, 4)
+    assert rel_l2(img, img_o) < 1e-4
+
+    txt = np.loadtxt(f"{pre}_fem_V_omega.txt")
+    assert txt.shape == (v_o.size, 2)
+    np.testing.assert_allclose(txt[:, 0], np.arange(v_o.size) * txt[1, 0], rtol=1e-9)
+    assert rel_l2(txt[:, 1], v_o) < 1e-4
+
+    lo, hi = img.min(), img.max()
+    lin = _tiff(f"{pre}_stem_image_output.tif")
+    np.testing.assert_array_equal(lin, np.floor((img - lo) * (65535.0 / (hi - lo)) + 0.5))
+    lg = _tiff(f"{pre}_stem_image_output_logscale.tif")
+    t = np.log(img - lo + 1) / np.log(hi - lo + 1)
+    np.testing.assert_array_equal(lg, np.where(t > 0, np.floor(t * 65535.0 + 0.5), 0))
+
+    for s, tag, k in (("1e-05", "first", 0), ("1", "first", 0), ("1e+20", "first", 0),
+                      ("1", "final", 15)):
+        d = _tiff(f"{pre}_{tag}_diffraction_scalefactor_{s}.tif").astype(float)
+        m = np.log(1.0 + float(s) * np.abs(ex_o[k]) ** 2)
+        m = np.fft.fftshift(m)
+        want = np.floor((m - m.min()) * (65535.0 / (m.max() - m.min())) + 0.5)
+        bad = np.abs(d - want) > 2
+        assert bad.mean() < 1e-3, (s, tag, bad.sum())

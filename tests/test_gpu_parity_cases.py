@@ -1,0 +1,212 @@
+"""GPU parity over the remaining transform lengths, optics, detector modes and
+edge cases of the raster, against the FP64 oracle (same tolerances as
+test_gpu_parity.py: exit intensity 1e-5, ADF / V_omega 1e-4 relative L2)."""
+import numpy as np
+import pytest
+
+from helpers import Case, random_atoms, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def msf():
+    import msf as m
+    m.lib()
+    return m
+
+
+def _exit_parity(msf, ctx, O, c, xp, yp, batch=0, tol=1e-5):
+    _, (_, _, ex_o) = c.run_oracle(O, xp, yp, exit_waves=True)
+    ctx.configure(batch=batch, transmit_mode=c.transmit_mode)
+    spec_g = c.gpu_specimen(msf, ctx)
+    ex_g = ctx.exit_waves(spec_g, c.optics(msf), xp, yp)
+    ctx.configure()
+    errs = [rel_l2(np.abs(ex_g[p]) ** 2, np.abs(ex_o[p]) ** 2) for p in range(ex_o.shape[0])]
+    assert max(errs) < tol, errs
+    # the band limit leaves nothing outside k^2 < bwcutoff_t^2
+    assert np.count_nonzero(ex_g[0]) <= np.count_nonzero(np.abs(ex_o[0]) > 0)
+
+
+@pytest.mark.parametrize("n", [96, 192, 256, 384])
+def test_exit_waves_other_lengths(msf, ctx, O, n):
+    """Radix-6/12/24 plans (96, 192, 384) and the 256-point plan."""
+    cell = (n / 8.0, n / 8.0, 6.0)
+    q, Z = random_atoms(50, cell, seed=n, species=(14, 40))
+    c = Case(nx=n, ny=n, cell=cell, voltage=300e3, q=q, Z=Z, min_slice=2.0, alpha_max=0.015)
+    _exit_parity(msf, ctx, O, c, np.array([0.3, cell[0] / 2 + 0.11]), np.array([1.7]), batch=2)
+
+
+def test_exit_waves_1024(msf, ctx, O):
+    """1024^2: the 16:8,8,4,4 plan and the 8-column tile of the column pass."""
+    cell = (40.0, 40.0, 3.0)
+    q, Z = random_atoms(30, cell, seed=11, species=(29,))
+    c = Case(nx=1024, ny=1024, cell=cell, voltage=200e3, q=q, Z=Z, min_slice=2.0,
+             alpha_max=0.02)
+    _exit_parity(msf, ctx, O, c, np.array([5.0, 31.3]), np.array([20.0]))
+
+
+def test_non_square_lengths(msf, ctx, O):
+    """nx != ny with different plans on the two axes (192 x 96)."""
+    cell = (18.0, 9.0, 4.0)
+    q, Z = random_atoms(25, cell, seed=5, species=(14,))
+    c = Case(nx=192, ny=96, cell=cell, voltage=300e3, q=q, Z=Z, min_slice=1.5, alpha_max=0.02)
+    _exit_parity(msf, ctx, O, c, np.array([2.0, 17.0]), np.array([4.4]))
+
+
+def test_corrected_probe_full_run(msf, ctx, O):
+    """adfstemcorr: Cs3 + Cs5 aberration function in the probe, ADF + FEM."""
+    q, Z = random_atoms(60, (12.0, 12.0, 8.0), seed=9, species=(14, 29))
+    c = Case(nx=128, ny=128, cell=(12.0, 12.0, 8.0), voltage=200e3, q=q, Z=Z, min_slice=1.5,
+             corrected=1, cs3=-1.5e5, cs5=4.0e7, defocus=-12.0, alpha_max=0.025, inner=0.03,
+             outer=0.18, fem=True, dr=0.5)
+    xp = np.linspace(0.2, 11.0, 4)
+    yp = np.linspace(0.9, 11.3, 3)
+    _, (img_o, v_o, _) = c.run_oracle(O, xp, yp)
+    spec_g = c.gpu_specimen(msf, ctx)
+    img_g, v_g = ctx.adfstem(spec_g, c.optics(msf), xp, yp, detector=c.detector(msf),
+                             fem=c.femcfg(msf))
+    assert rel_l2(img_g, img_o) < 1e-4
+    assert rel_l2(v_g, v_o) < 1e-4
+
+
+def test_physical_detector_mode(msf, ctx, O):
+    """MSF_ADF_PHYSICAL squares the thresholds; the ring differs from the
+    reference's k^2-vs-k ring and both match the oracle in their mode."""
+    q, Z = random_atoms(60, (12.0, 12.0, 8.0), seed=21, species=(29,))
+    imgs = {}
+    for mode in (0, 1):
+        c = Case(nx=128, ny=128, cell=(12.0, 12.0, 8.0), voltage=300e3, q=q, Z=Z,
+                 min_slice=1.5, alpha_max=0.02, inner=0.04, outer=0.2, adf_mode=mode)
+        xp = np.array([1.0, 6.0, 10.5])
+        yp = np.array([2.0, 8.0])
+        _, (img_o, _, _) = c.run_oracle(O, xp, yp)
+        spec_g = c.gpu_specimen(msf, ctx)
+        img_g, _ = ctx.adfstem(spec_g, c.optics(msf), xp, yp, detector=c.detector(msf))
+        assert rel_l2(img_g, img_o) < 1e-4, mode
+        imgs[mode] = img_g
+    assert rel_l2(imgs[0], imgs[1]) > 1e-2
+
+
+def test_si256_geometry_subset(msf, ctx, O):
+    """The 256^2 Si job (400 atoms in a 20 A cube, 300 kV, 10 mrad, ADF
+    40-200 mrad, minslice 2.0) on a 4 x 4 subset of its central raster."""
+    import specimens as SP
+    job, q, Z, xp, yp = SP.config_si_256(seed=1001)
+    c = Case(nx=256, ny=256, cell=(20.0, 20.0, 20.0), voltage=job["voltage"], q=q, Z=Z,
+             min_slice=job["min_slice"], cs3=job["cs3"], defocus=job["defocus"],
+             alpha_max=job["alpha_max"], inner=job["det_inner"], outer=job["det_outer"])
+    sub_x, sub_y = xp[::4], yp[5:9]
+    _, (img_o, _, _) = c.run_oracle(O, sub_x, sub_y)
+    spec_g = c.gpu_specimen(msf, ctx)
+    assert 8 <= spec_g.n_slices <= 12
+    img_g, _ = ctx.adfstem(spec_g, c.optics(msf), sub_x, sub_y, detector=c.detector(msf))
+    assert rel_l2(img_g, img_o) < 1e-4
+
+
+def test_positions_beyond_last_pixel_keep_previous_shift(msf, ctx, O):
+    """No grid index qualifies for x >= x[N-1] + dx/2: the previous position's
+    shift persists (the raster loop never resets it)."""
+    q, Z = random_atoms(40, (12.0, 12.0, 6.0), seed=13, species=(14,))
+    c = Case(nx=64, ny=64, cell=(12.0, 12.0, 6.0), voltage=300e3, q=q, Z=Z, min_slice=1.5,
+             alpha_max=0.02)
+    dx = 12.0 / 64
+    xp = np.array([3.0, 12.0 - dx / 4, 7.0])   # x1 keeps the x-shift of x0
+    yp = np.array([11.99, 4.0])                 # y0 keeps the previous y-shift
+    _exit_parity(msf, ctx, O, c, xp, yp)
+    spec_g = c.gpu_specimen(msf, ctx)
+    ex = ctx.exit_waves(spec_g, c.optics(msf), xp, yp)
+    # raster order is x outer, y inner: (x1, y0) inherits both shifts of
+    # (x0, y1), and (x1, y1) keeps the x-shift of x0; all three waves agree
+    np.testing.assert_array_equal(ex[1], ex[2])
+    np.testing.assert_array_equal(ex[1], ex[3])
+
+
+def test_empty_specimen_is_free_propagation(msf, ctx, O):
+    """No atoms: one slice of zperiod, t = 1, exit wave = M P psi0."""
+    c = Case(nx=64, ny=64, cell=(10.0, 10.0, 5.0), voltage=300e3,
+             q=np.zeros((0, 3)), Z=np.zeros(0, np.uint32), min_slice=1.0, alpha_max=0.02)
+    spec_g = c.gpu_specimen(msf, ctx)
+    assert spec_g.n_slices == 1
+    lb, th, na = spec_g.slice_info()
+    assert th[0] == 5.0 and na[0] == 0
+    _exit_parity(msf, ctx, O, c, np.array([1.0, 4.2]), np.array([3.3]))
+
+
+def test_error_paths(msf, ctx):
+    q, Z = random_atoms(10, (8.0, 8.0, 4.0), seed=1)
+    c = Case(nx=64, ny=64, cell=(8.0, 8.0, 4.0), voltage=300e3, q=q, Z=Z)
+    spec = c.gpu_specimen(msf, ctx)
+    # inner angle beyond the band limit (tem.cpp:721-732)
+    bad = msf.Detector(1, 10.0, 20.0, 0)
+    with pytest.raises(msf.MsfError) as e:
+        ctx.adfstem(spec, c.optics(msf), np.array([1.0]), np.array([1.0]), detector=bad)
+    assert e.value.code == -1
+    # negative detector angle (tem.cpp:710-720)
+    with pytest.raises(msf.MsfError) as e:
+        ctx.adfstem(spec, c.optics(msf), np.array([1.0]), np.array([1.0]),
+                    detector=msf.Detector(1, -0.01, 0.2, 0))
+    assert e.value.code == -1
+    # a position below the domain: the reference's size_t index would wrap
+    with pytest.raises(msf.MsfError) as e:
+        ctx.adfstem(spec, c.optics(msf), np.array([-1.0]), np.array([1.0]),
+                    detector=c.detector(msf))
+    assert e.value.code == -5
+    # atomic number outside the parameter table (tem_io.cpp:4768-4781)
+    with pytest.raises(msf.MsfError) as e:
+        ctx.specimen(c.grid(msf), q, np.full(10, 104, np.uint32), 1.0)
+    assert e.value.code == -1
+    # a transform length that is not compiled in
+    g = msf.make_grid(100, 100, 8.0, 8.0, 4.0, 300e3)
+    with pytest.raises(msf.MsfError) as e:
+        ctx.specimen(g, q, Z, 1.0)
+    assert e.value.code == -2
+    # the context still works after the failures
+    img, _ = ctx.adfstem(spec, c.optics(msf), np.array([1.0]), np.array([1.0]),
+                         detector=c.detector(msf))
+    assert np.isfinite(img).all()
+
+
+def test_graph_replay_matches_stream_launches(msf, ctx):
+    """The captured job graph gives bit-identical results on every replay,
+    with eager launches and new positions of the same count in between."""
+    q, Z = random_atoms(60, (12.0, 12.0, 8.0), seed=17, species=(14, 29))
+    c = Case(nx=64, ny=64, cell=(12.0, 12.0, 8.0), voltage=300e3, q=q, Z=Z, min_slice=1.5,
+             alpha_max=0.02, fem=True, dr=0.7)
+    spec = c.gpu_specimen(msf, ctx)
+    rasters = [(np.linspace(0.0, 11.0, 6), np.linspace(0.5, 11.5, 5)),
+               (np.linspace(1.3, 10.0, 6), np.linspace(0.1, 9.5, 5))]
+
+    def run(xp, yp, graph):
+        ctx.configure(batch=8, graph=graph)
+        out = ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
+                          fem=c.femcfg(msf))
+        return out, ctx.stats()["graph"]
+
+    try:
+        for rep in range(3):
+            for xp, yp in rasters:
+                (img_e, v_e), st_e = run(xp, yp, graph=False)
+                assert st_e == 0
+                (img_g, v_g), st_g = run(xp, yp, graph=True)
+                assert st_g == (1 if rep == 0 and xp is rasters[0][0] else 2)
+                np.testing.assert_array_equal(img_g, img_e)
+                np.testing.assert_array_equal(v_g, v_e)
+    finally:
+        ctx.configure()
+
+
+def test_exit_waves_for_explicit_shifts(msf, ctx):
+    """msf_exit_waves_shifted on the first and last raster positions equals
+    those positions of a whole-raster msf_exit_waves, bit for bit."""
+    q, Z = random_atoms(40, (12.0, 12.0, 6.0), seed=23, species=(14, 29))
+    c = Case(nx=64, ny=64, cell=(12.0, 12.0, 6.0), voltage=300e3, q=q, Z=Z, min_slice=1.5,
+             alpha_max=0.02)
+    xp = np.array([0.5, 4.0, 11.99])
+    yp = np.array([2.0, 11.97])
+    spec = c.gpu_specimen(msf, ctx)
+    full = ctx.exit_waves(spec, c.optics(msf), xp, yp)
+    sh = msf.probe_shifts(c.grid(msf), xp, yp)
+    sel = [0, xp.size * yp.size - 1]
+    part = ctx.exit_waves_shifted(spec, c.optics(msf), sh[sel])
+    np.testing.assert_array_equal(part, full[sel])
