@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -98,6 +99,7 @@ struct msf_ctx {
     msf_stats stats{};
     std::map<int, DevBuf> tw32, tw64, twnat;
     RunPlan plan;
+    std::set<msf_specimen*> specimens; // live specimens built on this context
 };
 
 struct msf_specimen {
@@ -255,6 +257,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
 {
     const double t_start = now_ms();
     if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+    if (sp->ctx != c) return fail(c, MSF_ERR_INVALID, "specimen was built on another context");
     if (int r = check_optics(c, o)) return r;
     if (shifts.empty()) return fail(c, MSF_ERR_INVALID, "empty raster");
     const msf::Domain& d = sp->dom;
@@ -600,6 +603,13 @@ int msf_destroy(msf_ctx* c)
     if (!c) return MSF_ERR_INVALID;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    // an instantiated graph refers to the stream it was launched on: destroy
+    // it (and every buffer its kernels use) before the stream
+    c->plan.drop_graph();
+    // specimens may outlive their context: detach them so that a later
+    // msf_specimen_free only releases their own device memory
+    for (msf_specimen* sp : c->specimens) sp->ctx = nullptr;
+    c->specimens.clear();
     c->tw32.clear();
     c->tw64.clear();
     c->twnat.clear();
@@ -806,6 +816,7 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
     }
     if (!e) e = hipStreamSynchronize(c->stream);
     if (e) return bail(fail(c, MSF_ERR_DEVICE, std::string("transmission build: ") + hipGetErrorString(e)));
+    c->specimens.insert(sp);
     *out = sp;
     return MSF_OK;
 }
@@ -816,7 +827,11 @@ int msf_specimen_free(msf_specimen* sp)
     if (sp->ctx) {
         (void)hipSetDevice(sp->ctx->device);
         (void)hipStreamSynchronize(sp->ctx->stream);
-        if (sp->ctx->plan.spec_id == sp->id) sp->ctx->plan.valid = false;
+        if (sp->ctx->plan.spec_id == sp->id) {
+            sp->ctx->plan.valid = false;
+            sp->ctx->plan.drop_graph();
+        }
+        sp->ctx->specimens.erase(sp);
     }
     delete sp;
     return MSF_OK;
@@ -838,6 +853,7 @@ int msf_specimen_slice_info(const msf_specimen* sp, double* lower, double* thick
 int msf_specimen_transmission(const msf_specimen* sp, uint32_t slice, double* t)
 {
     if (!sp || !t || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
+    if (!sp->ctx) return MSF_ERR_INVALID; // its context was destroyed
     const size_t M = (size_t)sp->dom.nx * sp->dom.ny;
     (void)hipSetDevice(sp->ctx->device);
     if (hipMemcpy(t, sp->t64.as<double2>() + slice * M, M * sizeof(double2), hipMemcpyDeviceToHost) !=

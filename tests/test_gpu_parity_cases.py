@@ -172,7 +172,7 @@ def test_graph_replay_matches_stream_launches(msf, ctx):
     with eager launches and new positions of the same count in between."""
     q, Z = random_atoms(60, (12.0, 12.0, 8.0), seed=17, species=(14, 29))
     c = Case(nx=64, ny=64, cell=(12.0, 12.0, 8.0), voltage=300e3, q=q, Z=Z, min_slice=1.5,
-             alpha_max=0.02, fem=True, dr=0.7)
+             alpha_max=0.02, inner=0.02, outer=0.2, fem=True, dr=0.7)
     spec = c.gpu_specimen(msf, ctx)
     rasters = [(np.linspace(0.0, 11.0, 6), np.linspace(0.5, 11.5, 5)),
                (np.linspace(1.3, 10.0, 6), np.linspace(0.1, 9.5, 5))]
@@ -210,3 +210,19 @@ def test_exit_waves_for_explicit_shifts(msf, ctx):
     sel = [0, xp.size * yp.size - 1]
     part = ctx.exit_waves_shifted(spec, c.optics(msf), sh[sel])
     np.testing.assert_array_equal(part, full[sel])
+
+
+def test_specimen_may_outlive_its_context(msf):
+    """Destroying a context detaches its specimens: freeing one afterwards
+    only releases its own memory, and queries on it fail cleanly."""
+    q, Z = random_atoms(10, (8.0, 8.0, 4.0), seed=2)
+    c = Case(nx=64, ny=64, cell=(8.0, 8.0, 4.0), voltage=300e3, q=q, Z=Z)
+    own = msf.Context(0)
+    spec = c.gpu_specimen(msf, own)
+    img, _ = own.adfstem(spec, c.optics(msf), np.array([1.0, 2.0]), np.array([1.0]),
+                         detector=msf.Detector(1, 0.02, 0.2, 0))
+    assert np.isfinite(img).all()
+    own.close()
+    with pytest.raises(msf.MsfError):
+        spec.transmission(0)
+    spec.free()
