@@ -146,11 +146,15 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     us, nbytes = kern[dom]
     achieved = nbytes / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    # HBM traffic of one full-batch launch from the committed counter
+    # summary (FETCH_SIZE/WRITE_SIZE per position, copy-calibrated)
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"{dom}_cuzr512_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            per_pos = json.load(f).get("hbm_bytes_per_position")
+        if per_pos:
+            traffic = float(per_pos) * st["batch"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "bytes_per_launch": nbytes, "avg_us": round(us, 2),
@@ -162,7 +166,7 @@ def main():
     plain = None
     if not args.no_plain:
         n = min(args.plain_positions, P)
-        ctx.configure(batch=1)
+        ctx.configure(batch=1, graph=False)   # one position per launch, stream launches
         xs = np.resize(xp, n)
         ctx.adfstem(spec, optics, xs[:1], yp[:1], fem=fem)
         t0 = time.perf_counter()

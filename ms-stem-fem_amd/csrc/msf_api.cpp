@@ -360,14 +360,19 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     if (int r = twiddles(c, nx, &twx, nullptr, &twx_nat)) return r;
     if (int r = twiddles(c, ny, &twy, nullptr, &twy_nat)) return r;
 
-    // batch depth: the fewest batches whose wave buffer stays within ~256 MB
-    // (the Infinity Cache), spread evenly over the raster
+    // batch depth: the fewest batches whose wave buffer stays within 96 MiB,
+    // spread evenly over the raster.  The buffer, the t and u0 slices and the
+    // partial sums then stay resident in the 256 MB Infinity Cache across the
+    // two passes of a slice (512^2: 64 positions per batch, measured best of
+    // 64..256).
     int64_t B = c->cfg.batch;
     if (B == 0) {
         const int64_t per_pos = plane * (int64_t)sizeof(float2);
-        const int64_t bmax = std::max<int64_t>(1, (256ll << 20) / per_pos);
+        int64_t bmax = std::max<int64_t>(1, (96ll << 20) / per_pos);
+        if (bmax >= 32) bmax -= bmax % 16; // whole groups of 16 positions per launch
         const int64_t nbat = (P + bmax - 1) / bmax;
         B = (P + nbat - 1) / nbat;
+        if (B > 16 && B % 16) B = std::min<int64_t>(bmax, B + 16 - B % 16);
     }
     B = std::max<int64_t>(1, std::min<int64_t>(B, P));
 
