@@ -81,6 +81,34 @@ def copy_calibration(nbytes):
     log(f"copy calibration: 3 x {nbytes} B device-to-device")
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """Device-to-device hipMemcpyAsync of nbytes timed with HIP events: the
+    streaming rate a plain copy reaches on this GPU (bytes read + written
+    per second), reported beside the roofline as the achievable reference."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    best = None
+    for _ in range(reps):
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        best = ms.value if best is None else min(best, ms.value)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +190,11 @@ def main():
                 | {"first_row_pass": round(pst["first_row_pass_us"], 2),
                    "last_col_pass": round(pst["last_col_pass_us"], 2),
                    "reduce": round(pst["reduce_us"], 2)}}
+
+    copy_gbs = round(copy_bandwidth(), 1)
+    roofline["copy_gbs"] = copy_gbs
+    roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    log(f"device copy: {copy_gbs:.0f} GB/s; {dom}: {achieved:.0f} GB/s of compulsory traffic")
 
     plain = None
     if not args.no_plain:
