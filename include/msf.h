@@ -170,6 +170,32 @@ int msf_run_stem(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optic
 int msf_probe_shifts(const msf_grid* grid, const double* x_p, uint32_t n_xp, const double* y_p,
                      uint32_t n_yp, int32_t* shifts);
 
+/* FEM outputs of one run, each n_bins doubles (msf_fem_bins) or NULL when
+ * not wanted.  Modes of adfstem.cpp:2489-3605:
+ *   v_omega       --V_omega (d1): N sum r^2 / (sum r)^2 - 1 of the ring means r of |psi|
+ *   vbar_r        --Vbar_r (d2): mean over positions of <|psi|^2>_ring / <|psi|>_ring^2 - 1
+ *   v_re          --V_re (d3): ring mean of <|psi|^2> over (ring mean of <|psi|>)^2, minus 1
+ *   omega_vimage  --Omega_vimage (d4): ring mean of the variance image <|psi|^2>/<|psi|>^2 - 1
+ *   v_gt17        --V_gt17: ring mean of <|psi|^2> over ring mean of <|psi|>^2
+ *   v_rbar        --V_rbar: ring mean of <|psi|>^2 over (ring mean of <|psi|>)^2, minus 1
+ * (<.> = mean over positions).  Empty and direct-beam bins follow the
+ * reference (NaN or 0 as its arithmetic gives). */
+typedef struct msf_fem_outputs {
+    double* v_omega;
+    double* vbar_r;
+    double* v_re;
+    double* omega_vimage;
+    double* v_gt17;
+    double* v_rbar;
+} msf_fem_outputs;
+
+/* msf_run_stem with every FEM mode above; fem->enabled must be set for any
+ * FEM output to be computed. */
+int msf_run_stem_fem(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
+                     const msf_detector* det, const msf_fem* fem,
+                     const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
+                     double* adf_image, const msf_fem_outputs* fem_out);
+
 /* Exit waves psi(k) after the last slice for the same raster, FP32 complex,
  * n_xp*n_yp*nx*ny*2 floats in the reference layout (zeros outside the band
  * limit).  Parity entry point: the reference keeps these only in memory. */

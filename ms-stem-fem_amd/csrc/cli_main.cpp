@@ -1,6 +1,6 @@
 // ms-stem-fem -- command line front end over the C ABI (include/msf.h,
 // include/msf_io.h).  Replaces the reference executable's main
-// (tem.cpp:81-1700) for the ADF-STEM and V_omega paths: the same command
+// (tem.cpp:81-1700) for the ADF-STEM and FEM paths: the same command
 // line flags and parameter-file keys (tem_io.cpp:5277-5700, :4063-4500),
 // the same required-parameter rules (check_runtime_flags,
 // tem_io.cpp:5858-6060), defaults, raster, Scherzer conditions and output
@@ -248,8 +248,7 @@ bool check_supported(const Flags& f)
 {
     struct Item { bool on; const char* what; };
     const Item items[] = {
-        {f.d2, "--Vbar_r"}, {f.d3, "--V_re"}, {f.d4, "--Omega_vimage"}, {f.gt17, "--V_gt17"},
-        {f.rva, "--V_rbar"}, {f.correlograph, "--correlograph*"},
+        {f.correlograph, "--correlograph*"},
         {f.bfctem_corrected || f.bfctem_uncorrected, "--bfctem*"},
         {f.mtf_file, "--mtf_file"}, {f.pap_tif, "--paptif"},
         {f.diffraction_output, "diffractionimages"},
@@ -423,6 +422,10 @@ int main(int argc, char** argv)
         std::printf("raster %d %d %.17g\ndetector %.17g %.17g\n", nrx, nry, p.raster_spacing,
                     p.det_inner, p.det_outer);
         std::printf("images %d\nfem %d\n", f.image_output ? 1 : 0, (f.fem && f.d1) ? 1 : 0);
+        std::printf("fem_modes%s%s%s%s%s%s\n", (f.fem && f.d1) ? " V_omega" : "",
+                    (f.fem && f.d2) ? " Vbar_r" : "", (f.fem && f.d3) ? " V_re" : "",
+                    (f.fem && f.d4) ? " Omega_vimage" : "", (f.fem && f.gt17) ? " V_gt17" : "",
+                    (f.fem && f.rva) ? " V_rbar" : "");
         size_t k = 0;
         for (const Mode& m : modes)
             if (m.on) {
@@ -452,7 +455,7 @@ int main(int argc, char** argv)
     det.outer = p.det_outer;
     det.threshold_mode = MSF_ADF_REFERENCE;
     msf_fem fem{};
-    fem.enabled = (f.fem && f.d1) ? 1 : 0;
+    fem.enabled = (f.fem && (f.d1 || f.d2 || f.d3 || f.d4 || f.gt17 || f.rva)) ? 1 : 0;
     fem.dr = p.dr;
     uint32_t nbins = 0;
     std::vector<double> edges;
@@ -472,17 +475,33 @@ int main(int argc, char** argv)
         if (!m.on) continue;
         const msf_optics& o = optics[k++];
         const std::string out = p.prefix + m.suffix;
-        std::vector<double> img(xp.size() * yp.size()), v(std::max<uint32_t>(nbins, 1));
-        if (msf_run_stem(c, spec, &o, &det, &fem, xp.data(), (uint32_t)xp.size(), yp.data(),
-                         (uint32_t)yp.size(), img.data(), v.data()) != MSF_OK)
+        std::vector<double> img(xp.size() * yp.size());
+        // one output array per requested FEM mode, with its file suffix
+        // (adfstem.cpp:2886, :2932, :3311, :3418, :3454, :3601)
+        struct FemFile { bool on; const char* suffix; std::vector<double> v; };
+        FemFile files[6] = {{f.fem && f.d1, "_fem_V_omega", {}}, {f.fem && f.d2, "_fem_Vbar_r", {}},
+                            {f.fem && f.d3, "_fem_V_re", {}}, {f.fem && f.d4, "_fem_Omega_vimage", {}},
+                            {f.fem && f.gt17, "_fem_V_gt17", {}}, {f.fem && f.rva, "_fem_V_rbar", {}}};
+        double* ptr[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        for (int m = 0; m < 6; ++m)
+            if (files[m].on) {
+                files[m].v.assign(std::max<uint32_t>(nbins, 1), 0.0);
+                ptr[m] = files[m].v.data();
+            }
+        const msf_fem_outputs fo{ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5]};
+        if (msf_run_stem_fem(c, spec, &o, &det, &fem, xp.data(), (uint32_t)xp.size(), yp.data(),
+                             (uint32_t)yp.size(), img.data(), &fo) != MSF_OK)
             return die("adfstem", c);
-        if (fem.enabled) { // adfstem.cpp:2822-2891
+        if (files[0].on) // adfstem.cpp:2851-2856
             for (uint32_t b = 0; b < nbins; ++b)
-                if (v[b] < 0.0)
-                    std::cout << " WARNING, fem --V_omega, ftem_d1[" << b << "] < 0.0 : " << v[b] << std::endl;
-            if (msf_write_variance_txt((out + "_fem_V_omega.txt").c_str(), v.data(), edges.data(), nbins))
+                if (files[0].v[b] < 0.0)
+                    std::cout << " WARNING, fem --V_omega, ftem_d1[" << b << "] < 0.0 : " << files[0].v[b]
+                              << std::endl;
+        for (int m = 0; m < 6; ++m)
+            if (files[m].on &&
+                msf_write_variance_txt((out + files[m].suffix + ".txt").c_str(), files[m].v.data(),
+                                       edges.data(), nbins))
                 std::cout << " failed to write variance data to a txt file : " << out << std::endl;
-        }
         if (f.image_output) { // adfstem.cpp:3764-3797, write_tif.cpp:1540-1600
             double mn = img[0], mx = img[0];
             for (double a : img) {

@@ -83,6 +83,13 @@ hipError_t launch_fem_welford(const double* partial, int groups, int batch, int 
                               const int32_t* counts, const uint8_t* zeroed, double* state,
                               hipStream_t s);
 hipError_t launch_fem_finalize(const double* state, int nbins, double* v, hipStream_t s);
+// Vbar_r accumulation over a batch (per-position ring ratios, raster order)
+hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbins, int off_abs,
+                           int off_sqr, const int32_t* counts, const uint8_t* zeroed,
+                           double* vsum, hipStream_t s);
+// per-pixel FP64 sums of |psi| and |psi|^2 over the positions of a batch
+hipError_t launch_pixel_moments(const float2* psi, int64_t plane, int batch, double* s1,
+                                double* s2, hipStream_t s);
 // compact exit wave -> full reference layout (FP32)
 hipError_t launch_expand_exit(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
                               int batch, float2* out, hipStream_t s);

@@ -589,6 +589,66 @@ hipError_t launch_fem_finalize(const double* state, int nbins, double* v, hipStr
     return hipGetLastError();
 }
 
+// Vbar_r (adfstem.cpp:2688-2750): per position and bin
+//   ((r2 / c) / ((r1 r1) / (c c))) - 1
+// with r1, r2 the ring sums of |psi| and |psi|^2 (zero on the direct beam)
+// and c*c an integer product, summed over positions in raster order.
+__global__ void fem_vbar_kernel(const double* partial, int groups, int batch, int nbins,
+                                int off_abs, int off_sqr, const int32_t* counts,
+                                const uint8_t* zeroed, double* vsum)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbins) return;
+    const int c = counts[k];
+    const double cc = (double)(c * c);
+    const bool z = zeroed[k] != 0;
+    double acc = vsum[k];
+    for (int b = 0; b < batch; ++b) {
+        const double r1 = z ? 0.0 : partial[(int64_t)b * groups + off_abs + k];
+        const double r2 = z ? 0.0 : partial[(int64_t)b * groups + off_sqr + k];
+        acc += ((r2 / (double)c) / ((r1 * r1) / cc)) - 1.0;
+    }
+    vsum[k] = acc;
+}
+
+hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbins, int off_abs,
+                           int off_sqr, const int32_t* counts, const uint8_t* zeroed,
+                           double* vsum, hipStream_t s)
+{
+    if (nbins <= 0 || batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fem_vbar_kernel, dim3((nbins + 255) / 256), dim3(256), 0, s, partial,
+                       groups, batch, nbins, off_abs, off_sqr, counts, zeroed, vsum);
+    return hipGetLastError();
+}
+
+// Per-pixel sums over positions of |psi| and |psi|^2 in raster order, FP64
+// (adfstem.cpp:2753-2781), on the compact plane of the exit waves.
+__global__ void pixel_moments_kernel(const float2* psi, int64_t plane, int batch, double* s1,
+                                     double* s2)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= plane) return;
+    double a = s1[p], q = s2[p];
+    for (int b = 0; b < batch; ++b) {
+        const float2 v = psi[(int64_t)b * plane + p];
+        const double re = v.x, im = v.y;
+        const double m2 = re * re + im * im;
+        a += sqrt(m2);
+        q += m2;
+    }
+    s1[p] = a;
+    s2[p] = q;
+}
+
+hipError_t launch_pixel_moments(const float2* psi, int64_t plane, int batch, double* s1,
+                                double* s2, hipStream_t s)
+{
+    if (plane <= 0 || batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pixel_moments_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0,
+                       s, psi, plane, batch, s1, s2);
+    return hipGetLastError();
+}
+
 __global__ void expand_exit_kernel(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
                                    int batch, float2* out)
 {

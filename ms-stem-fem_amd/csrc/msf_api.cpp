@@ -73,8 +73,11 @@ struct RunPlan {
     DevBuf U0, Psi0;                          // vacuum waves
     DevBuf seg_pix, tile_off, seg_kind, seg_of_group, cnt, zero;  // reduction groups
     int n_tiles = 0;
+    bool fem_sqr = false;                     // groups carry the |psi|^2 FEM rings (Vbar_r)
+    msf::Groups hg;                           // host copy for the per-pixel FEM modes
     int64_t cap_b = 0, cap_p = 0;
     DevBuf psi, part, segp, state, femv, adf, shift, exit;
+    DevBuf vsum, pm1, pm2;                    // Vbar_r sums; per-pixel |psi|, |psi|^2 sums
     // the job's launch sequence as an instantiated HIP graph, replayed while
     // every pointer and size it was captured with is unchanged
     hipGraphExec_t graph = nullptr;
@@ -228,9 +231,53 @@ int check_optics(msf_ctx* c, const msf_optics* o)
 
 struct RunOut {
     double* adf = nullptr;
-    double* fem = nullptr;
+    double* fem = nullptr;      // V_omega (d1)
+    double* vbar_r = nullptr;   // d2
+    double* v_re = nullptr;     // d3
+    double* omega_vimage = nullptr; // d4
+    double* v_gt17 = nullptr;
+    double* v_rbar = nullptr;   // rva
     float* exit = nullptr;
 };
+
+// V_re, Omega_vimage, V_gt17 and V_rbar from the per-pixel sums of |psi|
+// (s1) and |psi|^2 (s2) over P positions, on the compact plane
+// (adfstem.cpp:2942-2995 averages; :3169-3466 ring ratios; :3468-3605 the
+// variance image and its annular mean).  Ring sums run over every pixel of
+// a bin in the bins' sorted order; pixels the band limit leaves zero
+// contribute zero sums (and 0/0 to the variance image), as in the reference.
+void fem_pixel_modes(const msf::Domain& d, const msf::Groups& g, const std::vector<double>& s1,
+                     const std::vector<double>& s2, int64_t P, double alpha_max, const RunOut& out)
+{
+    const int nb = g.nbins;
+    const double np = (double)P;
+    const double l2 = d.lambda * d.lambda, a2 = alpha_max * alpha_max; // adfstem.cpp:95-96
+    for (int ii = 0; ii < nb; ++ii) {
+        double S2 = 0.0, Q2 = 0.0, S1 = 0.0, D = 0.0;
+        for (int32_t k = g.bin_off[ii]; k < g.bin_off[ii + 1]; ++k) {
+            const int32_t cp = g.bin_compact[k];
+            const double mag = (cp >= 0 ? s1[cp] : 0.0) / np;
+            const double sqr = (cp >= 0 ? s2[cp] : 0.0) / np;
+            S2 += sqr;
+            Q2 += mag * mag;
+            S1 += mag;
+            if (out.omega_vimage) {
+                const int i = g.bin_full[k] / d.ny, j = g.bin_full[k] % d.ny;
+                double v = (sqr / (mag * mag)) - 1.0;
+                const double ksqr = msf::ref_pow2(d.kx[i]) + msf::ref_pow2(d.ky[j]);
+                if (l2 * ksqr <= a2) v = 0.0;
+                D += v;
+            }
+        }
+        if (g.zeroed[ii]) S2 = Q2 = S1 = D = 0.0;
+        const int c = g.counts[ii];
+        const double cc = (double)(c * c);
+        if (out.v_gt17) out.v_gt17[ii] = (S2 / c) / (Q2 / c);
+        if (out.v_re) out.v_re[ii] = ((S2 / c) / (S1 * S1 / cc)) - 1.0;
+        if (out.v_rbar) out.v_rbar[ii] = ((Q2 / c) / (S1 * S1 / cc)) - 1.0;
+        if (out.omega_vimage) out.omega_vimage[ii] = D / c;
+    }
+}
 
 // Probe shifts in raster order (x outer, y inner), each index persisting
 // from the previous position when no grid point qualifies
@@ -263,7 +310,11 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const msf::Domain& d = sp->dom;
     const int nx = d.nx, ny = d.ny, S = sp->n_slices;
     const bool want_adf = det && det->enabled;
-    const bool want_fem = fem && fem->enabled;
+    const bool fem_on = fem && fem->enabled;
+    const bool want_d1 = fem_on && out.fem;
+    const bool want_d2 = fem_on && out.vbar_r;
+    const bool want_pix = fem_on && (out.v_re || out.omega_vimage || out.v_gt17 || out.v_rbar);
+    const bool want_fem = want_d1 || want_d2 || want_pix;
     if (want_adf) {
         if (det->inner < 0.0 || det->outer < 0.0)
             return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
@@ -293,7 +344,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const bool same_optics = pl.valid && pl.spec_id == sp->id &&
                              std::memcmp(&pl.optics, o, sizeof(msf_optics)) == 0;
     const bool same_groups = same_optics && std::memcmp(&pl.det, &dkey, sizeof dkey) == 0 &&
-                             std::memcmp(&pl.fem, &fkey, sizeof fkey) == 0;
+                             std::memcmp(&pl.fem, &fkey, sizeof fkey) == 0 &&
+                             pl.fem_sqr == want_d2;
     if (!same_optics || !same_groups) pl.valid = false;
 
     if (!same_groups) {
@@ -307,9 +359,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             }
         }
         const msf::Groups g = msf::build_groups(d, keep, ncp, want_adf, k1, k2, want_fem,
-                                                want_fem ? fem->dr : 1.0, o->alpha_max);
+                                                want_fem ? fem->dr : 1.0, o->alpha_max, want_d2);
         pl.G = (int)g.kind.size();
         pl.nbins = g.nbins;
+        pl.fem_sqr = want_d2;
         const msf::SegmentTiles st = msf::build_segments(g, 32);
         pl.n_tiles = st.n_tiles;
         HIPCHK(c, upload(pl.seg_pix, st.pix, c->stream));
@@ -319,6 +372,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, upload(pl.cnt, g.counts, c->stream));
         HIPCHK(c, upload(pl.zero, g.zeroed, c->stream));
         pl.cap_b = 0; // partial-sum buffers depend on G
+        pl.hg = g;
     }
     const int G = pl.G;
     const int nbins = pl.nbins;
@@ -393,8 +447,20 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     if (pl.state.bytes < (size_t)std::max(nbins, 1) * 3 * sizeof(double)) {
         HIPCHK(c, pl.state.alloc((size_t)std::max(nbins, 1) * 3 * sizeof(double)));
         HIPCHK(c, pl.femv.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
+        HIPCHK(c, pl.vsum.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
     }
+    if (want_pix && pl.pm1.bytes < (size_t)plane * sizeof(double)) {
+        HIPCHK(c, pl.pm1.alloc((size_t)plane * sizeof(double)));
+        HIPCHK(c, pl.pm2.alloc((size_t)plane * sizeof(double)));
+    }
+    // accumulators start at zero (outside any captured graph)
     HIPCHK(c, hipMemsetAsync(pl.state.p, 0, (size_t)std::max(nbins, 1) * 3 * sizeof(double), c->stream));
+    if (want_d2)
+        HIPCHK(c, hipMemsetAsync(pl.vsum.p, 0, (size_t)std::max(nbins, 1) * sizeof(double), c->stream));
+    if (want_pix) {
+        HIPCHK(c, hipMemsetAsync(pl.pm1.p, 0, (size_t)plane * sizeof(double), c->stream));
+        HIPCHK(c, hipMemsetAsync(pl.pm2.p, 0, (size_t)plane * sizeof(double), c->stream));
+    }
     HIPCHK(c, hipMemcpyAsync(pl.shift.p, shifts.data(), (size_t)P * sizeof(int2),
                              hipMemcpyHostToDevice, c->stream));
     if (out.exit && pl.exit.bytes < (size_t)B * M * sizeof(float2))
@@ -482,10 +548,17 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         if (want_adf)
             HIPCHK(c, msf::launch_collect_adf(dpart.as<double>(), G, nb, dadf.as<double>() + b0,
                                               c->stream));
-        if (want_fem)
+        if (want_d1)
             HIPCHK(c, msf::launch_fem_welford(dpart.as<double>(), G, nb, nbins, dcnt.as<int32_t>(),
                                               dzero.as<uint8_t>(), dstate.as<double>(),
                                               c->stream));
+        if (want_d2)
+            HIPCHK(c, msf::launch_fem_vbar(dpart.as<double>(), G, nb, nbins, pl.hg.fem_abs,
+                                           pl.hg.fem_sqr, dcnt.as<int32_t>(), dzero.as<uint8_t>(),
+                                           pl.vsum.as<double>(), c->stream));
+        if (want_pix)
+            HIPCHK(c, msf::launch_pixel_moments(dpsi.as<float2>(), plane, nb, pl.pm1.as<double>(),
+                                                pl.pm2.as<double>(), c->stream));
         if (out.exit) {
             HIPCHK(c, msf::launch_expand_exit(dpsi.as<float2>(), nx, ny, ncp, ka, kb, nb,
                                               dexit.as<float2>(), c->stream));
@@ -494,7 +567,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                                      c->stream));
         }
     }
-    if (want_fem)
+    if (want_d1)
         HIPCHK(c, msf::launch_fem_finalize(dstate.as<double>(), nbins, dfemv.as<double>(), c->stream));
     return MSF_OK;
     };
@@ -507,7 +580,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     if (!prof && !out.exit && !c->cfg.no_graph) {
         const std::vector<uint64_t> sig = {
             sp->id, (uint64_t)P, (uint64_t)B, (uint64_t)tmode, (uint64_t)want_adf,
-            (uint64_t)want_fem, (uint64_t)(uintptr_t)dpsi.p, (uint64_t)(uintptr_t)dpart.p,
+            (uint64_t)want_d1 | (uint64_t)want_d2 << 1 | (uint64_t)want_pix << 2,
+            (uint64_t)(uintptr_t)pl.vsum.p, (uint64_t)(uintptr_t)pl.pm1.p,
+            (uint64_t)(uintptr_t)pl.pm2.p, (uint64_t)pl.hg.fem_abs, (uint64_t)pl.hg.fem_sqr,
+            (uint64_t)plane, (uint64_t)(uintptr_t)dpsi.p, (uint64_t)(uintptr_t)dpart.p,
             (uint64_t)(uintptr_t)pl.segp.p, (uint64_t)(uintptr_t)dadf.p,
             (uint64_t)(uintptr_t)dshift.p, (uint64_t)(uintptr_t)dstate.p,
             (uint64_t)(uintptr_t)dfemv.p, (uint64_t)(uintptr_t)dU0.p, (uint64_t)(uintptr_t)dPsi0.p,
@@ -545,8 +621,19 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
 
     if (want_adf && out.adf)
         HIPCHK(c, hipMemcpy(out.adf, dadf.p, (size_t)P * sizeof(double), hipMemcpyDeviceToHost));
-    if (want_fem && out.fem && nbins > 0)
+    if (want_d1 && nbins > 0)
         HIPCHK(c, hipMemcpy(out.fem, dfemv.p, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost));
+    if (want_d2 && nbins > 0) { // adfstem.cpp:2896-2915: the mean over positions
+        std::vector<double> vs(nbins);
+        HIPCHK(c, hipMemcpy(vs.data(), pl.vsum.p, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nbins; ++k) out.vbar_r[k] = vs[k] / (double)P;
+    }
+    if (want_pix && nbins > 0) {
+        std::vector<double> s1((size_t)plane), s2((size_t)plane);
+        HIPCHK(c, hipMemcpy(s1.data(), pl.pm1.p, (size_t)plane * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(s2.data(), pl.pm2.p, (size_t)plane * sizeof(double), hipMemcpyDeviceToHost));
+        fem_pixel_modes(d, pl.hg, s1, s2, P, o->alpha_max, out);
+    }
 
     msf_stats st{};
     st.setup_ms = t_loop - t_start;
@@ -917,6 +1004,32 @@ int msf_run_stem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const 
     RunOut out;
     out.adf = adf_image;
     out.fem = fem_v;
+    return run_internal(c, sp, o, det, fem, shifts, out);
+}
+
+int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
+                     const msf_detector* det, const msf_fem* fem, const double* x_p,
+                     uint32_t n_xp, const double* y_p, uint32_t n_yp, double* adf_image,
+                     const msf_fem_outputs* fo)
+{
+    if (!c) return MSF_ERR_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
+    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+    std::vector<int2> shifts;
+    if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
+        return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
+    RunOut out;
+    out.adf = adf_image;
+    if (fo) {
+        out.fem = fo->v_omega;
+        out.vbar_r = fo->vbar_r;
+        out.v_re = fo->v_re;
+        out.omega_vimage = fo->omega_vimage;
+        out.v_gt17 = fo->v_gt17;
+        out.v_rbar = fo->v_rbar;
+    }
     return run_internal(c, sp, o, det, fem, shifts, out);
 }
 

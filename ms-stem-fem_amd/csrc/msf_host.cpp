@@ -167,7 +167,7 @@ int fem_bin_count(const Domain& d, double dr, double& delta_k)
 }
 
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
-                    double k2, bool fem, double dr, double alpha_max)
+                    double k2, bool fem, double dr, double alpha_max, bool fem_sqr)
 {
     Groups g;
     const int nx = d.nx, ny = d.ny;
@@ -204,6 +204,8 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
                              return a.first < b.first;
                          });
         size_t it = 0;
+        g.fem_abs = (int)g.kind.size();
+        g.bin_off.push_back(0);
         for (int ii = 0; ii < g.nbins; ++ii) {
             const double lower = (unsigned int)ii * dk;
             const double upper = (unsigned int)(ii + 1) * dk;
@@ -212,12 +214,27 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
             for (; it < m.size() && m[it].first <= (upper * upper); ++it) {
                 const int i = (int)(m[it].second / ny), j = (int)(m[it].second % ny);
                 ++count;
-                if (kept(i, j)) g.pixels.push_back(compact(i, j));
+                const bool k = kept(i, j);
+                if (k) g.pixels.push_back(compact(i, j));
+                g.bin_full.push_back((int32_t)m[it].second);
+                g.bin_compact.push_back(k ? compact(i, j) : -1);
             }
+            g.bin_off.push_back((int32_t)g.bin_full.size());
             g.counts.push_back(count);
             g.zeroed.push_back(d.lambda * upper <= alpha_max ? 1 : 0);
             g.offsets.push_back((int32_t)g.pixels.size());
             g.kind.push_back(1);
+        }
+        if (fem_sqr) { // the same rings summing |psi|^2 (integrate_out_phi_double, d2)
+            g.fem_sqr = (int)g.kind.size();
+            for (int ii = 0; ii < g.nbins; ++ii) {
+                for (int32_t k = g.offsets[g.fem_abs + ii]; k < g.offsets[g.fem_abs + ii + 1]; ++k) {
+                    const int32_t px = g.pixels[k];
+                    g.pixels.push_back(px);
+                }
+                g.offsets.push_back((int32_t)g.pixels.size());
+                g.kind.push_back(0);
+            }
         }
     }
     return g;

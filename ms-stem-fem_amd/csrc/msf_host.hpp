@@ -104,9 +104,17 @@ struct Groups {
     std::vector<uint8_t> zeroed;  // FEM: direct-beam bins (adfstem.cpp:2518-2522)
     std::vector<double> edges;    // FEM: lower edges ii*delta_k
     int nbins = 0;
+    int fem_abs = -1;             // first group of the FEM |psi| rings (-1: none)
+    int fem_sqr = -1;             // first group of the FEM |psi|^2 rings (-1: none)
+    // every pixel of every FEM bin in the bins' sorted order (bin ii owns
+    // [bin_off[ii], bin_off[ii+1])): full-grid index i*ny + j and compact
+    // offset, -1 where the slice step's band limit leaves the pixel zero
+    std::vector<int32_t> bin_off, bin_full, bin_compact;
 };
+// group 0: the ADF ring; then, with fem, nbins |psi| rings and, with
+// fem_sqr, nbins |psi|^2 rings over the same pixels
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
-                    double k2, bool fem, double dr, double alpha_max);
+                    double k2, bool fem, double dr, double alpha_max, bool fem_sqr = false);
 
 // Groups cut into segments of at most `seg` pixels, 64 segments per wavefront
 // tile, pixel offsets interleaved [tile][k][lane] (-1 pads short segments) so

@@ -59,6 +59,13 @@ class Model(C.Structure):
                 ("zmin", C.c_double)]
 
 
+FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
+
+
+class FemOutputs(C.Structure):
+    _fields_ = [(m, C.c_void_p) for m in FEM_MODES]
+
+
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
                 ("profile_kernels", C.c_int32), ("col_tile", C.c_int32),
@@ -88,7 +95,7 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_scherzer_alphamax", "msf_specimen_build", "msf_specimen_free",
            "msf_specimen_slices", "msf_specimen_slice_info", "msf_specimen_transmission",
            "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves",
-           "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted"]
+           "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -130,6 +137,9 @@ def lib():
                                      C.c_uint32, C.POINTER(C.c_float)]
         L.msf_probe_shifts.argtypes = [C.POINTER(Grid), _dp, C.c_uint32, _dp, C.c_uint32,
                                        C.POINTER(C.c_int32)]
+        L.msf_run_stem_fem.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(Detector),
+                                       C.POINTER(Fem), _dp, C.c_uint32, _dp, C.c_uint32, _dp,
+                                       C.POINTER(FemOutputs)]
         L.msf_exit_waves_shifted.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(C.c_int32),
                                              C.c_uint32, C.POINTER(C.c_float)]
         L.msf_scherzer_cs3.argtypes = [C.c_double, C.c_double]
@@ -323,6 +333,22 @@ class Context:
         if v is not None:
             v = v[:fem_bins(spec.grid, fm.dr).size]
         return img, v
+
+    def adfstem_fem(self, spec, optics, x_p, y_p, fem, modes=FEM_MODES, detector=None):
+        """msf_run_stem_fem: returns (adf image or None, {mode: values [n_bins]})."""
+        x_p = np.ascontiguousarray(x_p, np.float64)
+        y_p = np.ascontiguousarray(y_p, np.float64)
+        det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
+        img = np.zeros(x_p.size * y_p.size) if det.enabled else None
+        nb = fem_bins(spec.grid, fem.dr).size
+        outs = {m: np.zeros(max(nb, 1)) for m in modes}
+        fo = FemOutputs(*[outs[m].ctypes.data if m in outs else None for m in FEM_MODES])
+        self._check(lib().msf_run_stem_fem(self._h, spec._h, C.byref(optics), C.byref(det),
+                                           C.byref(fem), _p(x_p), x_p.size, _p(y_p), y_p.size,
+                                           _p(img), C.byref(fo)), "msf_run_stem_fem")
+        if img is not None:
+            img = img.reshape(x_p.size, y_p.size)
+        return img, {m: v[:nb] for m, v in outs.items()}
 
     def exit_waves_shifted(self, spec, optics, shifts):
         sh = np.ascontiguousarray(shifts, np.int32).reshape(-1, 2)

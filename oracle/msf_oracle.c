@@ -618,6 +618,123 @@ void orc_fem_finalize(const double* sum_r, const double* sum_r2, int nbins,
     }
 }
 
+/* integrate_out_phi_double (variance.cpp:303-510) over the bins of
+ * adfstem.cpp:947-1066, followed by the direct-beam zeroing that every
+ * caller applies (e.g. adfstem.cpp:2709-2715, :3182-3188). */
+static void fem_ring_sums(const double* data, const kmag* m, size_t M, double delta_k,
+                          int nbins, double lambda, double alpha_max, double* radial,
+                          int32_t* counts)
+{
+    size_t it = 0;
+    for (int ii = 0; ii < nbins; ii++) {
+        radial[ii] = 0.0;
+        counts[ii] = 0;
+        const double upper = (unsigned int)(ii + 1) * delta_k;
+        for (; it < M && m[it].v <= (upper * upper); ++it) {
+            radial[ii] += data[m[it].idx];
+            counts[ii] += 1;
+        }
+    }
+    for (int ii = 0; ii < nbins; ii++) {
+        const double upper = (unsigned int)(ii + 1) * delta_k;
+        if (lambda * upper <= alpha_max) radial[ii] = 0.0;
+    }
+}
+
+/* The FEM modes other than V_omega, from the per-position intensities
+ * |psi|^2 (P x nx*ny, raster order):
+ *   Vbar_r  (d2)   adfstem.cpp:2688-2750 accumulate, :2896-2940 finalize
+ *   V_re    (d3), V_gt17, V_rbar (rva)   :2753-2781 accumulate, :2942-2995
+ *                  and :3169-3466 finalize
+ *   Omega_vimage (d4)                    :3468-3605
+ * radial1 holds the per-position |psi| ring sums (already zeroed on the
+ * direct beam).  Any output pointer may be NULL. */
+int orc_fem_modes(const double* mag2, long P, const double* kx, const double* ky, int Nx, int Ny,
+                  double delta_k, int nbins, double lambda, double alpha_max,
+                  const double* radial1, const int32_t* counts1, double* vbar_r, double* v_re,
+                  double* omega_vimage, double* v_gt17, double* v_rbar)
+{
+    const size_t M = (size_t)Nx * Ny;
+    const size_t nb = nbins ? (size_t)nbins : 1;
+    kmag* m = (kmag*)malloc(sizeof(kmag) * M);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const size_t k = (size_t)j + (size_t)i * Ny;
+            m[k].v = kx[i] * kx[i] + ky[j] * ky[j];
+            m[k].idx = (long)k;
+        }
+    qsort(m, M, sizeof(kmag), kmag_cmp);
+    double* r2 = (double*)malloc(sizeof(double) * nb);
+    int32_t* c2 = (int32_t*)malloc(sizeof(int32_t) * nb);
+    if (vbar_r) {
+        double* vsum = (double*)calloc(nb, sizeof(double));
+        for (long p = 0; p < P; p++) {
+            fem_ring_sums(mag2 + (size_t)p * M, m, M, delta_k, nbins, lambda, alpha_max, r2, c2);
+            const double* r1 = radial1 + (size_t)p * nbins;
+            const int32_t* c1 = counts1 + (size_t)p * nbins;
+            for (int ii = 0; ii < nbins; ii++)
+                vsum[ii] += ((r2[ii] / c2[ii]) / ((r1[ii] * r1[ii]) / (c1[ii] * c1[ii]))) - 1.0;
+        }
+        for (int ii = 0; ii < nbins; ii++) vbar_r[ii] = vsum[ii] / (double)P;
+        free(vsum);
+    }
+    if (v_re || omega_vimage || v_gt17 || v_rbar) {
+        /* per-pixel sums over positions in raster order (:2735-2760), then
+         * the averages of :2916-2939 */
+        double* mag_sum = (double*)calloc(M, sizeof(double));
+        double* mag_sqr_sum = (double*)calloc(M, sizeof(double));
+        for (long p = 0; p < P; p++)
+            for (size_t k = 0; k < M; k++) {
+                const double a2 = mag2[(size_t)p * M + k];
+                mag_sum[k] += sqrt(a2);
+                mag_sqr_sum[k] += a2;
+            }
+        double* mag_sum_sqr = (double*)malloc(sizeof(double) * M);
+        for (size_t k = 0; k < M; k++) {
+            mag_sum[k] = mag_sum[k] / (double)P;
+            mag_sum_sqr[k] = mag_sum[k] * mag_sum[k];
+            mag_sqr_sum[k] = mag_sqr_sum[k] / (double)P;
+        }
+        double* S2 = (double*)malloc(sizeof(double) * nb);
+        double* Q2 = (double*)malloc(sizeof(double) * nb);
+        double* S1 = (double*)malloc(sizeof(double) * nb);
+        int32_t* cS2 = (int32_t*)malloc(sizeof(int32_t) * nb);
+        int32_t* cQ2 = (int32_t*)malloc(sizeof(int32_t) * nb);
+        int32_t* cS1 = (int32_t*)malloc(sizeof(int32_t) * nb);
+        fem_ring_sums(mag_sqr_sum, m, M, delta_k, nbins, lambda, alpha_max, S2, cS2);
+        fem_ring_sums(mag_sum_sqr, m, M, delta_k, nbins, lambda, alpha_max, Q2, cQ2);
+        fem_ring_sums(mag_sum, m, M, delta_k, nbins, lambda, alpha_max, S1, cS1);
+        for (int ii = 0; ii < nbins; ii++) {
+            if (v_gt17) v_gt17[ii] = (S2[ii] / cS2[ii]) / (Q2[ii] / cQ2[ii]);
+            if (v_re)
+                v_re[ii] = ((S2[ii] / cS2[ii]) / (S1[ii] * S1[ii] / (cS1[ii] * cS1[ii]))) - 1.0;
+            if (v_rbar)
+                v_rbar[ii] = ((Q2[ii] / cQ2[ii]) / (S1[ii] * S1[ii] / (cS1[ii] * cS1[ii]))) - 1.0;
+        }
+        if (omega_vimage) {
+            /* variance image, zeroed inside the aperture, annular mean */
+            const double l2 = lambda * lambda, a2 = alpha_max * alpha_max;
+            double* var = (double*)malloc(sizeof(double) * M);
+            for (int i = 0; i < Nx; i++)
+                for (int j = 0; j < Ny; j++) {
+                    const size_t k = (size_t)j + (size_t)Ny * i;
+                    var[k] = (mag_sqr_sum[k] / (mag_sum[k] * mag_sum[k])) - 1.0;
+                    const double ksqr = ref_pow2(kx[i]) + ref_pow2(ky[j]);
+                    if (l2 * ksqr <= a2) var[k] = 0.0;
+                }
+            double* D = (double*)malloc(sizeof(double) * nb);
+            int32_t* cD = (int32_t*)malloc(sizeof(int32_t) * nb);
+            fem_ring_sums(var, m, M, delta_k, nbins, lambda, alpha_max, D, cD);
+            for (int ii = 0; ii < nbins; ii++) omega_vimage[ii] = D[ii] / cD[ii];
+            free(var); free(D); free(cD);
+        }
+        free(mag_sum); free(mag_sqr_sum); free(mag_sum_sqr);
+        free(S2); free(Q2); free(S1); free(cS2); free(cQ2); free(cS1);
+    }
+    free(m); free(r2); free(c2);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* tem.cpp:944-1276: unique species, templates (scatterer_pap_LUT),
  * auto-slicing, per-slice propagator and transmission function. */
@@ -696,6 +813,15 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
                 const double* x_p, int nxp, const double* y_p, int nyp,
                 int n_threads, double* adf_image, double* fem_v, double* exit_psi)
 {
+    return orc_adfstem_fem(job, spec, x_p, nxp, y_p, nyp, n_threads, adf_image, fem_v, NULL,
+                           NULL, NULL, NULL, NULL, exit_psi);
+}
+
+int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
+                    const double* x_p, int nxp, const double* y_p, int nyp, int n_threads,
+                    double* adf_image, double* fem_v, double* vbar_r, double* v_re,
+                    double* omega_vimage, double* v_gt17, double* v_rbar, double* exit_psi)
+{
     const int Nx = job->nx, Ny = job->ny;
     const size_t M = (size_t)Nx * Ny;
     const long P = (long)nxp * nyp;
@@ -718,6 +844,8 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
     if (job->fem) nbins = orc_fem_nbins(kx, ky, job->dr, bwt, &delta_k);
     double* radial = (double*)malloc(sizeof(double) * (size_t)P * (nbins ? nbins : 1));
     int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)P * (nbins ? nbins : 1));
+    const int other_modes = job->fem && (vbar_r || v_re || omega_vimage || v_gt17 || v_rbar);
+    double* mag2 = other_modes ? (double*)malloc(sizeof(double) * (size_t)P * M) : NULL;
     double k1 = job->det_inner / lambda, k2 = job->det_outer / lambda;
     if (job->adf_mode == 1) { k1 = k1 * k1; k2 = k2 * k2; }
     int rc = 0;
@@ -740,6 +868,9 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
                 orc_fem_radial(psi, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max,
                                radial + (size_t)p * nbins, counts + (size_t)p * nbins);
             if (exit_psi) memcpy(exit_psi + 2 * M * p, psi, sizeof(double) * 2 * M);
+            if (mag2)
+                for (size_t k = 0; k < M; k++)
+                    mag2[(size_t)p * M + k] = psi[2 * k] * psi[2 * k] + psi[2 * k + 1] * psi[2 * k + 1];
         }
         free(psi);
     }
@@ -755,6 +886,11 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
             }
         orc_fem_finalize(s1, s2, nbins, P, fem_v);
         free(s1); free(s2);
+    }
+    if (mag2) {
+        orc_fem_modes(mag2, P, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max, radial,
+                      counts, vbar_r, v_re, omega_vimage, v_gt17, v_rbar);
+        free(mag2);
     }
     free(kx); free(ky); free(xx); free(yy); free(probe); free(sx); free(sy);
     free(radial); free(counts);

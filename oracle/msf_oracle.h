@@ -92,6 +92,10 @@ int orc_fem_radial(const double* psi, const double* kx, const double* ky, int nx
                    double* radial, int32_t* counts);
 void orc_fem_finalize(const double* sum_r, const double* sum_r2, int nbins,
                       int64_t n_positions, double* v);
+int orc_fem_modes(const double* mag2, long n_positions, const double* kx, const double* ky,
+                  int nx, int ny, double delta_k, int nbins, double lambda, double alpha_max,
+                  const double* radial1, const int32_t* counts1, double* vbar_r, double* v_re,
+                  double* omega_vimage, double* v_gt17, double* v_rbar);
 
 /* ---- whole job (tem.cpp setup + adfstem()) ---- */
 typedef struct orc_specimen {
@@ -109,6 +113,12 @@ void orc_specimen_free(orc_specimen* s);
 int orc_adfstem(const orc_job* job, const orc_specimen* spec,
                 const double* x_p, int nxp, const double* y_p, int nyp,
                 int n_threads, double* adf_image, double* fem_v, double* exit_psi);
+/* the same with the other FEM modes (Vbar_r, V_re, Omega_vimage, V_gt17,
+ * V_rbar); each output may be NULL */
+int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
+                    const double* x_p, int nxp, const double* y_p, int nyp, int n_threads,
+                    double* adf_image, double* fem_v, double* vbar_r, double* v_re,
+                    double* omega_vimage, double* v_gt17, double* v_rbar, double* exit_psi);
 
 #ifdef __cplusplus
 }

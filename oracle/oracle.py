@@ -97,6 +97,10 @@ def lib():
         L.orc_specimen_free.argtypes = [C.POINTER(_Spec)]
         L.orc_adfstem.argtypes = [C.POINTER(Job), C.POINTER(_Spec), _dp, C.c_int, _dp, C.c_int,
                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_adfstem_fem.argtypes = [C.POINTER(Job), C.POINTER(_Spec), _dp, C.c_int, _dp,
+                                      C.c_int, C.c_int] + [C.c_void_p] * 8
+        L.orc_fem_modes.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                    C.c_int, C.c_double, C.c_double, _dp, _i32] + [C.c_void_p] * 5
         _lib = L
     return _lib
 
@@ -308,3 +312,40 @@ def adfstem(job, spec, x_p, y_p, n_threads=0, exit_waves=False):
     if v is not None:
         v = v[:nb]
     return img, v, ex
+
+
+FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
+
+
+def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0):
+    """Whole raster with the FEM modes of adfstem.cpp:2489-3605: returns
+    (adf image or None, {mode: values [nbins]})."""
+    x_p = np.ascontiguousarray(x_p, np.float64)
+    y_p = np.ascontiguousarray(y_p, np.float64)
+    P = x_p.size * y_p.size
+    kx, ky, _, _ = domains(job.nx, job.ny, job.xperiod, job.yperiod)
+    _, bwt = bwcutoffs(job.nx, job.ny, job.xperiod, job.yperiod)
+    nb, _ = fem_nbins(kx, ky, job.dr, bwt)
+    img = np.zeros(P) if job.adf else None
+    outs = {m: np.zeros(max(nb, 1)) for m in modes}
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+    rc = lib().orc_adfstem_fem(C.byref(job), C.byref(spec._s), x_p, x_p.size, y_p, y_p.size,
+                               n_threads, ptr(img), *[ptr(outs.get(m)) for m in FEM_MODES], None)
+    assert rc == 0, rc
+    if img is not None:
+        img = img.reshape(x_p.size, y_p.size)
+    return img, {m: v[:nb] for m, v in outs.items()}
+
+
+def fem_modes(mag2, kx, ky, delta_k, nbins, lam, alpha_max, radial1, counts1):
+    """orc_fem_modes on given per-position intensities mag2 [P, nx, ny]."""
+    mag2 = np.ascontiguousarray(mag2, np.float64)
+    P, nx, ny = mag2.shape
+    outs = {m: np.zeros(max(nbins, 1)) for m in FEM_MODES[1:]}
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = lib().orc_fem_modes(mag2.reshape(-1), P, kx, ky, nx, ny, delta_k, nbins, lam, alpha_max,
+                             np.ascontiguousarray(radial1, np.float64).reshape(-1),
+                             np.ascontiguousarray(counts1, np.int32).reshape(-1),
+                             *[ptr(outs[m]) for m in FEM_MODES[1:]])
+    assert rc == 0
+    return {m: v[:nbins] for m, v in outs.items()}

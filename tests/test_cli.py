@@ -113,7 +113,8 @@ def test_dupe_replicates_for_fem_only(tmp_path, model):
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
       "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--bogus"], "unknown flag"),
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorrfem",
-      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--V_re"], "mode outside the build"),
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--correlograph"],
+     "mode outside the build"),
     (["-m", "64", "64", "3e5", "-a", "missing.xyz", "--output_prefix", "o", "--adfstemuncorr",
       "--cs3", "0", "--defocus", "0", "--alphamax", "0.01"], "unreadable model"),
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
@@ -138,3 +139,17 @@ def test_parameter_file_rejects_unknown_key_and_drops_unterminated_last_line(tmp
                   "output_prefix o\nadfstemuncorr\ncs3 0\ndefocus 0\nalphamax 0.01")
     rc, _, out = run(["--parameter_file", str(pf), "--plan_only"], tmp_path)
     assert rc == 1 and "lacking required parameters" in out
+
+
+def test_fem_mode_selection(tmp_path, model):
+    base = ["-m", "64", "64", "300000", "-a", model, "--output_prefix", "o", "--cs3", "0",
+            "--defocus", "0", "--alphamax", "0.01", "--plan_only"]
+    # V_omega is the default only when no FEM mode is named (tem_io.cpp:5705-5712)
+    rc, plan, out = run(base + ["--adfstemuncorrfem"], tmp_path)
+    assert rc == 0, out
+    assert plan["fem_modes"] == ["V_omega"]
+    rc, plan, _ = run(base + ["--adfstemuncorrfem", "--V_re", "--V_gt17"], tmp_path)
+    assert plan["fem_modes"][0].split() == ["V_re", "V_gt17"]
+    # the mode flags do nothing without an FEM run mode
+    rc, plan, _ = run(base + ["--adfstemuncorr", "--Vbar_r"], tmp_path)
+    assert plan["fem_modes"][0].split() == []

@@ -41,7 +41,8 @@ def test_cli_adfstem_fem_run(tmp_path, O):
     (tmp_path / "m.xyz").write_text("\n".join(lines) + "\n")
     args = [CLI, "-m", "128", "128", "300000", "-a", "m.xyz", "--output_prefix", "run",
             "--adfstemuncorrfem", "--cs3", "1.0e6", "--defocus", "25", "--alphamax", "0.02",
-            "--rasterspacing", "2.5", "--minslice", "1.5", "--dr", "0.5", "--images"]
+            "--rasterspacing", "2.5", "--minslice", "1.5", "--dr", "0.5", "--images",
+            "--V_omega", "--Vbar_r", "--V_re", "--Omega_vimage", "--V_gt17", "--V_rbar"]
     print("cli:", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -53,6 +54,8 @@ def test_cli_adfstem_fem_run(tmp_path, O):
              cs3=1.0e6, defocus=25.0, alpha_max=0.02, inner=0.040, outer=0.200, fem=True, dr=0.5)
     xp = np.arange(4) * 2.5
     _, (img_o, v_o, ex_o) = c.run_oracle(O, xp, xp, exit_waves=True)
+    job = c.oracle_job(O)
+    _, modes_o = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, xp)
 
     pre = tmp_path / "run_adfstem_uncorr"
     raw = open(f"{pre}_stem_image_output.f64", "rb").read()
@@ -65,6 +68,12 @@ def test_cli_adfstem_fem_run(tmp_path, O):
     assert txt.shape == (v_o.size, 2)
     np.testing.assert_allclose(txt[:, 0], np.arange(v_o.size) * txt[1, 0], rtol=1e-9)
     assert rel_l2(txt[:, 1], v_o) < 1e-4
+
+    for mode, suffix in (("vbar_r", "Vbar_r"), ("v_re", "V_re"), ("omega_vimage", "Omega_vimage"),
+                         ("v_gt17", "V_gt17"), ("v_rbar", "V_rbar")):
+        t = np.loadtxt(f"{pre}_fem_{suffix}.txt")
+        np.testing.assert_array_equal(t[:, 0], txt[:, 0])
+        assert rel_l2(t[:, 1], modes_o[mode]) < 1e-4, mode
 
     lo, hi = img.min(), img.max()
     lin = _tiff(f"{pre}_stem_image_output.tif")
