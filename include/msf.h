@@ -28,6 +28,7 @@ extern "C" {
 #endif
 
 #define MSF_OK 0
+#define MSF_PENDING 1            /* msf_poll: the job is queued or running */
 #define MSF_ERR_INVALID (-1)     /* bad argument (flags.fail paths of check_runtime_flags) */
 #define MSF_ERR_UNSUPPORTED (-2) /* grid size without a compiled transform length */
 #define MSF_ERR_DEVICE (-3)      /* HIP runtime error */
@@ -207,6 +208,41 @@ int msf_exit_waves(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* opt
  * final-position diffraction images (adfstem.cpp:2140-2230, :3780-3797). */
 int msf_exit_waves_shifted(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
                            const int32_t* shifts, uint32_t n, float* psi);
+
+/* ------------------------------------------------------------ serving */
+/* One self-contained job: specimen atoms, optics, raster and detector/FEM
+ * settings (inputs are copied by msf_submit), and the caller's output
+ * arrays, which must stay valid until msf_wait returns for the job. */
+typedef struct msf_job {
+    msf_grid grid;
+    const double* q_xyz;               /* 3 * n_atoms positions [A] */
+    const uint32_t* Z;
+    uint32_t n_atoms;
+    double min_slice;
+    msf_optics optics;
+    const double* x_p;
+    uint32_t n_xp;
+    const double* y_p;
+    uint32_t n_yp;
+    msf_detector detector;
+    msf_fem fem;
+    double* adf_image;                 /* n_xp*n_yp doubles when detector.enabled */
+    msf_fem_outputs fem_out;           /* optional FEM outputs */
+} msf_job;
+
+/* No reference counterpart: the reference runs one job per process.  A
+ * context owns one worker thread (started by the first submission) that
+ * runs queued jobs in submission order on the context's stream, each as
+ * msf_specimen_build + msf_run_stem_fem + msf_specimen_free.  Direct calls
+ * on the same context wait for the job in progress.  msf_destroy finishes
+ * the queue before it returns. */
+int msf_submit(msf_ctx* ctx, const msf_job* job, uint64_t* id);
+/* MSF_OK or the job's error code once finished, MSF_PENDING while queued or
+ * running, MSF_ERR_INVALID for an unknown id */
+int msf_poll(msf_ctx* ctx, uint64_t id);
+/* blocks until the job has finished; returns its status (the error text is
+ * then available from msf_last_error) and forgets the job */
+int msf_wait(msf_ctx* ctx, uint64_t id);
 
 #ifdef __cplusplus
 }

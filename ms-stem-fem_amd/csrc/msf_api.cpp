@@ -13,10 +13,15 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -94,7 +99,37 @@ struct RunPlan {
     }
 };
 
+// A job of the serving queue: inputs copied at submission, outputs written
+// to the caller's arrays.
+struct ServeJob {
+    uint64_t id = 0;
+    msf_grid grid{};
+    std::vector<double> q;
+    std::vector<uint32_t> Z;
+    double min_slice = 0;
+    msf_optics optics{};
+    std::vector<double> xp, yp;
+    msf_detector det{};
+    msf_fem fem{};
+    double* adf = nullptr;
+    msf_fem_outputs fo{};
+};
+
+struct ServeState {
+    std::thread worker;
+    std::mutex m;
+    std::condition_variable work_cv, done_cv;
+    std::deque<ServeJob> queue;
+    std::set<uint64_t> pending;                              // queued or running
+    std::map<uint64_t, std::pair<int, std::string>> done;    // status, message
+    uint64_t next_id = 1;
+    bool stop = false;
+};
+
 struct msf_ctx {
+    std::recursive_mutex mu;    // serialises device work of the context
+    std::mutex serve_mu;        // guards the creation of the serving worker
+    std::unique_ptr<ServeState> serve;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -130,6 +165,9 @@ int fail(msf_ctx* c, int code, const std::string& m)
     if (c) c->err = m;
     return code;
 }
+
+// device work of a context is serialised (direct calls and the serving worker)
+#define MSF_LOCK(c) std::lock_guard<std::recursive_mutex> msf_lock_((c)->mu)
 
 #define HIPCHK(ctx, expr)                                                                     \
     do {                                                                                      \
@@ -693,19 +731,33 @@ int msf_create(msf_ctx** out, int device)
 int msf_destroy(msf_ctx* c)
 {
     if (!c) return MSF_ERR_INVALID;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    // an instantiated graph refers to the stream it was launched on: destroy
-    // it (and every buffer its kernels use) before the stream
-    c->plan.drop_graph();
-    // specimens may outlive their context: detach them so that a later
-    // msf_specimen_free only releases their own device memory
-    for (msf_specimen* sp : c->specimens) sp->ctx = nullptr;
-    c->specimens.clear();
-    c->tw32.clear();
-    c->tw64.clear();
-    c->twnat.clear();
-    (void)hipStreamDestroy(c->stream);
+    std::unique_lock<std::mutex> serve_lk(c->serve_mu);
+    if (c->serve) { // finish the queued jobs, then stop the worker
+        {
+            std::lock_guard<std::mutex> lk(c->serve->m);
+            c->serve->stop = true;
+        }
+        c->serve->work_cv.notify_all();
+        if (c->serve->worker.joinable()) c->serve->worker.join();
+        c->serve.reset();
+    }
+    serve_lk.unlock();
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        // an instantiated graph refers to the stream it was launched on:
+        // destroy it (and every buffer its kernels use) before the stream
+        c->plan.drop_graph();
+        // specimens may outlive their context: detach them so that a later
+        // msf_specimen_free only releases their own device memory
+        for (msf_specimen* sp : c->specimens) sp->ctx = nullptr;
+        c->specimens.clear();
+        c->tw32.clear();
+        c->tw64.clear();
+        c->twnat.clear();
+        (void)hipStreamDestroy(c->stream);
+    }
     delete c;
     return MSF_OK;
 }
@@ -715,6 +767,7 @@ const char* msf_last_error(const msf_ctx* c) { return c ? c->err.c_str() : "no c
 int msf_set_config(msf_ctx* c, const msf_config* cfg)
 {
     if (!c || !cfg) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     if (cfg->transmit_mode != MSF_TRANSMIT_REFERENCE && cfg->transmit_mode != MSF_TRANSMIT_EXACT)
         return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
     if (cfg->col_tile != 0 && cfg->col_tile != 8 && cfg->col_tile != 16)
@@ -726,6 +779,7 @@ int msf_set_config(msf_ctx* c, const msf_config* cfg)
 int msf_get_stats(const msf_ctx* c, msf_stats* out)
 {
     if (!c || !out) return MSF_ERR_INVALID;
+    MSF_LOCK(const_cast<msf_ctx*>(c));
     *out = c->stats;
     return MSF_OK;
 }
@@ -760,6 +814,7 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
                        uint32_t n, double min_slice, msf_specimen** out)
 {
     if (!c || !out) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     *out = nullptr;
     if (int r = check_grid(c, grid)) return r;
     if (n > 0 && (!q || !Z)) return fail(c, MSF_ERR_INVALID, "atom arrays are NULL");
@@ -917,6 +972,7 @@ int msf_specimen_free(msf_specimen* sp)
 {
     if (!sp) return MSF_ERR_INVALID;
     if (sp->ctx) {
+        MSF_LOCK(sp->ctx);
         (void)hipSetDevice(sp->ctx->device);
         (void)hipStreamSynchronize(sp->ctx->stream);
         if (sp->ctx->plan.spec_id == sp->id) {
@@ -946,6 +1002,7 @@ int msf_specimen_transmission(const msf_specimen* sp, uint32_t slice, double* t)
 {
     if (!sp || !t || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
     if (!sp->ctx) return MSF_ERR_INVALID; // its context was destroyed
+    MSF_LOCK(sp->ctx);
     const size_t M = (size_t)sp->dom.nx * sp->dom.ny;
     (void)hipSetDevice(sp->ctx->device);
     if (hipMemcpy(t, sp->t64.as<double2>() + slice * M, M * sizeof(double2), hipMemcpyDeviceToHost) !=
@@ -957,6 +1014,7 @@ int msf_specimen_transmission(const msf_specimen* sp, uint32_t slice, double* t)
 int msf_probe_realspace(msf_ctx* c, const msf_grid* grid, const msf_optics* o, double* probe)
 {
     if (!c || !probe) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     if (int r = check_grid(c, grid)) return r;
     if (int r = check_optics(c, o)) return r;
     HIPCHK(c, hipSetDevice(c->device));
@@ -993,6 +1051,7 @@ int msf_run_stem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const 
                  uint32_t n_yp, double* adf_image, double* fem_v)
 {
     if (!c) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     HIPCHK(c, hipSetDevice(c->device));
     if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
     if (fem && fem->enabled && !fem_v) return fail(c, MSF_ERR_INVALID, "fem_v is NULL");
@@ -1013,6 +1072,7 @@ int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                      const msf_fem_outputs* fo)
 {
     if (!c) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     HIPCHK(c, hipSetDevice(c->device));
     if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
     if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
@@ -1052,6 +1112,7 @@ int msf_exit_waves_shifted(msf_ctx* c, const msf_specimen* sp, const msf_optics*
                            const int32_t* shifts, uint32_t n, float* psi)
 {
     if (!c || !psi || !shifts) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     HIPCHK(c, hipSetDevice(c->device));
     if (n == 0) return fail(c, MSF_ERR_INVALID, "empty shift list");
     std::vector<int2> sh(n);
@@ -1065,6 +1126,7 @@ int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, cons
                    uint32_t n_xp, const double* y_p, uint32_t n_yp, float* psi)
 {
     if (!c || !psi) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
     HIPCHK(c, hipSetDevice(c->device));
     if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
     if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
@@ -1074,6 +1136,112 @@ int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, cons
     RunOut out;
     out.exit = psi;
     return run_internal(c, sp, o, nullptr, nullptr, shifts, out);
+}
+
+// ---------------------------------------------------------------- serving
+namespace {
+
+void serve_loop(msf_ctx* c)
+{
+    ServeState& st = *c->serve;
+    for (;;) {
+        ServeJob job;
+        {
+            std::unique_lock<std::mutex> lk(st.m);
+            st.work_cv.wait(lk, [&] { return st.stop || !st.queue.empty(); });
+            if (st.queue.empty()) return; // stop requested and nothing left
+            job = std::move(st.queue.front());
+            st.queue.pop_front();
+        }
+        int rc;
+        std::string msg;
+        {
+            MSF_LOCK(c);
+            msf_specimen* sp = nullptr;
+            rc = msf_specimen_build(c, &job.grid, job.q.data(), job.Z.data(),
+                                    (uint32_t)job.Z.size(), job.min_slice, &sp);
+            if (rc == MSF_OK)
+                rc = msf_run_stem_fem(c, sp, &job.optics, &job.det, &job.fem, job.xp.data(),
+                                      (uint32_t)job.xp.size(), job.yp.data(),
+                                      (uint32_t)job.yp.size(), job.adf, &job.fo);
+            if (rc != MSF_OK) msg = c->err;
+            if (sp) msf_specimen_free(sp);
+        }
+        {
+            std::lock_guard<std::mutex> lk(st.m);
+            st.pending.erase(job.id);
+            st.done[job.id] = {rc, msg};
+        }
+        st.done_cv.notify_all();
+    }
+}
+
+} // namespace
+
+int msf_submit(msf_ctx* c, const msf_job* j, uint64_t* id)
+{
+    if (!c || !j || !id) return MSF_ERR_INVALID;
+    if (!j->x_p || !j->y_p || j->n_xp == 0 || j->n_yp == 0 || (j->n_atoms && (!j->q_xyz || !j->Z)))
+        return MSF_ERR_INVALID;
+    if (j->detector.enabled && !j->adf_image) return MSF_ERR_INVALID;
+    ServeJob job;
+    job.grid = j->grid;
+    job.q.assign(j->q_xyz, j->q_xyz + 3 * (size_t)j->n_atoms);
+    job.Z.assign(j->Z, j->Z + j->n_atoms);
+    job.min_slice = j->min_slice;
+    job.optics = j->optics;
+    job.xp.assign(j->x_p, j->x_p + j->n_xp);
+    job.yp.assign(j->y_p, j->y_p + j->n_yp);
+    job.det = j->detector;
+    job.fem = j->fem;
+    job.adf = j->adf_image;
+    job.fo = j->fem_out;
+    {
+        std::lock_guard<std::mutex> lk(c->serve_mu);
+        if (!c->serve) {
+            c->serve.reset(new ServeState);
+            c->serve->worker = std::thread(serve_loop, c);
+        }
+    }
+    ServeState& st = *c->serve;
+    {
+        std::lock_guard<std::mutex> lk(st.m);
+        job.id = st.next_id++;
+        *id = job.id;
+        st.pending.insert(job.id);
+        st.queue.push_back(std::move(job));
+    }
+    st.work_cv.notify_one();
+    return MSF_OK;
+}
+
+int msf_poll(msf_ctx* c, uint64_t id)
+{
+    if (!c || !c->serve) return MSF_ERR_INVALID;
+    ServeState& st = *c->serve;
+    std::lock_guard<std::mutex> lk(st.m);
+    auto it = st.done.find(id);
+    if (it != st.done.end()) return it->second.first;
+    return st.pending.count(id) ? MSF_PENDING : MSF_ERR_INVALID;
+}
+
+int msf_wait(msf_ctx* c, uint64_t id)
+{
+    if (!c || !c->serve) return MSF_ERR_INVALID;
+    ServeState& st = *c->serve;
+    std::pair<int, std::string> r;
+    {
+        std::unique_lock<std::mutex> lk(st.m);
+        if (!st.done.count(id) && !st.pending.count(id)) return MSF_ERR_INVALID;
+        st.done_cv.wait(lk, [&] { return st.done.count(id) != 0; });
+        r = st.done[id];
+        st.done.erase(id);
+    }
+    if (r.first != MSF_OK) {
+        MSF_LOCK(c);
+        c->err = "job " + std::to_string(id) + ": " + r.second;
+    }
+    return r.first;
 }
 
 } // extern "C"

@@ -19,6 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmsf.so")
 
 TRANSMIT_REFERENCE, TRANSMIT_EXACT = 0, 1
+PENDING = 1
 ADF_REFERENCE, ADF_PHYSICAL = 0, 1
 
 _ERRORS = {-1: "invalid argument", -2: "unsupported size", -3: "device error",
@@ -66,6 +67,21 @@ class FemOutputs(C.Structure):
     _fields_ = [(m, C.c_void_p) for m in FEM_MODES]
 
 
+class Job(C.Structure):
+    _fields_ = [("grid", Grid), ("q_xyz", C.c_void_p), ("Z", C.c_void_p), ("n_atoms", C.c_uint32),
+                ("min_slice", C.c_double), ("optics", Optics), ("x_p", C.c_void_p),
+                ("n_xp", C.c_uint32), ("y_p", C.c_void_p), ("n_yp", C.c_uint32),
+                ("detector", Detector), ("fem", Fem), ("adf_image", C.c_void_p),
+                ("fem_out", FemOutputs)]
+
+
+class Ticket:
+    """A submitted job: its id and the output arrays the worker fills."""
+
+    def __init__(self, job_id, img, outs, keep):
+        self.id, self.img, self.outs, self._keep = job_id, img, outs, keep
+
+
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
                 ("profile_kernels", C.c_int32), ("col_tile", C.c_int32),
@@ -95,7 +111,8 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_scherzer_alphamax", "msf_specimen_build", "msf_specimen_free",
            "msf_specimen_slices", "msf_specimen_slice_info", "msf_specimen_transmission",
            "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves",
-           "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem"]
+           "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem",
+           "msf_submit", "msf_poll", "msf_wait"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -142,6 +159,9 @@ def lib():
                                        C.POINTER(FemOutputs)]
         L.msf_exit_waves_shifted.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(C.c_int32),
                                              C.c_uint32, C.POINTER(C.c_float)]
+        L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
+        L.msf_poll.argtypes = [vp, C.c_uint64]
+        L.msf_wait.argtypes = [vp, C.c_uint64]
         L.msf_scherzer_cs3.argtypes = [C.c_double, C.c_double]
         L.msf_scherzer_cs3.restype = C.c_double
         # host-side readers and writers (include/msf_io.h)
@@ -349,6 +369,40 @@ class Context:
         if img is not None:
             img = img.reshape(x_p.size, y_p.size)
         return img, {m: v[:nb] for m, v in outs.items()}
+
+    def submit(self, grid, q, Z, min_slice, optics, x_p, y_p, detector=None, fem=None,
+               modes=("v_omega",)):
+        """msf_submit: queue a whole job (specimen build + run) on the
+        context's worker thread; returns a Ticket for poll() / wait()."""
+        q = np.ascontiguousarray(q, np.float64).reshape(-1)
+        Z = np.ascontiguousarray(Z, np.uint32)
+        x_p = np.ascontiguousarray(x_p, np.float64)
+        y_p = np.ascontiguousarray(y_p, np.float64)
+        det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
+        fm = fem if fem is not None else Fem(0, 0.0)
+        img = np.zeros(x_p.size * y_p.size) if det.enabled else None
+        outs, nb = {}, 0
+        if fm.enabled:
+            nb = fem_bins(grid, fm.dr).size
+            outs = {m: np.zeros(max(nb, 1)) for m in modes}
+        fo = FemOutputs(*[outs[m].ctypes.data if m in outs else None for m in FEM_MODES])
+        job = Job(grid, q.ctypes.data, Z.ctypes.data, Z.size, min_slice, optics, x_p.ctypes.data,
+                  x_p.size, y_p.ctypes.data, y_p.size, det, fm,
+                  img.ctypes.data if img is not None else None, fo)
+        jid = C.c_uint64()
+        self._check(lib().msf_submit(self._h, C.byref(job), C.byref(jid)), "msf_submit")
+        return Ticket(jid.value, img, outs, (q, Z, x_p, y_p, nb))
+
+    def poll(self, ticket):
+        return lib().msf_poll(self._h, ticket.id)
+
+    def wait(self, ticket):
+        """Returns (adf image [n_xp, n_yp] or None, {mode: values})."""
+        self._check(lib().msf_wait(self._h, ticket.id), "msf_wait")
+        img = ticket.img
+        nx, ny, nb = ticket._keep[2].size, ticket._keep[3].size, ticket._keep[4]
+        return ((img.reshape(nx, ny) if img is not None else None),
+                {m: v[:nb] for m, v in ticket.outs.items()})
 
     def exit_waves_shifted(self, spec, optics, shifts):
         sh = np.ascontiguousarray(shifts, np.int32).reshape(-1, 2)

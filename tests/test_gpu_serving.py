@@ -1,0 +1,84 @@
+"""The serving queue (msf_submit / msf_poll / msf_wait): jobs of mixed sizes
+submitted together, each equal bit for bit to the same job run directly and
+within 1e-4 of the FP64 oracle; a failing job does not stop the others."""
+import numpy as np
+import pytest
+
+from helpers import Case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def msf():
+    import msf as m
+    m.lib()
+    return m
+
+
+def _jobs(n=6, seed=3000):
+    """A small mixed workload in the manner of a batched submission: N in
+    {64, 96, 128}, cell side L with N / L >= 8 (so a 0.05-0.18 rad ADF ring
+    fits inside the band limit at 300 kV), Si at 0.05 A^-3, minslice 2.0,
+    explicit rasters of side 3..5 at spacing L / side."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n):
+        N = int(rng.choice([64, 96, 128]))
+        L = float(rng.uniform(N / 12.0, N / 8.0))
+        n_at = max(1, int(round(0.05 * L * L * 4.0)))
+        q = rng.random((n_at, 3)) * [L, L, 4.0]
+        side = int(rng.integers(3, 6))
+        xp = np.arange(side) * (L / side)
+        c = Case(nx=N, ny=N, cell=(L, L, 4.0), voltage=300e3, q=q,
+                 Z=np.full(n_at, 14, np.uint32), min_slice=2.0, alpha_max=0.015,
+                 inner=0.05, outer=0.18, fem=True, dr=0.7)
+        jobs.append((c, xp))
+    return jobs
+
+
+def test_submitted_jobs_match_direct_runs_and_oracle(msf, ctx, O):
+    jobs = _jobs()
+    tickets = []
+    for c, xp in jobs:
+        tickets.append(ctx.submit(c.grid(msf), c.q, c.Z, c.min_slice, c.optics(msf), xp, xp,
+                                  detector=c.detector(msf), fem=c.femcfg(msf),
+                                  modes=("v_omega", "v_re")))
+    results = [None] * len(jobs)
+    for k in reversed(range(len(jobs))):          # waiting order does not matter
+        results[k] = ctx.wait(tickets[k])
+        assert ctx.poll(tickets[k]) == -1          # forgotten after wait
+    for (c, xp), (img, fem) in zip(jobs, results):
+        spec = c.gpu_specimen(msf, ctx)
+        img_d, fem_d = ctx.adfstem_fem(spec, c.optics(msf), xp, xp, c.femcfg(msf),
+                                       modes=("v_omega", "v_re"), detector=c.detector(msf))
+        np.testing.assert_array_equal(img, img_d)
+        for m in fem:
+            np.testing.assert_array_equal(fem[m], fem_d[m])
+        _, (img_o, v_o, _) = c.run_oracle(O, xp, xp)
+        assert rel_l2(img, img_o) < 1e-4
+        assert rel_l2(fem["v_omega"], v_o) < 1e-4
+        spec.free()
+
+
+def test_failing_job_does_not_stop_the_queue(msf, ctx):
+    (c, xp), (c2, xp2) = _jobs(2, seed=7)
+    bad = ctx.submit(c.grid(msf), c.q, np.full(c.Z.size, 200, np.uint32), c.min_slice,
+                     c.optics(msf), xp, xp, detector=c.detector(msf))
+    good = ctx.submit(c2.grid(msf), c2.q, c2.Z, c2.min_slice, c2.optics(msf), xp2, xp2,
+                      detector=c2.detector(msf))
+    assert ctx.poll(good) in (0, msf.PENDING)
+    with pytest.raises(msf.MsfError) as e:
+        ctx.wait(bad)
+    assert e.value.code == -1 and "atomic number" in str(e.value)
+    img, _ = ctx.wait(good)
+    assert np.isfinite(img).all() and img.max() > 0
+
+
+def test_destroy_finishes_the_queue(msf):
+    (c, xp), = _jobs(1, seed=11)
+    own = msf.Context(0)
+    t = own.submit(c.grid(msf), c.q, c.Z, c.min_slice, c.optics(msf), xp, xp,
+                   detector=c.detector(msf))
+    own.close()                                   # joins the worker after the job ran
+    assert np.isfinite(t.img).all() and t.img.max() > 0
