@@ -142,11 +142,28 @@ int msf_specimen_slice_info(const msf_specimen* spec, double* lower_bound, doubl
 /* transmission function of one slice, FP64 as built (2*nx*ny doubles) */
 int msf_specimen_transmission(const msf_specimen* spec, uint32_t slice, double* t);
 
+/* propagator of one slice as built (slice.cpp:1293-1363): nx and ny complex
+ * doubles (either may be NULL) */
+int msf_specimen_propagator(const msf_specimen* spec, uint32_t slice, double* px, double* py);
+
 /* ------------------------------------------------------------ probe */
 /* Normalised real-space probe of adfstem.cpp:586-817 at (xx[0], yy[0]):
  * 2*nx*ny doubles. */
 int msf_probe_realspace(msf_ctx* ctx, const msf_grid* grid, const msf_optics* optics,
                         double* probe);
+
+/* Initial k-space wave of one position (adfstem.cpp:1590-1671,
+ * :1831-1837): the normalised probe circularly shifted by (sx, sy) pixels,
+ * FFT / sqrt(NxNy); 2*nx*ny doubles. */
+int msf_probe_k(msf_ctx* ctx, const msf_grid* grid, const msf_optics* optics, int32_t sx,
+                int32_t sy, double* psi_k);
+
+/* One slice step in FP64 on `count` k-space waves in place
+ * (adfstem.cpp:1866-2132 for slice `slice`: IFFT, transmission update in
+ * the configured transmit_mode, FFT, band limit, propagator): a parity
+ * entry point for single steps; the production loop is msf_run_stem. */
+int msf_slice_step_batch(msf_ctx* ctx, const msf_specimen* spec, uint32_t slice, double* psi_k,
+                         uint32_t count);
 
 /* ------------------------------------------------------------ run */
 /* Number of V_omega bins (adfstem.cpp:947-972) and their lower edges

@@ -130,6 +130,9 @@ hipError_t launch_finish_transmit(double2* a, int64_t n, double nxny, double s, 
 // one vacuum slice step in k-space: a *= keep ? P_x[i] P_y[j] * scale : 0
 hipError_t launch_vacuum_step(double2* a, int nx, int ny, const double2* px, const double2* py,
                               const int2* keep, double scale, hipStream_t s);
+// FP64 transmission update of `count` real-space waves by one slice's t
+hipError_t launch_transmit_f64(double2* a, const double2* t, int64_t m, int count, int mode,
+                               double s, hipStream_t st);
 // out = (float2) a
 hipError_t launch_to_f32(const double2* a, int64_t n, float2* out, hipStream_t s);
 // out[i][j'] = (float2) a[i][j' mod ny], j' < 2*ny

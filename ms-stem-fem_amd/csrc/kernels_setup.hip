@@ -409,4 +409,36 @@ hipError_t launch_scale_f64(double2* a, int64_t n, const double* norm2, double f
     return hipGetLastError();
 }
 
+// FP64 transmission update of one slice on `count` real-space waves
+// (adfstem.cpp:1922-1952): mode 0 forms the imaginary part from the already
+// updated real part, as the reference's two statements do; mode 1 is the
+// complex product.  Both divide by s = sqrt(NxNy).
+__global__ void transmit_f64_kernel(double2* a, const double2* t, int64_t m, int count, int mode,
+                                    double s)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * count) return;
+    const double2 tt = t[idx % m];
+    const double2 u = a[idx];
+    double2 v;
+    if (mode == 0) {
+        v.x = (tt.x * u.x - tt.y * u.y) / s;
+        v.y = (tt.x * u.y + tt.y * v.x) / s;
+    } else {
+        v.x = (tt.x * u.x - tt.y * u.y) / s;
+        v.y = (tt.x * u.y + tt.y * u.x) / s;
+    }
+    a[idx] = v;
+}
+
+hipError_t launch_transmit_f64(double2* a, const double2* t, int64_t m, int count, int mode,
+                               double s, hipStream_t st)
+{
+    const int64_t n = m * count;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transmit_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a,
+                       t, m, count, mode, s);
+    return hipGetLastError();
+}
+
 } // namespace msf

@@ -1011,6 +1011,92 @@ int msf_specimen_transmission(const msf_specimen* sp, uint32_t slice, double* t)
     return MSF_OK;
 }
 
+int msf_specimen_propagator(const msf_specimen* sp, uint32_t slice, double* px, double* py)
+{
+    if (!sp || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
+    if (!sp->ctx) return MSF_ERR_INVALID;
+    MSF_LOCK(sp->ctx);
+    (void)hipSetDevice(sp->ctx->device);
+    const int nx = sp->dom.nx, ny = sp->dom.ny;
+    if (px && hipMemcpy(px, sp->px64.as<double2>() + (size_t)slice * nx, nx * sizeof(double2),
+                        hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the propagator failed");
+    if (py && hipMemcpy(py, sp->py64.as<double2>() + (size_t)slice * ny, ny * sizeof(double2),
+                        hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the propagator failed");
+    return MSF_OK;
+}
+
+int msf_probe_k(msf_ctx* c, const msf_grid* grid, const msf_optics* o, int32_t sx, int32_t sy,
+                double* psi_k)
+{
+    if (!c || !psi_k) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
+    if (int r = check_grid(c, grid)) return r;
+    if (sx < 0 || sy < 0 || sx >= (int32_t)grid->nx || sy >= (int32_t)grid->ny)
+        return fail(c, MSF_ERR_DOMAIN, "shift outside the grid");
+    const int nx = (int)grid->nx, ny = (int)grid->ny;
+    const size_t M = (size_t)nx * ny;
+    std::vector<double> probe(2 * M);
+    if (int r = msf_probe_realspace(c, grid, o, probe.data())) return r;
+    // adfstem.cpp:1590-1671: circular shift by (sx, sy) pixels
+    for (int i = 0; i < nx; ++i) {
+        const int ix = (i < sx) ? i + nx - sx : i - sx;
+        for (int j = 0; j < ny; ++j) {
+            const int iy = (j < sy) ? j + ny - sy : j - sy;
+            psi_k[2 * ((size_t)i * ny + j)] = probe[2 * ((size_t)ix * ny + iy)];
+            psi_k[2 * ((size_t)i * ny + j) + 1] = probe[2 * ((size_t)ix * ny + iy) + 1];
+        }
+    }
+    // :1831-1837: FFT on the device, / sqrt(NxNy)
+    const double2 *twx, *twy;
+    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+    DevBuf d;
+    HIPCHK(c, d.alloc(M * sizeof(double2)));
+    HIPCHK(c, hipMemcpyAsync(d.p, psi_k, M * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, msf::fft2d_f64(d.as<double2>(), nx, ny, 1, false, twx, twy, c->stream));
+    HIPCHK(c, hipMemcpyAsync(psi_k, d.p, M * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double s = (nx == ny) ? (double)nx : std::sqrt((double)M);
+    for (size_t k = 0; k < 2 * M; ++k) psi_k[k] = psi_k[k] / s;
+    return MSF_OK;
+}
+
+int msf_slice_step_batch(msf_ctx* c, const msf_specimen* sp, uint32_t slice, double* psi_k,
+                         uint32_t count)
+{
+    if (!c || !sp || !psi_k || count == 0) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
+    if (sp->ctx != c) return fail(c, MSF_ERR_INVALID, "specimen was built on another context");
+    if ((int)slice >= sp->n_slices) return fail(c, MSF_ERR_INVALID, "slice index out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    const msf::Domain& d = sp->dom;
+    const int nx = d.nx, ny = d.ny;
+    const size_t M = (size_t)nx * ny;
+    const double2 *twx, *twy;
+    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+    DevBuf w;
+    HIPCHK(c, w.alloc(M * count * sizeof(double2)));
+    double2* v = w.as<double2>();
+    HIPCHK(c, hipMemcpyAsync(v, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    // adfstem.cpp:1891 IFFT; :1922-1952 transmission / s; :1986-1992 FFT / s;
+    // bw_limit.cpp:30-47 and slice.cpp:1366-1421 mask and propagator
+    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, true, twx, twy, c->stream));
+    HIPCHK(c, msf::launch_transmit_f64(v, sp->t64.as<double2>() + (size_t)slice * M, (int64_t)M,
+                                       (int)count, c->cfg.transmit_mode, d.sqrt_nxny, c->stream));
+    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, false, twx, twy, c->stream));
+    for (uint32_t k = 0; k < count; ++k)
+        HIPCHK(c, msf::launch_vacuum_step(v + (size_t)k * M, nx, ny,
+                                          sp->px64.as<double2>() + (size_t)slice * nx,
+                                          sp->py64.as<double2>() + (size_t)slice * ny,
+                                          sp->dkeep.as<int2>(), 1.0 / d.sqrt_nxny, c->stream));
+    HIPCHK(c, hipMemcpyAsync(psi_k, v, M * count * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MSF_OK;
+}
+
 int msf_probe_realspace(msf_ctx* c, const msf_grid* grid, const msf_optics* o, double* probe)
 {
     if (!c || !probe) return MSF_ERR_INVALID;

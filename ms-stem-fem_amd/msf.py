@@ -112,7 +112,8 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_specimen_slices", "msf_specimen_slice_info", "msf_specimen_transmission",
            "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves",
            "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem",
-           "msf_submit", "msf_poll", "msf_wait"]
+           "msf_submit", "msf_poll", "msf_wait", "msf_specimen_propagator", "msf_probe_k",
+           "msf_slice_step_batch"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -159,6 +160,10 @@ def lib():
                                        C.POINTER(FemOutputs)]
         L.msf_exit_waves_shifted.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(C.c_int32),
                                              C.c_uint32, C.POINTER(C.c_float)]
+        L.msf_specimen_propagator.argtypes = [vp, C.c_uint32, _dp, _dp]
+        L.msf_probe_k.argtypes = [vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int32, C.c_int32,
+                                  _dp]
+        L.msf_slice_step_batch.argtypes = [vp, vp, C.c_uint32, _dp, C.c_uint32]
         L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
         L.msf_poll.argtypes = [vp, C.c_uint64]
         L.msf_wait.argtypes = [vp, C.c_uint64]
@@ -404,6 +409,21 @@ class Context:
         return ((img.reshape(nx, ny) if img is not None else None),
                 {m: v[:nb] for m, v in ticket.outs.items()})
 
+    def probe_k(self, grid, optics, sx, sy):
+        out = np.zeros((grid.nx, grid.ny), np.complex128)
+        self._check(lib().msf_probe_k(self._h, C.byref(grid), C.byref(optics), sx, sy,
+                                      _p(out.view(np.float64))), "msf_probe_k")
+        return out
+
+    def slice_step(self, spec, slice_index, psi_k):
+        """msf_slice_step_batch on a copy of psi_k [count, nx, ny] (complex)."""
+        w = np.ascontiguousarray(psi_k, np.complex128).copy()
+        count = w.shape[0] if w.ndim == 3 else 1
+        self._check(lib().msf_slice_step_batch(self._h, spec._h, slice_index,
+                                               _p(w.view(np.float64)), count),
+                    "msf_slice_step_batch")
+        return w
+
     def exit_waves_shifted(self, spec, optics, shifts):
         sh = np.ascontiguousarray(shifts, np.int32).reshape(-1, 2)
         out = np.zeros((sh.shape[0], spec.grid.nx, spec.grid.ny), np.complex64)
@@ -455,6 +475,15 @@ class Specimen:
         if rc:
             raise MsfError(rc, "msf_specimen_slice_info")
         return lb, th, na
+
+    def propagator(self, s):
+        px = np.zeros(self.grid.nx, np.complex128)
+        py = np.zeros(self.grid.ny, np.complex128)
+        rc = lib().msf_specimen_propagator(self._h, s, _p(px.view(np.float64)),
+                                           _p(py.view(np.float64)))
+        if rc:
+            raise MsfError(rc, "msf_specimen_propagator")
+        return px, py
 
     def transmission(self, s):
         out = np.zeros((self.grid.nx, self.grid.ny), np.complex128)

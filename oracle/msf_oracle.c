@@ -475,11 +475,9 @@ void orc_probe_shifts(const double* xx, int nx, const double* yy, int ny,
  * imaginary part reads the already-updated real part); FORWARD FFT;
  * / sqrtNxNy; bw_limit (bw_limit.cpp:30-47); propagate (slice.cpp:1366-1421).
  * transmit_mode 1 replaces the in-place update by the exact complex product. */
-int orc_propagate_position(const double* probe, int64_t sx, int64_t sy,
-                           int n_slices, const double* t_stack,
-                           const double* px, const double* py,
-                           const double* kx, const double* ky, int Nx, int Ny,
-                           double bwcutoff_t, int transmit_mode, double* psi)
+/* adfstem.cpp:1590-1671, :1831-1837: the probe circularly shifted by
+ * (sx, sy) pixels, then psi_k = FFT(psi_x) / sqrt(NxNy). */
+int orc_initial_wave(const double* probe, int64_t sx, int64_t sy, int Nx, int Ny, double* psi)
 {
     const size_t M = (size_t)Nx * Ny;
     const double NxNy = (double)Nx * Ny;
@@ -495,41 +493,67 @@ int orc_propagate_position(const double* probe, int64_t sx, int64_t sy,
     }
     if (orc_fft2d(psi, Nx, Ny, -1)) return -1;
     for (size_t k = 0; k < 2 * M; k++) psi[k] = psi[k] / s;
+    return 0;
+}
+
+/* One slice of adfstem.cpp:1866-2132: psi_x = IFFT(psi_k); the transmission
+ * update of :1922-1952 (mode 0 overwrites the real part before the
+ * imaginary part is formed, mode 1 is the complex product), / sqrt(NxNy);
+ * psi_k = FFT(psi_x) / sqrt(NxNy); bw_limit; slice::propagate. */
+int orc_slice_step(double* psi, const double* t, const double* pxn, const double* pyn,
+                   const double* kx, const double* ky, int Nx, int Ny, double bwcutoff_t,
+                   int transmit_mode)
+{
+    const size_t M = (size_t)Nx * Ny;
+    const double NxNy = (double)Nx * Ny;
+    const double s = (Nx == Ny) ? (double)Nx : sqrt(NxNy);
     const double bw_sqr = bwcutoff_t * bwcutoff_t;
-    for (int n = 0; n < n_slices; n++) {
-        const double* t = t_stack + 2 * M * n;
-        const double* pxn = px + 2 * (size_t)Nx * n;
-        const double* pyn = py + 2 * (size_t)Ny * n;
-        if (orc_fft2d(psi, Nx, Ny, +1)) return -1;
-        for (size_t k = 0; k < M; k++) {
-            const double tr = t[2 * k], ti = t[2 * k + 1];
-            if (transmit_mode == 0) {
-                psi[2 * k] = (tr * psi[2 * k] - ti * psi[2 * k + 1]) / s;
-                psi[2 * k + 1] = (tr * psi[2 * k + 1] + ti * psi[2 * k]) / s;
+    if (orc_fft2d(psi, Nx, Ny, +1)) return -1;
+    for (size_t k = 0; k < M; k++) {
+        const double tr = t[2 * k], ti = t[2 * k + 1];
+        if (transmit_mode == 0) {
+            psi[2 * k] = (tr * psi[2 * k] - ti * psi[2 * k + 1]) / s;
+            psi[2 * k + 1] = (tr * psi[2 * k + 1] + ti * psi[2 * k]) / s;
+        } else {
+            const double ur = psi[2 * k], ui = psi[2 * k + 1];
+            psi[2 * k] = (tr * ur - ti * ui) / s;
+            psi[2 * k + 1] = (tr * ui + ti * ur) / s;
+        }
+    }
+    if (orc_fft2d(psi, Nx, Ny, -1)) return -1;
+    for (size_t k = 0; k < 2 * M; k++) psi[k] = psi[k] / s;
+    bw_limit(psi, kx, Nx, ky, Ny, bwcutoff_t);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const size_t k = (size_t)j + (size_t)i * Ny;
+            const double k_sqr = ref_pow2(kx[i]) + ref_pow2(ky[j]);
+            if (k_sqr < bw_sqr) {
+                const double pr = pxn[2 * i] * pyn[2 * j] - pxn[2 * i + 1] * pyn[2 * j + 1];
+                const double pi = pxn[2 * i] * pyn[2 * j + 1] + pxn[2 * i + 1] * pyn[2 * j];
+                const double fr = psi[2 * k], fi = psi[2 * k + 1];
+                psi[2 * k] = (pr * fr - pi * fi);
+                psi[2 * k + 1] = (pr * fi + pi * fr);
             } else {
-                const double ur = psi[2 * k], ui = psi[2 * k + 1];
-                psi[2 * k] = (tr * ur - ti * ui) / s;
-                psi[2 * k + 1] = (tr * ui + ti * ur) / s;
+                psi[2 * k] = 0.0;
+                psi[2 * k + 1] = 0.0;
             }
         }
-        if (orc_fft2d(psi, Nx, Ny, -1)) return -1;
-        for (size_t k = 0; k < 2 * M; k++) psi[k] = psi[k] / s;
-        bw_limit(psi, kx, Nx, ky, Ny, bwcutoff_t);
-        for (int i = 0; i < Nx; i++)
-            for (int j = 0; j < Ny; j++) {
-                const size_t k = (size_t)j + (size_t)i * Ny;
-                const double k_sqr = ref_pow2(kx[i]) + ref_pow2(ky[j]);
-                if (k_sqr < bw_sqr) {
-                    const double pr = pxn[2 * i] * pyn[2 * j] - pxn[2 * i + 1] * pyn[2 * j + 1];
-                    const double pi = pxn[2 * i] * pyn[2 * j + 1] + pxn[2 * i + 1] * pyn[2 * j];
-                    const double fr = psi[2 * k], fi = psi[2 * k + 1];
-                    psi[2 * k] = (pr * fr - pi * fi);
-                    psi[2 * k + 1] = (pr * fi + pi * fr);
-                } else {
-                    psi[2 * k] = 0.0;
-                    psi[2 * k + 1] = 0.0;
-                }
-            }
+    return 0;
+}
+
+int orc_propagate_position(const double* probe, int64_t sx, int64_t sy,
+                           int n_slices, const double* t_stack,
+                           const double* px, const double* py,
+                           const double* kx, const double* ky, int Nx, int Ny,
+                           double bwcutoff_t, int transmit_mode, double* psi)
+{
+    const size_t M = (size_t)Nx * Ny;
+    const int r = orc_initial_wave(probe, sx, sy, Nx, Ny, psi);
+    if (r) return r;
+    for (int n = 0; n < n_slices; n++) {
+        if (orc_slice_step(psi, t_stack + 2 * M * n, px + 2 * (size_t)Nx * n,
+                           py + 2 * (size_t)Ny * n, kx, ky, Nx, Ny, bwcutoff_t, transmit_mode))
+            return -1;
     }
     return 0;
 }

@@ -78,6 +78,10 @@ void orc_probe_shifts(const double* xx, int nx, const double* yy, int ny,
                       int64_t* sx, int64_t* sy);
 
 /* ---- slice loop, detectors ---- */
+int orc_initial_wave(const double* probe, int64_t sx, int64_t sy, int nx, int ny, double* psi);
+int orc_slice_step(double* psi, const double* t, const double* px, const double* py,
+                   const double* kx, const double* ky, int nx, int ny, double bwcutoff_t,
+                   int transmit_mode);
 int orc_propagate_position(const double* probe, int64_t sx, int64_t sy,
                            int n_slices, const double* t_stack,
                            const double* px, const double* py,

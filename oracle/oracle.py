@@ -97,6 +97,9 @@ def lib():
         L.orc_specimen_free.argtypes = [C.POINTER(_Spec)]
         L.orc_adfstem.argtypes = [C.POINTER(Job), C.POINTER(_Spec), _dp, C.c_int, _dp, C.c_int,
                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_initial_wave.argtypes = [_cp, C.c_int64, C.c_int64, C.c_int, C.c_int, _cp]
+        L.orc_slice_step.argtypes = [_cp, _cp, _cp, _cp, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                     C.c_int]
         L.orc_adfstem_fem.argtypes = [C.POINTER(Job), C.POINTER(_Spec), _dp, C.c_int, _dp,
                                       C.c_int, C.c_int] + [C.c_void_p] * 8
         L.orc_fem_modes.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, C.c_int, C.c_double,
@@ -349,3 +352,23 @@ def fem_modes(mag2, kx, ky, delta_k, nbins, lam, alpha_max, radial1, counts1):
                              *[ptr(outs[m]) for m in FEM_MODES[1:]])
     assert rc == 0
     return {m: v[:nbins] for m, v in outs.items()}
+
+
+def initial_wave(probe_x, sx, sy):
+    """orc_initial_wave: shifted probe, FFT / sqrt(NxNy)."""
+    nx, ny = probe_x.shape
+    out = np.zeros((nx, ny), np.complex128)
+    assert lib().orc_initial_wave(np.ascontiguousarray(probe_x, np.complex128), sx, sy, nx, ny,
+                                  out) == 0
+    return out
+
+
+def slice_step(psi_k, t, px, py, kx, ky, bwcutoff_t, transmit_mode=0):
+    """orc_slice_step on a copy of psi_k [nx, ny]."""
+    w = np.ascontiguousarray(psi_k, np.complex128).copy()
+    nx, ny = w.shape
+    assert lib().orc_slice_step(w, np.ascontiguousarray(t, np.complex128),
+                                np.ascontiguousarray(px, np.complex128),
+                                np.ascontiguousarray(py, np.complex128), kx, ky, nx, ny,
+                                bwcutoff_t, transmit_mode) == 0
+    return w

@@ -226,3 +226,35 @@ def test_specimen_may_outlive_its_context(msf):
     with pytest.raises(msf.MsfError):
         spec.transmission(0)
     spec.free()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_single_step_entry_points_match_oracle(msf, ctx, O, mode):
+    """msf_probe_k, msf_specimen_propagator and msf_slice_step_batch (FP64 on
+    the device) against orc_initial_wave, the oracle's propagators and
+    orc_slice_step, for every slice of a small specimen."""
+    q, Z = random_atoms(30, (9.0, 9.0, 6.0), seed=29, species=(14, 40))
+    c = Case(nx=64, ny=96, cell=(9.0, 12.0, 6.0), voltage=200e3, q=q, Z=Z, min_slice=1.5,
+             alpha_max=0.02, cs3=4e5, defocus=15.0, transmit_mode=mode)
+    job = c.oracle_job(O)
+    spec_o = O.Specimen(job, c.q, c.Z.astype(np.int32))
+    spec_g = c.gpu_specimen(msf, ctx)
+    kx, ky, xx, yy = O.domains(64, 96, 9.0, 12.0)
+    _, bwt = O.bwcutoffs(64, 96, 9.0, 12.0)
+    probe = O.probe(job)
+    shifts = [(0, 0), (7, 40), (63, 95)]
+    w_g = np.stack([ctx.probe_k(c.grid(msf), c.optics(msf), sx, sy) for sx, sy in shifts])
+    w_o = np.stack([O.initial_wave(probe, sx, sy) for sx, sy in shifts])
+    assert rel_l2(w_g, w_o) < 1e-11
+    ctx.configure(transmit_mode=mode)
+    try:
+        for n in range(spec_g.n_slices):
+            px, py = spec_g.propagator(n)
+            np.testing.assert_allclose(px, spec_o.px[n], rtol=0, atol=1e-15)
+            np.testing.assert_allclose(py, spec_o.py[n], rtol=0, atol=1e-15)
+            w_g = ctx.slice_step(spec_g, n, w_o)   # same input on both sides
+            w_o = np.stack([O.slice_step(w, spec_o.t_stack[n], spec_o.px[n], spec_o.py[n], kx,
+                                         ky, bwt, mode) for w in w_o])
+            assert rel_l2(w_g, w_o) < 1e-10, n
+    finally:
+        ctx.configure()

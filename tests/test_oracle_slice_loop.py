@@ -137,3 +137,23 @@ def test_fem_variance_closed_forms(O):
     assert v[1] == pytest.approx(0.0, abs=1e-15)   # constant r
     assert v[2] == 0.0                              # sum r == 0 -> 0
     assert np.isnan(v[3])                           # empty ring (0/0) stays NaN
+
+
+def test_factored_step_reproduces_the_position_loop(O):
+    """orc_initial_wave followed by orc_slice_step over every slice is the
+    whole-position loop orc_propagate_position, bit for bit."""
+    from helpers import Case, random_atoms
+    q, Z = random_atoms(20, (6.0, 6.0, 4.0), seed=12, species=(14, 29))
+    c = Case(nx=32, ny=32, cell=(6.0, 6.0, 4.0), voltage=300e3, q=q, Z=Z, min_slice=1.0,
+             alpha_max=0.02)
+    job = c.oracle_job(O)
+    spec = O.Specimen(job, c.q, c.Z.astype(np.int32))
+    kx, ky, xx, yy = O.domains(32, 32, 6.0, 6.0)
+    _, bwt = O.bwcutoffs(32, 32, 6.0, 6.0)
+    probe = O.probe(job)
+    _, _, ex = O.adfstem(job, spec, np.array([xx[5]]), np.array([yy[9]]), exit_waves=True)
+    sx, sy = (np.asarray(a)[0] for a in O.probe_shifts(xx, yy, np.array([xx[5]]), np.array([yy[9]])))
+    w = O.initial_wave(probe, int(sx), int(sy))
+    for n in range(spec.n_slices):
+        w = O.slice_step(w, spec.t_stack[n], spec.px[n], spec.py[n], kx, ky, bwt)
+    np.testing.assert_array_equal(w, ex[0])
