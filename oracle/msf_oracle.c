@@ -759,6 +759,86 @@ int orc_fem_modes(const double* mag2, long P, const double* kx, const double* ky
     return 0;
 }
 
+/* adfstem.cpp:969-985, :1013-1027: the correlograph's phi bins number the k-bin
+ * boundaries ii*delta_k (ii = 0..nbins) lying in [0.2, 1.0], minus one. */
+int orc_correlograph_nphi(double delta_k, int nbins)
+{
+    unsigned int count = 0;
+    for (unsigned int ii = 0; ii < (unsigned int)nbins + 1; ++ii) {
+        const double k_bdy = ii * delta_k;
+        if ((k_bdy >= 0.2) && (k_bdy <= 1.0)) ++count;
+    }
+    return count > 0 ? (int)count - 1 : 0;
+}
+
+/* The correlograph (adfstem.cpp:1013-1027 bins, variance.cpp:226-250 phi
+ * sectors, adfstem.cpp:2603-2650 per position, :3620-3632 average), from the
+ * per-position intensities |psi|^2 and ring sums of |psi| (zeroed on the
+ * direct beam).  Sector jj of ring ii holds pixels with
+ * phi = PI + atan2(ky, kx) in (jj dphi, (jj+1) dphi]; per position
+ * c = count_ii * I_ii,jj / r_ii - 1 where that ratio is >= 1 and all three are
+ * non-zero, else 0; avg = sum_c / P and sqr = sum_c^2, index ii + jj*nbins. */
+int orc_correlograph(const double* mag2, long P, const double* kx, const double* ky, int Nx,
+                     int Ny, double delta_k, int nbins, double lambda, double alpha_max,
+                     const double* radial1, const int32_t* counts1, double* avg, double* sqr)
+{
+    const size_t M = (size_t)Nx * Ny;
+    const int nphi = orc_correlograph_nphi(delta_k, nbins);
+    if (nphi <= 0) return 0;
+    const double delta_phi = 2 * REF_PI / nphi;
+    kmag* m = (kmag*)malloc(sizeof(kmag) * M);
+    int32_t* sector = (int32_t*)malloc(sizeof(int32_t) * M);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const size_t k = (size_t)j + (size_t)i * Ny;
+            m[k].v = kx[i] * kx[i] + ky[j] * ky[j];
+            m[k].idx = (long)k;
+            const double phi = REF_PI + atan2(ky[j], kx[i]);
+            double lower = 0 * delta_phi;
+            sector[k] = -1;
+            for (int jj = 0; jj < nphi; jj++) { /* variance.cpp:229-250 */
+                if ((phi > lower) && (phi <= (unsigned int)(jj + 1) * delta_phi)) {
+                    sector[k] = jj;
+                    break;
+                }
+                lower = (unsigned int)jj * delta_phi;
+            }
+        }
+    qsort(m, M, sizeof(kmag), kmag_cmp);
+    const size_t NC = (size_t)nbins * nphi;
+    double* I = (double*)malloc(sizeof(double) * NC);
+    for (size_t c = 0; c < NC; c++) { avg[c] = 0.0; if (sqr) sqr[c] = 0.0; }
+    for (long p = 0; p < P; p++) {
+        for (size_t c = 0; c < NC; c++) I[c] = 0.0;
+        size_t it = 0;
+        for (int ii = 0; ii < nbins; ii++) {
+            const double upper = (unsigned int)(ii + 1) * delta_k;
+            for (; it < M && m[it].v <= (upper * upper); ++it) {
+                const long k = m[it].idx;
+                if (sector[k] >= 0)
+                    I[(size_t)sector[k] + (size_t)ii * nphi] += sqrt(mag2[(size_t)p * M + k]);
+            }
+            if (lambda * upper <= alpha_max)
+                for (int jj = 0; jj < nphi; jj++) I[(size_t)jj + (size_t)ii * nphi] = 0.0;
+        }
+        const double* r = radial1 + (size_t)p * nbins;
+        const int32_t* cnt = counts1 + (size_t)p * nbins;
+        for (int ii = 0; ii < nbins; ii++)
+            for (int jj = 0; jj < nphi; jj++) {
+                const double Iv = I[(size_t)jj + (size_t)ii * nphi];
+                const double t = (cnt[ii] * Iv / r[ii]);
+                double c = 0.0;
+                if ((t >= 1) && (cnt[ii] != 0) && (Iv != 0) && (r[ii] != 0)) c = t - 1;
+                const size_t ci = (size_t)ii + (size_t)jj * nbins;
+                avg[ci] += c;
+                if (sqr) sqr[ci] += c * c;
+            }
+    }
+    for (size_t c = 0; c < NC; c++) avg[c] = avg[c] / P;
+    free(m); free(sector); free(I);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* tem.cpp:944-1276: unique species, templates (scatterer_pap_LUT),
  * auto-slicing, per-slice propagator and transmission function. */
@@ -838,13 +918,14 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
                 int n_threads, double* adf_image, double* fem_v, double* exit_psi)
 {
     return orc_adfstem_fem(job, spec, x_p, nxp, y_p, nyp, n_threads, adf_image, fem_v, NULL,
-                           NULL, NULL, NULL, NULL, exit_psi);
+                           NULL, NULL, NULL, NULL, NULL, NULL, exit_psi);
 }
 
 int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                     const double* x_p, int nxp, const double* y_p, int nyp, int n_threads,
                     double* adf_image, double* fem_v, double* vbar_r, double* v_re,
-                    double* omega_vimage, double* v_gt17, double* v_rbar, double* exit_psi)
+                    double* omega_vimage, double* v_gt17, double* v_rbar, double* corr_avg,
+                    double* corr_sqr, double* exit_psi)
 {
     const int Nx = job->nx, Ny = job->ny;
     const size_t M = (size_t)Nx * Ny;
@@ -868,7 +949,8 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
     if (job->fem) nbins = orc_fem_nbins(kx, ky, job->dr, bwt, &delta_k);
     double* radial = (double*)malloc(sizeof(double) * (size_t)P * (nbins ? nbins : 1));
     int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)P * (nbins ? nbins : 1));
-    const int other_modes = job->fem && (vbar_r || v_re || omega_vimage || v_gt17 || v_rbar);
+    const int other_modes =
+        job->fem && (vbar_r || v_re || omega_vimage || v_gt17 || v_rbar || corr_avg);
     double* mag2 = other_modes ? (double*)malloc(sizeof(double) * (size_t)P * M) : NULL;
     double k1 = job->det_inner / lambda, k2 = job->det_outer / lambda;
     if (job->adf_mode == 1) { k1 = k1 * k1; k2 = k2 * k2; }
@@ -914,6 +996,9 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
     if (mag2) {
         orc_fem_modes(mag2, P, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max, radial,
                       counts, vbar_r, v_re, omega_vimage, v_gt17, v_rbar);
+        if (corr_avg)
+            orc_correlograph(mag2, P, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max,
+                             radial, counts, corr_avg, corr_sqr);
         free(mag2);
     }
     free(kx); free(ky); free(xx); free(yy); free(probe); free(sx); free(sy);

@@ -122,7 +122,12 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
 int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                     const double* x_p, int nxp, const double* y_p, int nyp, int n_threads,
                     double* adf_image, double* fem_v, double* vbar_r, double* v_re,
-                    double* omega_vimage, double* v_gt17, double* v_rbar, double* exit_psi);
+                    double* omega_vimage, double* v_gt17, double* v_rbar, double* corr_avg,
+                    double* corr_sqr, double* exit_psi);
+int orc_correlograph_nphi(double delta_k, int nbins);
+int orc_correlograph(const double* mag2, long n_positions, const double* kx, const double* ky,
+                     int nx, int ny, double delta_k, int nbins, double lambda, double alpha_max,
+                     const double* radial1, const int32_t* counts1, double* avg, double* sqr);
 
 #ifdef __cplusplus
 }

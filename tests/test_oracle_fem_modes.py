@@ -75,3 +75,44 @@ def test_pixel_pattern_constant_over_positions(O, geom):
     np.testing.assert_allclose(out["vbar_r"][live], out["v_re"][live], rtol=1e-12)
     np.testing.assert_allclose(out["v_gt17"][live], 1.0, rtol=1e-12)
     np.testing.assert_allclose(out["omega_vimage"][live], 0.0, atol=1e-12)
+
+
+def test_correlograph_uniform_field(O):
+    """A uniform |psi| gives, per ring ii and phi sector jj, count_ii *
+    (n_s A) / (count_ii A) - 1 = n_s - 1 (n_s = pixels of the sector), 0 on
+    empty sectors and on direct-beam rings; the sectors of a ring hold all of
+    its pixels."""
+    nx = ny = 48
+    kx, ky, _, _ = O.domains(nx, ny, 6.0, 6.0)
+    _, bwt = O.bwcutoffs(nx, ny, 6.0, 6.0)
+    nb, dk = O.fem_nbins(kx, ky, 0.5, bwt)
+    nphi = O.lib().orc_correlograph_nphi(dk, nb)
+    assert nphi > 4
+    lam, alpha = 0.025, 0.025 * dk
+    P, A = 3, 0.4
+    mag = np.full((P, nx, ny), A)
+    r1 = np.zeros((P, nb)); c1 = np.zeros((P, nb), np.int32)
+    for p in range(P):
+        r1[p], c1[p] = O.fem_radial(mag[p].astype(np.complex128), kx, ky, dk, nb, lam, alpha)
+    avg = np.zeros(nb * nphi); sqr = np.zeros(nb * nphi)
+    assert O.lib().orc_correlograph((mag ** 2).reshape(-1), P, kx, ky, nx, ny, dk, nb, lam,
+                                    alpha, r1.reshape(-1), c1.reshape(-1), avg, sqr) == 0
+    avg = avg.reshape(nphi, nb)
+    sqr = sqr.reshape(nphi, nb)
+    # sector populations, counted independently
+    K2 = kx[:, None] ** 2 + ky[None, :] ** 2
+    phi = 3.14159265369 + np.arctan2(ky[None, :], kx[:, None])
+    dphi = 2 * 3.14159265369 / nphi
+    sec = np.clip(np.ceil(phi / dphi).astype(int) - 1, 0, nphi - 1)
+    order = np.argsort(K2, axis=None, kind="stable")
+    ring = np.full(nx * ny, -1)
+    for ii in range(nb):
+        lo = (ii * dk) ** 2 if ii else -1.0
+        m = (K2.reshape(-1) <= ((ii + 1) * dk) ** 2) & (K2.reshape(-1) > lo)
+        ring[m] = ii
+    for ii in range(nb):
+        zeroed = lam * (ii + 1) * dk <= alpha
+        ns = np.bincount(sec.reshape(-1)[ring == ii], minlength=nphi)
+        want = np.where(ns > 0, ns - 1.0, 0.0) if not zeroed else np.zeros(nphi)
+        np.testing.assert_allclose(avg[:, ii], want, atol=1e-12, err_msg=str(ii))
+        np.testing.assert_allclose(sqr[:, ii], P * want ** 2, atol=1e-10)
