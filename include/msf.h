@@ -169,6 +169,13 @@ int msf_slice_step_batch(msf_ctx* ctx, const msf_specimen* spec, uint32_t slice,
 /* Number of V_omega bins (adfstem.cpp:947-972) and their lower edges
  * (n_bins doubles, may be NULL). */
 int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lower_edges);
+/* Number of correlograph phi sectors (adfstem.cpp:969-985, :1013-1027): the
+ * V_omega bin boundaries ii*delta_k (ii = 0..n_bins) lying in [0.2, 1.0] 1/A,
+ * minus one; sector jj spans phi = PI + atan2(ky, kx) in
+ * (jj, jj+1] * 2 PI / n_phi.  k_edges (n_bins + 1 doubles) and phi_edges
+ * (n_phi + 1 doubles) receive the boundaries when not NULL. */
+int msf_correlograph_bins(const msf_grid* grid, double dr, uint32_t* n_phi, double* k_edges,
+                          double* phi_edges);
 
 /* Replaces TEM_NS::adfstem() (adfstem.hpp:66-121, body adfstem.cpp:37-4096)
  * for the default modes: raster loop over x_p (outer) and y_p (inner),
@@ -197,7 +204,13 @@ int msf_probe_shifts(const msf_grid* grid, const double* x_p, uint32_t n_xp, con
  *   v_gt17        --V_gt17: ring mean of <|psi|^2> over ring mean of <|psi|>^2
  *   v_rbar        --V_rbar: ring mean of <|psi|>^2 over (ring mean of <|psi|>)^2, minus 1
  * (<.> = mean over positions).  Empty and direct-beam bins follow the
- * reference (NaN or 0 as its arithmetic gives). */
+ * reference (NaN or 0 as its arithmetic gives).
+ *   correlograph_avg, correlograph_sqr_sum   --correlograph (adfstem.cpp:2603-2650,
+ *       :3620-3632): n_bins * n_phi doubles (msf_correlograph_bins), index
+ *       ii + jj * n_bins for ring ii and phi sector jj; per position
+ *       c = count_ii * S_ii,jj / S_ii - 1 (S: sums of |psi| over the sector
+ *       and the ring) where that ratio is >= 1, else 0; the mean of c and
+ *       the sum of c^2 over positions. */
 typedef struct msf_fem_outputs {
     double* v_omega;
     double* vbar_r;
@@ -205,6 +218,8 @@ typedef struct msf_fem_outputs {
     double* omega_vimage;
     double* v_gt17;
     double* v_rbar;
+    double* correlograph_avg;
+    double* correlograph_sqr_sum;
 } msf_fem_outputs;
 
 /* msf_run_stem with every FEM mode above; fem->enabled must be set for any

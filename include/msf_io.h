@@ -61,6 +61,13 @@ int msf_write_tiff16(const char* path, const double* data, uint32_t height, uint
 int msf_write_variance_txt(const char* path, const double* values, const double* lower_edges,
                            uint32_t n_bins);
 
+/* output_correlograph_to_txt (io_txt.cpp:96-165): first line "0 " and the
+ * lower phi boundaries phi_edges[jj] (jj < n_phi), then one line per k bin ii
+ * (ii < n_k): k_edges[ii] and data[ii + jj * n_k] for every jj; every value
+ * "%.10e " and every line ending in a space before the newline. */
+int msf_write_correlograph_txt(const char* path, const double* data, const double* k_edges,
+                               uint32_t n_k, const double* phi_edges, uint32_t n_phi);
+
 /* float64 sidecar: a one-line text header "msf-f64 <rows> <cols>\n"
  * followed by rows*cols little-endian doubles, row-major */
 int msf_write_f64(const char* path, const double* data, uint32_t rows, uint32_t cols);

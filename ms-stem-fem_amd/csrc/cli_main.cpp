@@ -35,6 +35,7 @@ struct Flags {
     bool bfctem_corrected = false, bfctem_uncorrected = false;
     bool fem = false, d1 = false, d2 = false, d3 = false, d4 = false, gt17 = false, rva = false;
     bool correlograph = false, image_output = false, diffraction_output = false;
+    bool correlograph_variance = false, correlograph_every = false;
     bool netcdf_images = false, netcdf_variance = false, pap_tif = false, dupe = false;
     bool mtf_file = false, mtf_resolution = false;
     bool lammps_pre = false, lammps_steps = false, lammps_samples = false;
@@ -123,9 +124,10 @@ bool read_parameter_file(const std::string& name, Flags& f, Params& p)
         else if (key == "diffractionimages") f.diffraction_output = true;
         else if (key == "netcdfimages") f.netcdf_images = true;
         else if (key == "netcdfvariance") f.netcdf_variance = true;
-        else if (key == "correlograph" || key == "correlograph_variance" ||
-                 key == "correlograph_everyimage" || key == "correlograph_everytxt")
-            f.correlograph = true;
+        else if (key == "correlograph") f.correlograph = true;
+        else if (key == "correlograph_variance") f.correlograph = f.correlograph_variance = true;
+        else if (key == "correlograph_everyimage" || key == "correlograph_everytxt")
+            f.correlograph = f.correlograph_every = true;
         else if (key == "lammps_pretem_file_name") f.lammps_pre = true;
         else if (key == "lammps_tem_steps") f.lammps_steps = true;
         else if (key == "lammps_tem_samples") f.lammps_samples = true;
@@ -208,9 +210,10 @@ void read_cmdline(const std::vector<std::string>& args, Flags& f, Params& p)
         else if (a == "--images") f.image_output = true;
         else if (a == "--netcdfimages") f.netcdf_images = true;
         else if (a == "--netcdfvariance") f.netcdf_variance = true;
-        else if (a == "--correlograph" || a == "--correlograph_variance" ||
-                 a == "--correlograph_everyimage" || a == "--correlograph_everytxt")
-            f.correlograph = true;
+        else if (a == "--correlograph") f.correlograph = true;
+        else if (a == "--correlograph_variance") f.correlograph = f.correlograph_variance = true;
+        else if (a == "--correlograph_everyimage" || a == "--correlograph_everytxt")
+            f.correlograph = f.correlograph_every = true;
         else if (a == "--lammps_preTEM_file_name") { f.lammps_pre = true; i += 1; }
         else if (a == "--lammps_TEM_steps") { f.lammps_steps = true; i += 1; }
         else if (a == "--lammps_TEM_samples") { f.lammps_samples = true; i += 1; }
@@ -248,7 +251,7 @@ bool check_supported(const Flags& f)
 {
     struct Item { bool on; const char* what; };
     const Item items[] = {
-        {f.correlograph, "--correlograph*"},
+        {f.correlograph_every, "--correlograph_everyimage / --correlograph_everytxt"},
         {f.bfctem_corrected || f.bfctem_uncorrected, "--bfctem*"},
         {f.mtf_file, "--mtf_file"}, {f.pap_tif, "--paptif"},
         {f.diffraction_output, "diffractionimages"},
@@ -297,6 +300,53 @@ int write_diffraction(const std::string& prefix, const float* psi, long nx, long
     }
     const std::string name = prefix + "_diffraction_scalefactor_" + to_string_g(s) + ".tif";
     return msf_write_tiff16(name.c_str(), mag.data(), (uint32_t)nx, (uint32_t)ny, mn, mx, 0, 1, xres, yres);
+}
+
+// Correlograph outputs (adfstem.cpp:3617-3745): the average as text and a
+// linear 16-bit image (phi along x, k along y); with the variance flag, the
+// text file of the sums of squares (the reference writes those under the
+// variance name) and log and linear images of the variance over the k bins
+// from the one holding 0.2 up to, not including, the one holding 1.0 1/A.
+// Empty images are skipped (the reference reads element 0 of an empty array).
+int write_correlograph(const std::string& out, const std::vector<double>& avg,
+                       const std::vector<double>& sqr, const std::vector<double>& k_edges,
+                       const std::vector<double>& phi_edges, uint32_t nk, uint32_t nphi,
+                       size_t P, bool variance)
+{
+    const double kres = (nk / (1.0 - 0.2)) * 1.0e8; // adfstem.cpp:2651-2654
+    const double phires = (double)(nphi / 360);      // integer division, as in the reference
+    auto image = [&](const double* v, uint32_t w, uint32_t h, const std::string& name, int lg) {
+        if ((size_t)w * h == 0) return MSF_OK;
+        double mn = v[0], mx = v[0];
+        for (size_t i = 0; i < (size_t)w * h; ++i) {
+            if (v[i] > mx) mx = v[i];
+            if (v[i] < mn) mn = v[i];
+        }
+        return msf_write_tiff16(name.c_str(), v, w, h, mn, mx, lg, 3, phires, kres);
+    };
+    int w = msf_write_correlograph_txt((out + "_correlograph_avg.txt").c_str(), avg.data(),
+                                       k_edges.data(), nk, phi_edges.data(), nphi);
+    w |= image(avg.data(), nphi, nk, out + "_correlograph_avg.tif", 0);
+    if (!variance) return w;
+    size_t start = 0, end = nk ? nk - 1 : 0;
+    for (size_t ii = 0; ii < nk; ++ii) {
+        if (k_edges[ii] <= 0.2 && k_edges[ii + 1] > 0.2) start = ii;
+        if (k_edges[ii] <= 1.0 && k_edges[ii + 1] > 1.0) end = ii;
+    }
+    const size_t nred = (nk && end >= start) ? end - start : 0;
+    std::vector<double> var(nred * nphi);
+    for (size_t ii = start; ii < start + nred; ++ii)
+        for (size_t jj = 0; jj < nphi; ++jj) {
+            const size_t ci = ii + jj * nk;
+            const double t = sqr[ci] / (P * avg[ci] * avg[ci]);
+            var[(ii - start) + jj * nred] =
+                (sqr[ci] != 0 && P != 0 && avg[ci] != 0 && t >= 1) ? t - 1 : 0;
+        }
+    w |= msf_write_correlograph_txt((out + "_correlograph_variance.txt").c_str(), sqr.data(),
+                                    k_edges.data(), nk, phi_edges.data(), nphi);
+    w |= image(var.data(), nphi, (uint32_t)nred, out + "_correlograph_variance_logscale.tif", 1);
+    w |= image(var.data(), nphi, (uint32_t)nred, out + "_correlograph_variance.tif", 0);
+    return w;
 }
 
 } // namespace
@@ -422,10 +472,12 @@ int main(int argc, char** argv)
         std::printf("raster %d %d %.17g\ndetector %.17g %.17g\n", nrx, nry, p.raster_spacing,
                     p.det_inner, p.det_outer);
         std::printf("images %d\nfem %d\n", f.image_output ? 1 : 0, (f.fem && f.d1) ? 1 : 0);
-        std::printf("fem_modes%s%s%s%s%s%s\n", (f.fem && f.d1) ? " V_omega" : "",
+        std::printf("fem_modes%s%s%s%s%s%s%s%s\n", (f.fem && f.d1) ? " V_omega" : "",
                     (f.fem && f.d2) ? " Vbar_r" : "", (f.fem && f.d3) ? " V_re" : "",
                     (f.fem && f.d4) ? " Omega_vimage" : "", (f.fem && f.gt17) ? " V_gt17" : "",
-                    (f.fem && f.rva) ? " V_rbar" : "");
+                    (f.fem && f.rva) ? " V_rbar" : "",
+                    (f.fem && f.correlograph) ? " correlograph" : "",
+                    (f.fem && f.correlograph_variance) ? " correlograph_variance" : "");
         size_t k = 0;
         for (const Mode& m : modes)
             if (m.on) {
@@ -455,14 +507,22 @@ int main(int argc, char** argv)
     det.outer = p.det_outer;
     det.threshold_mode = MSF_ADF_REFERENCE;
     msf_fem fem{};
-    fem.enabled = (f.fem && (f.d1 || f.d2 || f.d3 || f.d4 || f.gt17 || f.rva)) ? 1 : 0;
+    const bool corr = f.fem && f.correlograph;
+    fem.enabled = (f.fem && (f.d1 || f.d2 || f.d3 || f.d4 || f.gt17 || f.rva || corr)) ? 1 : 0;
     fem.dr = p.dr;
-    uint32_t nbins = 0;
-    std::vector<double> edges;
+    uint32_t nbins = 0, nphi = 0;
+    std::vector<double> edges, k_edges, phi_edges;
     if (fem.enabled) {
         if (msf_fem_bins(&grid, p.dr, &nbins, nullptr) != MSF_OK) return die("fem bins", c);
         edges.resize(std::max<uint32_t>(nbins, 1));
         msf_fem_bins(&grid, p.dr, &nbins, edges.data());
+    }
+    if (corr) {
+        k_edges.resize(nbins + 1);
+        if (msf_correlograph_bins(&grid, p.dr, &nphi, k_edges.data(), nullptr) != MSF_OK)
+            return die("correlograph bins", c);
+        phi_edges.resize(nphi + 1);
+        msf_correlograph_bins(&grid, p.dr, &nphi, k_edges.data(), phi_edges.data());
     }
     std::vector<int32_t> shifts(2 * xp.size() * yp.size());
     if (msf_probe_shifts(&grid, xp.data(), (uint32_t)xp.size(), yp.data(), (uint32_t)yp.size(),
@@ -488,7 +548,10 @@ int main(int argc, char** argv)
                 files[m].v.assign(std::max<uint32_t>(nbins, 1), 0.0);
                 ptr[m] = files[m].v.data();
             }
-        const msf_fem_outputs fo{ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5]};
+        const size_t ncorr = std::max<size_t>((size_t)nbins * nphi, 1);
+        std::vector<double> c_avg(corr ? ncorr : 0), c_sqr(corr ? ncorr : 0);
+        const msf_fem_outputs fo{ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5],
+                                 corr ? c_avg.data() : nullptr, corr ? c_sqr.data() : nullptr};
         if (msf_run_stem_fem(c, spec, &o, &det, &fem, xp.data(), (uint32_t)xp.size(), yp.data(),
                              (uint32_t)yp.size(), img.data(), &fo) != MSF_OK)
             return die("adfstem", c);
@@ -502,6 +565,11 @@ int main(int argc, char** argv)
                 msf_write_variance_txt((out + files[m].suffix + ".txt").c_str(), files[m].v.data(),
                                        edges.data(), nbins))
                 std::cout << " failed to write variance data to a txt file : " << out << std::endl;
+        if (corr && write_correlograph(out, c_avg, c_sqr, k_edges, phi_edges, nbins, nphi,
+                                       xp.size() * yp.size(), f.correlograph_variance)) {
+            std::cerr << "Error writing the correlograph: " << msf_io_last_error() << std::endl;
+            rc = EXIT_FAILURE;
+        }
         if (f.image_output) { // adfstem.cpp:3764-3797, write_tif.cpp:1540-1600
             double mn = img[0], mx = img[0];
             for (double a : img) {

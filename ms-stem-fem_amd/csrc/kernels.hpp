@@ -87,6 +87,11 @@ hipError_t launch_fem_finalize(const double* state, int nbins, double* v, hipStr
 hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbins, int off_abs,
                            int off_sqr, const int32_t* counts, const uint8_t* zeroed,
                            double* vsum, hipStream_t s);
+// correlograph sums of c and c^2 per (ring, phi sector) over a batch
+hipError_t launch_fem_correlograph(const double* partial, int groups, int batch, int nbins,
+                                   int nphi, int off_abs, int off_corr, const int32_t* counts,
+                                   const uint8_t* zeroed, double* csum, double* csqr,
+                                   hipStream_t s);
 // per-pixel FP64 sums of |psi| and |psi|^2 over the positions of a batch
 hipError_t launch_pixel_moments(const float2* psi, int64_t plane, int batch, double* s1,
                                 double* s2, hipStream_t s);

@@ -621,6 +621,48 @@ hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbi
     return hipGetLastError();
 }
 
+// Correlograph (adfstem.cpp:2603-2650): per position, ring ii and phi sector
+// jj, c = count_ii * I_ii,jj / r_ii - 1 when that ratio is >= 1 and count,
+// I and r are all non-zero, else 0; I and r are the sector and ring sums of
+// |psi| (zero on direct-beam rings).  Sums of c and c^2 over positions in
+// raster order, stored [ii + jj * nbins].
+__global__ void fem_correlograph_kernel(const double* partial, int groups, int batch, int nbins,
+                                        int nphi, int off_abs, int off_corr,
+                                        const int32_t* counts, const uint8_t* zeroed,
+                                        double* csum, double* csqr)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x; // ii * nphi + jj
+    if (k >= nbins * nphi) return;
+    const int ii = k / nphi, jj = k - ii * nphi;
+    const int64_t o = (int64_t)ii + (int64_t)jj * nbins;
+    const int c = counts[ii];
+    const bool z = zeroed[ii] != 0;
+    double a = csum[o], q = csqr[o];
+    for (int b = 0; b < batch; ++b) {
+        const double r = z ? 0.0 : partial[(int64_t)b * groups + off_abs + ii];
+        const double I = z ? 0.0 : partial[(int64_t)b * groups + off_corr + k];
+        const double t = (c * I / r);
+        double v = 0.0;
+        if ((t >= 1) && (c != 0) && (I != 0) && (r != 0)) v = t - 1;
+        a += v;
+        q += v * v;
+    }
+    csum[o] = a;
+    csqr[o] = q;
+}
+
+hipError_t launch_fem_correlograph(const double* partial, int groups, int batch, int nbins,
+                                   int nphi, int off_abs, int off_corr, const int32_t* counts,
+                                   const uint8_t* zeroed, double* csum, double* csqr,
+                                   hipStream_t s)
+{
+    if (nbins <= 0 || nphi <= 0 || batch <= 0) return hipSuccess;
+    const int n = nbins * nphi;
+    hipLaunchKernelGGL(fem_correlograph_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partial,
+                       groups, batch, nbins, nphi, off_abs, off_corr, counts, zeroed, csum, csqr);
+    return hipGetLastError();
+}
+
 // Per-pixel sums over positions of |psi| and |psi|^2 in raster order, FP64
 // (adfstem.cpp:2753-2781), on the compact plane of the exit waves.
 __global__ void pixel_moments_kernel(const float2* psi, int64_t plane, int batch, double* s1,

@@ -79,10 +79,12 @@ struct RunPlan {
     DevBuf seg_pix, tile_off, seg_kind, seg_of_group, cnt, zero;  // reduction groups
     int n_tiles = 0;
     bool fem_sqr = false;                     // groups carry the |psi|^2 FEM rings (Vbar_r)
+    bool corr = false;                        // groups carry the correlograph sectors
     msf::Groups hg;                           // host copy for the per-pixel FEM modes
     int64_t cap_b = 0, cap_p = 0;
     DevBuf psi, part, segp, state, femv, adf, shift, exit;
     DevBuf vsum, pm1, pm2;                    // Vbar_r sums; per-pixel |psi|, |psi|^2 sums
+    DevBuf csum, csqr;                        // correlograph sums of c and c^2
     // the job's launch sequence as an instantiated HIP graph, replayed while
     // every pointer and size it was captured with is unchanged
     hipGraphExec_t graph = nullptr;
@@ -275,6 +277,8 @@ struct RunOut {
     double* omega_vimage = nullptr; // d4
     double* v_gt17 = nullptr;
     double* v_rbar = nullptr;   // rva
+    double* corr_avg = nullptr; // correlograph, [ii + jj * nbins]
+    double* corr_sqr = nullptr;
     float* exit = nullptr;
 };
 
@@ -352,7 +356,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const bool want_d1 = fem_on && out.fem;
     const bool want_d2 = fem_on && out.vbar_r;
     const bool want_pix = fem_on && (out.v_re || out.omega_vimage || out.v_gt17 || out.v_rbar);
-    const bool want_fem = want_d1 || want_d2 || want_pix;
+    const bool want_corr = fem_on && (out.corr_avg || out.corr_sqr);
+    const bool want_fem = want_d1 || want_d2 || want_pix || want_corr;
     if (want_adf) {
         if (det->inner < 0.0 || det->outer < 0.0)
             return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
@@ -383,7 +388,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                              std::memcmp(&pl.optics, o, sizeof(msf_optics)) == 0;
     const bool same_groups = same_optics && std::memcmp(&pl.det, &dkey, sizeof dkey) == 0 &&
                              std::memcmp(&pl.fem, &fkey, sizeof fkey) == 0 &&
-                             pl.fem_sqr == want_d2;
+                             pl.fem_sqr == want_d2 && pl.corr == want_corr;
     if (!same_optics || !same_groups) pl.valid = false;
 
     if (!same_groups) {
@@ -397,10 +402,12 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             }
         }
         const msf::Groups g = msf::build_groups(d, keep, ncp, want_adf, k1, k2, want_fem,
-                                                want_fem ? fem->dr : 1.0, o->alpha_max, want_d2);
+                                                want_fem ? fem->dr : 1.0, o->alpha_max, want_d2,
+                                                want_corr);
         pl.G = (int)g.kind.size();
         pl.nbins = g.nbins;
         pl.fem_sqr = want_d2;
+        pl.corr = want_corr;
         const msf::SegmentTiles st = msf::build_segments(g, 32);
         pl.n_tiles = st.n_tiles;
         HIPCHK(c, upload(pl.seg_pix, st.pix, c->stream));
@@ -414,6 +421,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     }
     const int G = pl.G;
     const int nbins = pl.nbins;
+    const int nphi = want_corr ? pl.hg.nphi : 0;
+    const size_t ncorr = (size_t)std::max(nbins * nphi, 1);
 
     if (!same_optics) {
         // vacuum wave (t = 1) of every slice, FP64, from the probe of
@@ -487,6 +496,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, pl.femv.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
         HIPCHK(c, pl.vsum.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
     }
+    if (want_corr && pl.csum.bytes < ncorr * sizeof(double)) {
+        HIPCHK(c, pl.csum.alloc(ncorr * sizeof(double)));
+        HIPCHK(c, pl.csqr.alloc(ncorr * sizeof(double)));
+    }
     if (want_pix && pl.pm1.bytes < (size_t)plane * sizeof(double)) {
         HIPCHK(c, pl.pm1.alloc((size_t)plane * sizeof(double)));
         HIPCHK(c, pl.pm2.alloc((size_t)plane * sizeof(double)));
@@ -495,6 +508,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     HIPCHK(c, hipMemsetAsync(pl.state.p, 0, (size_t)std::max(nbins, 1) * 3 * sizeof(double), c->stream));
     if (want_d2)
         HIPCHK(c, hipMemsetAsync(pl.vsum.p, 0, (size_t)std::max(nbins, 1) * sizeof(double), c->stream));
+    if (want_corr) {
+        HIPCHK(c, hipMemsetAsync(pl.csum.p, 0, ncorr * sizeof(double), c->stream));
+        HIPCHK(c, hipMemsetAsync(pl.csqr.p, 0, ncorr * sizeof(double), c->stream));
+    }
     if (want_pix) {
         HIPCHK(c, hipMemsetAsync(pl.pm1.p, 0, (size_t)plane * sizeof(double), c->stream));
         HIPCHK(c, hipMemsetAsync(pl.pm2.p, 0, (size_t)plane * sizeof(double), c->stream));
@@ -594,6 +611,11 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             HIPCHK(c, msf::launch_fem_vbar(dpart.as<double>(), G, nb, nbins, pl.hg.fem_abs,
                                            pl.hg.fem_sqr, dcnt.as<int32_t>(), dzero.as<uint8_t>(),
                                            pl.vsum.as<double>(), c->stream));
+        if (want_corr)
+            HIPCHK(c, msf::launch_fem_correlograph(dpart.as<double>(), G, nb, nbins, nphi,
+                                                   pl.hg.fem_abs, pl.hg.corr, dcnt.as<int32_t>(),
+                                                   dzero.as<uint8_t>(), pl.csum.as<double>(),
+                                                   pl.csqr.as<double>(), c->stream));
         if (want_pix)
             HIPCHK(c, msf::launch_pixel_moments(dpsi.as<float2>(), plane, nb, pl.pm1.as<double>(),
                                                 pl.pm2.as<double>(), c->stream));
@@ -618,7 +640,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     if (!prof && !out.exit && !c->cfg.no_graph) {
         const std::vector<uint64_t> sig = {
             sp->id, (uint64_t)P, (uint64_t)B, (uint64_t)tmode, (uint64_t)want_adf,
-            (uint64_t)want_d1 | (uint64_t)want_d2 << 1 | (uint64_t)want_pix << 2,
+            (uint64_t)want_d1 | (uint64_t)want_d2 << 1 | (uint64_t)want_pix << 2 |
+                (uint64_t)want_corr << 3,
+            (uint64_t)(uintptr_t)pl.csum.p, (uint64_t)(uintptr_t)pl.csqr.p, (uint64_t)nphi,
+            (uint64_t)pl.hg.corr,
             (uint64_t)(uintptr_t)pl.vsum.p, (uint64_t)(uintptr_t)pl.pm1.p,
             (uint64_t)(uintptr_t)pl.pm2.p, (uint64_t)pl.hg.fem_abs, (uint64_t)pl.hg.fem_sqr,
             (uint64_t)plane, (uint64_t)(uintptr_t)dpsi.p, (uint64_t)(uintptr_t)dpart.p,
@@ -671,6 +696,15 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, hipMemcpy(s1.data(), pl.pm1.p, (size_t)plane * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(s2.data(), pl.pm2.p, (size_t)plane * sizeof(double), hipMemcpyDeviceToHost));
         fem_pixel_modes(d, pl.hg, s1, s2, P, o->alpha_max, out);
+    }
+    if (want_corr && nbins * nphi > 0) { // adfstem.cpp:3620-3632: the mean over positions
+        const size_t n = (size_t)nbins * nphi;
+        std::vector<double> cs(n);
+        HIPCHK(c, hipMemcpy(cs.data(), pl.csum.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (out.corr_avg)
+            for (size_t k = 0; k < n; ++k) out.corr_avg[k] = cs[k] / (double)P;
+        if (out.corr_sqr)
+            HIPCHK(c, hipMemcpy(out.corr_sqr, pl.csqr.p, n * sizeof(double), hipMemcpyDeviceToHost));
     }
 
     msf_stats st{};
@@ -1132,6 +1166,25 @@ int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lowe
     return MSF_OK;
 }
 
+int msf_correlograph_bins(const msf_grid* grid, double dr, uint32_t* n_phi, double* k_edges,
+                          double* phi_edges)
+{
+    if (!grid || !n_phi || !(dr > 0.0) || grid->nx < 2 || grid->ny < 2) return MSF_ERR_INVALID;
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    double dk = 0.0;
+    const int nb = msf::fem_bin_count(d, dr, dk);
+    const int np = msf::correlograph_nphi(dk, nb);
+    *n_phi = (uint32_t)np;
+    if (k_edges)
+        for (unsigned int ii = 0; ii < (unsigned int)nb + 1; ++ii) k_edges[ii] = ii * dk;
+    if (phi_edges && np > 0) {
+        const double dphi = 2 * msf::kRefPi / np;
+        for (unsigned int ii = 0; ii < (unsigned int)np + 1; ++ii) phi_edges[ii] = ii * dphi;
+    }
+    return MSF_OK;
+}
+
 int msf_run_stem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const msf_detector* det,
                  const msf_fem* fem, const double* x_p, uint32_t n_xp, const double* y_p,
                  uint32_t n_yp, double* adf_image, double* fem_v)
@@ -1175,6 +1228,8 @@ int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         out.omega_vimage = fo->omega_vimage;
         out.v_gt17 = fo->v_gt17;
         out.v_rbar = fo->v_rbar;
+        out.corr_avg = fo->correlograph_avg;
+        out.corr_sqr = fo->correlograph_sqr_sum;
     }
     return run_internal(c, sp, o, det, fem, shifts, out);
 }

@@ -166,8 +166,19 @@ int fem_bin_count(const Domain& d, double dr, double& delta_k)
     return (int)nb;
 }
 
+int correlograph_nphi(double delta_k, int nbins)
+{
+    unsigned int count = 0;
+    for (unsigned int ii = 0; ii < (unsigned int)nbins + 1; ++ii) {
+        const double k_bdy = ii * delta_k;
+        if ((k_bdy >= 0.2) && (k_bdy <= 1.0)) ++count;
+    }
+    return count > 0 ? (int)count - 1 : 0;
+}
+
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
-                    double k2, bool fem, double dr, double alpha_max, bool fem_sqr)
+                    double k2, bool fem, double dr, double alpha_max, bool fem_sqr,
+                    bool correlograph)
 {
     Groups g;
     const int nx = d.nx, ny = d.ny;
@@ -224,6 +235,33 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
             g.zeroed.push_back(d.lambda * upper <= alpha_max ? 1 : 0);
             g.offsets.push_back((int32_t)g.pixels.size());
             g.kind.push_back(1);
+        }
+        if (correlograph) {
+            // ring ii, sector jj: phi = PI + atan2(ky, kx) in (jj dphi, (jj+1) dphi]
+            // (adfstem.cpp:1013-1027, variance.cpp:226-250)
+            g.nphi = correlograph_nphi(dk, g.nbins);
+            const double dphi = g.nphi > 0 ? 2 * kRefPi / g.nphi : 0.0;
+            std::vector<std::vector<int32_t>> sec((size_t)g.nbins * std::max(g.nphi, 0));
+            for (int ii = 0; ii < g.nbins && g.nphi > 0; ++ii)
+                for (int32_t k = g.bin_off[ii]; k < g.bin_off[ii + 1]; ++k) {
+                    if (g.bin_compact[k] < 0) continue;
+                    const int i = g.bin_full[k] / ny, j = g.bin_full[k] % ny;
+                    const double phi = kRefPi + std::atan2(d.ky[j], d.kx[i]);
+                    double lower = 0 * dphi;
+                    for (int jj = 0; jj < g.nphi; ++jj) {
+                        if ((phi > lower) && (phi <= (unsigned int)(jj + 1) * dphi)) {
+                            sec[(size_t)ii * g.nphi + jj].push_back(g.bin_compact[k]);
+                            break;
+                        }
+                        lower = (unsigned int)jj * dphi;
+                    }
+                }
+            g.corr = (int)g.kind.size();
+            for (const std::vector<int32_t>& v : sec) {
+                g.pixels.insert(g.pixels.end(), v.begin(), v.end());
+                g.offsets.push_back((int32_t)g.pixels.size());
+                g.kind.push_back(1);
+            }
         }
         if (fem_sqr) { // the same rings summing |psi|^2 (integrate_out_phi_double, d2)
             g.fem_sqr = (int)g.kind.size();

@@ -106,6 +106,8 @@ struct Groups {
     int nbins = 0;
     int fem_abs = -1;             // first group of the FEM |psi| rings (-1: none)
     int fem_sqr = -1;             // first group of the FEM |psi|^2 rings (-1: none)
+    int corr = -1;                // first correlograph sector group, [ii * nphi + jj] (-1: none)
+    int nphi = 0;                 // correlograph phi sectors per ring
     // every pixel of every FEM bin in the bins' sorted order (bin ii owns
     // [bin_off[ii], bin_off[ii+1])): full-grid index i*ny + j and compact
     // offset, -1 where the slice step's band limit leaves the pixel zero
@@ -114,7 +116,10 @@ struct Groups {
 // group 0: the ADF ring; then, with fem, nbins |psi| rings and, with
 // fem_sqr, nbins |psi|^2 rings over the same pixels
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
-                    double k2, bool fem, double dr, double alpha_max, bool fem_sqr = false);
+                    double k2, bool fem, double dr, double alpha_max, bool fem_sqr = false,
+                    bool correlograph = false);
+// adfstem.cpp:969-985: the correlograph's phi sector count for nbins rings of width delta_k
+int correlograph_nphi(double delta_k, int nbins);
 
 // Groups cut into segments of at most `seg` pixels, 64 segments per wavefront
 // tile, pixel offsets interleaved [tile][k][lane] (-1 pads short segments) so

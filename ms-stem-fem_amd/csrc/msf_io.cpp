@@ -486,6 +486,26 @@ int msf_write_variance_txt(const char* path, const double* values, const double*
     return MSF_OK;
 }
 
+int msf_write_correlograph_txt(const char* path, const double* data, const double* k_edges,
+                               uint32_t n_k, const double* phi_edges, uint32_t n_phi)
+{
+    if (!path || (n_k && !k_edges) || (n_phi && !phi_edges) || (n_k && n_phi && !data))
+        return io_fail(MSF_ERR_INVALID, "NULL argument");
+    FILE* fo = std::fopen(path, "w");
+    if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Unable to write variance to file: ") + path);
+    std::fputs("0 ", fo); // the (0,0) corner, not data
+    for (uint32_t jj = 0; jj < n_phi; ++jj) std::fprintf(fo, "%.10e ", phi_edges[jj]);
+    std::fputc('\n', fo);
+    for (uint32_t ii = 0; ii < n_k; ++ii) {
+        std::fprintf(fo, "%.10e ", k_edges[ii]);
+        for (uint32_t jj = 0; jj < n_phi; ++jj)
+            std::fprintf(fo, "%.10e ", data[(size_t)ii + (size_t)jj * n_k]);
+        std::fputc('\n', fo);
+    }
+    if (std::fclose(fo) != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
+    return MSF_OK;
+}
+
 int msf_write_f64(const char* path, const double* data, uint32_t rows, uint32_t cols)
 {
     if (!path || (rows && cols && !data)) return io_fail(MSF_ERR_INVALID, "NULL argument");

@@ -64,7 +64,8 @@ FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
 
 
 class FemOutputs(C.Structure):
-    _fields_ = [(m, C.c_void_p) for m in FEM_MODES]
+    _fields_ = [(m, C.c_void_p) for m in FEM_MODES] + [("correlograph_avg", C.c_void_p),
+                                                       ("correlograph_sqr_sum", C.c_void_p)]
 
 
 class Job(C.Structure):
@@ -104,7 +105,7 @@ class Stats(C.Structure):
 
 IO_EXPORTS = ["msf_io_last_error", "msf_element_z", "msf_guess_z", "msf_read_xyz",
               "msf_read_lammps_data", "msf_model_free", "msf_write_tiff16",
-              "msf_write_variance_txt", "msf_write_f64"]
+              "msf_write_variance_txt", "msf_write_f64", "msf_write_correlograph_txt"]
 
 EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf_get_stats",
            "msf_supported_length", "msf_wavelength", "msf_scherzer_defocus",
@@ -113,7 +114,7 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves",
            "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem",
            "msf_submit", "msf_poll", "msf_wait", "msf_specimen_propagator", "msf_probe_k",
-           "msf_slice_step_batch"]
+           "msf_slice_step_batch", "msf_correlograph_bins"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -149,6 +150,8 @@ def lib():
         L.msf_specimen_transmission.argtypes = [vp, C.c_uint32, _dp]
         L.msf_probe_realspace.argtypes = [vp, C.POINTER(Grid), C.POINTER(Optics), _dp]
         L.msf_fem_bins.argtypes = [C.POINTER(Grid), C.c_double, C.POINTER(C.c_uint32), _dp]
+        L.msf_correlograph_bins.argtypes = [C.POINTER(Grid), C.c_double, C.POINTER(C.c_uint32),
+                                            _dp, _dp]
         L.msf_run_stem.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(Detector),
                                    C.POINTER(Fem), _dp, C.c_uint32, _dp, C.c_uint32, _dp, _dp]
         L.msf_exit_waves.argtypes = [vp, vp, C.POINTER(Optics), _dp, C.c_uint32, _dp,
@@ -182,6 +185,8 @@ def lib():
                                        C.c_double, C.c_int, C.c_uint32, C.c_double, C.c_double]
         L.msf_write_variance_txt.argtypes = [C.c_char_p, _dp, _dp, C.c_uint32]
         L.msf_write_f64.argtypes = [C.c_char_p, _dp, C.c_uint32, C.c_uint32]
+        L.msf_write_correlograph_txt.argtypes = [C.c_char_p, _dp, _dp, C.c_uint32, _dp,
+                                                 C.c_uint32]
         _lib = L
     return _lib
 
@@ -296,6 +301,31 @@ def fem_bins(grid, dr):
     return edges[:n.value]
 
 
+def correlograph_bins(grid, dr):
+    """msf_correlograph_bins: (n_phi, k boundaries [n_bins + 1], phi boundaries [n_phi + 1])."""
+    n = C.c_uint32()
+    nb = fem_bins(grid, dr).size
+    k = np.zeros(nb + 1)
+    rc = lib().msf_correlograph_bins(C.byref(grid), dr, C.byref(n), _p(k), None)
+    if rc:
+        raise MsfError(rc, "msf_correlograph_bins")
+    phi = np.zeros(n.value + 1)
+    lib().msf_correlograph_bins(C.byref(grid), dr, C.byref(n), _p(k), _p(phi))
+    return n.value, k, phi
+
+
+def write_correlograph_txt(path, data, k_edges, phi_edges):
+    """msf_write_correlograph_txt; data [n_phi, n_k] (k fastest)."""
+    data = np.ascontiguousarray(data, np.float64)
+    n_phi, n_k = data.shape
+    k = np.ascontiguousarray(k_edges, np.float64)
+    phi = np.ascontiguousarray(phi_edges, np.float64)
+    assert k.size >= n_k and phi.size >= n_phi
+    rc = lib().msf_write_correlograph_txt(str(path).encode(), _p(data), _p(k), n_k, _p(phi), n_phi)
+    if rc:
+        raise MsfError(rc, lib().msf_io_last_error().decode())
+
+
 class Context:
     """One device context (msf_create); one per host thread."""
 
@@ -359,21 +389,35 @@ class Context:
             v = v[:fem_bins(spec.grid, fm.dr).size]
         return img, v
 
-    def adfstem_fem(self, spec, optics, x_p, y_p, fem, modes=FEM_MODES, detector=None):
-        """msf_run_stem_fem: returns (adf image or None, {mode: values [n_bins]})."""
+    def adfstem_fem(self, spec, optics, x_p, y_p, fem, modes=FEM_MODES, detector=None,
+                    correlograph=False):
+        """msf_run_stem_fem: returns (adf image or None, {mode: values [n_bins]});
+        with correlograph, also "correlograph_avg" and "correlograph_sqr_sum"
+        shaped [n_phi, n_bins]."""
         x_p = np.ascontiguousarray(x_p, np.float64)
         y_p = np.ascontiguousarray(y_p, np.float64)
         det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
         img = np.zeros(x_p.size * y_p.size) if det.enabled else None
         nb = fem_bins(spec.grid, fem.dr).size
         outs = {m: np.zeros(max(nb, 1)) for m in modes}
-        fo = FemOutputs(*[outs[m].ctypes.data if m in outs else None for m in FEM_MODES])
+        ptrs = [outs[m].ctypes.data if m in outs else None for m in FEM_MODES]
+        nphi = 0
+        if correlograph:
+            nphi = correlograph_bins(spec.grid, fem.dr)[0]
+            for m in ("correlograph_avg", "correlograph_sqr_sum"):
+                outs[m] = np.zeros(max(nb * nphi, 1))
+                ptrs.append(outs[m].ctypes.data)
+        fo = FemOutputs(*ptrs)
         self._check(lib().msf_run_stem_fem(self._h, spec._h, C.byref(optics), C.byref(det),
                                            C.byref(fem), _p(x_p), x_p.size, _p(y_p), y_p.size,
                                            _p(img), C.byref(fo)), "msf_run_stem_fem")
         if img is not None:
             img = img.reshape(x_p.size, y_p.size)
-        return img, {m: v[:nb] for m, v in outs.items()}
+        res = {m: v[:nb] for m, v in outs.items() if m in FEM_MODES}
+        if correlograph:
+            for m in ("correlograph_avg", "correlograph_sqr_sum"):
+                res[m] = outs[m][:nb * nphi].reshape(nphi, nb)
+        return img, res
 
     def submit(self, grid, q, Z, min_slice, optics, x_p, y_p, detector=None, fem=None,
                modes=("v_omega",)):

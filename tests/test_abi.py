@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol(msf):
         assert hasattr(lib, n), n
     assert sorted(msf.EXPORTS) == names
     io = _declared("msf_io.h")
-    assert len(io) == 9
+    assert len(io) == 10
     for n in io:
         assert hasattr(lib, n), n
     assert sorted(msf.IO_EXPORTS) == io
@@ -89,3 +89,21 @@ def test_probe_shifts_match_oracle(msf, O):
     with pytest.raises(msf.MsfError) as e:
         msf.probe_shifts(g, np.array([-0.5]), np.array([1.0]))
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("n, L, dr", [(128, 12.0, 0.5), (256, 20.0, 0.70710678118654746),
+                                      (64, 40.0, 1.0), (512, 51.2, 0.25)])
+def test_correlograph_bins_match_oracle(msf, O, n, L, dr):
+    """n_phi = (# of boundaries ii*dk in [0.2, 1.0]) - 1 (adfstem.cpp:969-985),
+    counted here independently; boundaries ii*dk and ii*2PI/n_phi with the
+    reference's PI."""
+    g = msf.make_grid(n, n, L, L, 5.0, 200e3)
+    nb = msf.fem_bins(g, dr).size
+    nphi, k, phi = msf.correlograph_bins(g, dr)
+    dk = k[1]
+    assert k.size == nb + 1
+    np.testing.assert_array_equal(k, np.array([ii * dk for ii in range(nb + 1)]))
+    want = sum(1 for ii in range(nb + 1) if 0.2 <= ii * dk <= 1.0) - 1
+    assert nphi == max(want, 0) == O.lib().orc_correlograph_nphi(dk, nb)
+    if nphi:
+        np.testing.assert_array_equal(phi, np.arange(nphi + 1) * (2 * 3.14159265369 / nphi))

@@ -113,7 +113,7 @@ def test_dupe_replicates_for_fem_only(tmp_path, model):
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
       "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--bogus"], "unknown flag"),
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorrfem",
-      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--correlograph"],
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--correlograph_everytxt"],
      "mode outside the build"),
     (["-m", "64", "64", "3e5", "-a", "missing.xyz", "--output_prefix", "o", "--adfstemuncorr",
       "--cs3", "0", "--defocus", "0", "--alphamax", "0.01"], "unreadable model"),
@@ -153,3 +153,22 @@ def test_fem_mode_selection(tmp_path, model):
     # the mode flags do nothing without an FEM run mode
     rc, plan, _ = run(base + ["--adfstemuncorr", "--Vbar_r"], tmp_path)
     assert plan["fem_modes"][0].split() == []
+
+
+def test_correlograph_flags(tmp_path, model):
+    base = ["-m", "64", "64", "300000", "-a", model, "--output_prefix", "o", "--cs3", "0",
+            "--defocus", "0", "--alphamax", "0.01", "--plan_only", "--adfstemuncorrfem"]
+    # naming only the correlograph drops the V_omega default (tem_io.cpp:5705-5712)
+    rc, plan, out = run(base + ["--correlograph"], tmp_path)
+    assert rc == 0, out
+    assert plan["fem_modes"][0].split() == ["correlograph"]
+    # the variance flag implies the correlograph (tem_io.cpp:5646-5650)
+    rc, plan, _ = run(base + ["--correlograph_variance", "--V_re"], tmp_path)
+    assert plan["fem_modes"][0].split() == ["V_re", "correlograph", "correlograph_variance"]
+    pf = tmp_path / "p.txt"
+    pf.write_text("correlograph_variance\n")
+    rc, plan, _ = run(["--parameter_file", str(pf)] + base, tmp_path)
+    assert plan["fem_modes"][0].split() == ["correlograph", "correlograph_variance"]
+    for flag in ("--correlograph_everyimage", "--correlograph_everytxt"):
+        rc, _, out = run(base + [flag], tmp_path)
+        assert rc == 1 and "not supported" in out, flag

@@ -90,3 +90,66 @@ def test_cli_adfstem_fem_run(tmp_path, O):
         want = np.floor((m - m.min()) * (65535.0 / (m.max() - m.min())) + 0.5)
         bad = np.abs(d - want) > 2
         assert bad.mean() < 1e-3, (s, tag, bad.sum())
+
+
+def test_cli_correlograph_run(tmp_path, O):
+    """--correlograph_variance: the average and sums-of-squares text files
+    against the oracle (1e-4 relative L2) and the images derived from them
+    (adfstem.cpp:3617-3745)."""
+    rng = np.random.default_rng(31)
+    n_at = 45
+    q = rng.random((n_at, 3)) * [10.0, 10.0, 6.0]
+    names = rng.choice(["Si", "Cu"], n_at)
+    lines = [f"{n_at}", "10 10 6"] + [f"{e} {x:.6f} {y:.6f} {z:.6f}" for e, (x, y, z) in zip(names, q)]
+    (tmp_path / "m.xyz").write_text("\n".join(lines) + "\n")
+    # alphamax 0.01 at 300 kV leaves the rings beyond 0.51 1/A outside the
+    # zeroed direct beam, so the variance over k in [0.2, 1.0] is populated
+    args = [CLI, "-m", "128", "128", "300000", "-a", "m.xyz", "--output_prefix", "run",
+            "--adfstemuncorrfem", "--cs3", "1.0e6", "--defocus", "25", "--alphamax", "0.01",
+            "--rasterspacing", "2.5", "--minslice", "1.5", "--dr", "0.5",
+            "--correlograph_variance"]
+    print("cli:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert not (tmp_path / "run_adfstem_uncorr_fem_V_omega.txt").exists()  # not the default here
+
+    qq = np.array([[float(v) for v in l.split()[1:]] for l in lines[2:]])
+    Z = np.array([14 if e == "Si" else 29 for e in names], np.uint32)
+    c = Case(nx=128, ny=128, cell=(10.0, 10.0, 6.0), voltage=300e3, q=qq, Z=Z, min_slice=1.5,
+             cs3=1.0e6, defocus=25.0, alpha_max=0.01, fem=True, dr=0.5)
+    xp = np.arange(4) * 2.5
+    job = c.oracle_job(O)
+    _, mo = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, xp,
+                          modes=("v_omega",), correlograph=True)
+
+    pre = tmp_path / "run_adfstem_uncorr"
+    nphi, nk = mo["correlograph_avg"].shape
+    avg = np.loadtxt(f"{pre}_correlograph_avg.txt")
+    sqr = np.loadtxt(f"{pre}_correlograph_variance.txt")
+    assert avg.shape == sqr.shape == (nk + 1, nphi + 1)
+    assert avg[0, 0] == 0 and np.array_equal(avg[:, 0], sqr[:, 0])
+    np.testing.assert_allclose(avg[0, 1:], np.arange(nphi) * 2 * 3.14159265369 / nphi, rtol=1e-9)
+    dk = avg[2, 0]
+    np.testing.assert_allclose(avg[1:, 0], np.arange(nk) * dk, rtol=1e-9)
+    a, s = avg[1:, 1:].T, sqr[1:, 1:].T       # [phi, k]
+    assert rel_l2(a, mo["correlograph_avg"]) < 1e-4
+    assert rel_l2(s, mo["correlograph_sqr_sum"]) < 1e-4
+
+    # images: phi down the rows, k across; the variance over the k bins from
+    # the one holding 0.2 up to the one holding 1.0 (excluded)
+    ca = _tiff(f"{pre}_correlograph_avg.tif").astype(float)
+    want = np.floor((a - a.min()) * (65535.0 / (a.max() - a.min())) + 0.5)
+    assert ca.shape == (nphi, nk) and np.abs(ca - want).max() <= 1
+    kb = np.arange(nk + 1) * dk
+    s0 = [i for i in range(nk) if kb[i] <= 0.2 < kb[i + 1]][0]
+    s1 = [i for i in range(nk) if kb[i] <= 1.0 < kb[i + 1]][0]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = s[:, s0:s1] / (16 * a[:, s0:s1] ** 2)
+    var = np.where((s[:, s0:s1] != 0) & (a[:, s0:s1] != 0) & (t >= 1), t - 1, 0.0)
+    assert var.max() > 0.1
+    cv = _tiff(f"{pre}_correlograph_variance.tif").astype(float)
+    want = np.floor((var - var.min()) * (65535.0 / (var.max() - var.min())) + 0.5)
+    assert cv.shape == var.shape and np.abs(cv - want).max() <= 2
+    cl = _tiff(f"{pre}_correlograph_variance_logscale.tif").astype(float)
+    tl = np.log(var - var.min() + 1) / np.log(var.max() - var.min() + 1)
+    assert np.abs(cl - np.where(tl > 0, np.floor(tl * 65535.0 + 0.5), 0)).max() <= 2

@@ -71,3 +71,38 @@ def test_fem_mode_subsets(msf, ctx):
         _, part = ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf), modes=sub)
         for m in sub:
             np.testing.assert_array_equal(part[m], full[m], err_msg=m)
+
+
+def test_correlograph_matches_oracle(msf, ctx, O):
+    """The correlograph (adfstem.cpp:2603-2650, :3620-3632): mean and sum of
+    squares over positions of the per-position sector ratios, [n_phi, n_bins],
+    against the oracle at 1e-4 relative L2; identical for any batch depth."""
+    c = _case()
+    xp = np.linspace(0.3, 11.2, 4)
+    yp = np.linspace(0.8, 11.6, 5)
+    job = c.oracle_job(O)
+    _, o = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, yp,
+                         modes=("v_omega",), correlograph=True)
+    spec = c.gpu_specimen(msf, ctx)
+    nphi, k_edges, phi_edges = msf.correlograph_bins(spec.grid, c.dr)
+    nb = o["v_omega"].size
+    assert nphi == O.lib().orc_correlograph_nphi(k_edges[1], nb) and nphi > 4
+    print("correlograph", nb, nphi, flush=True)
+    _, g = ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf), modes=("v_omega",),
+                           correlograph=True)
+    for m in ("correlograph_avg", "correlograph_sqr_sum"):
+        assert g[m].shape == o[m].shape == (nphi, nb), m
+        assert (o[m] != 0).sum() > nb, m          # a populated correlograph
+        err = rel_l2(g[m], o[m])
+        print(m, err, flush=True)
+        assert err < 1e-4, (m, err)
+    np.testing.assert_array_equal(g["v_omega"], ctx.adfstem_fem(
+        spec, c.optics(msf), xp, yp, c.femcfg(msf), modes=("v_omega",))[1]["v_omega"])
+    try:
+        ctx.configure(batch=7, graph=False)
+        _, g7 = ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf), modes=(),
+                                correlograph=True)
+    finally:
+        ctx.configure()
+    for m in ("correlograph_avg", "correlograph_sqr_sum"):
+        np.testing.assert_array_equal(g7[m], g[m], err_msg=m)

@@ -223,3 +223,20 @@ def test_f64_sidecar_roundtrip(tmp_path):
     head, body = raw.split(b"\n", 1)
     assert head == b"msf-f64 3 4"
     np.testing.assert_array_equal(np.frombuffer(body, "<f8").reshape(3, 4), a)
+
+
+def test_correlograph_txt_format(tmp_path):
+    """io_txt.cpp:96-165: "0 " and the phi boundaries, then per k bin its
+    boundary and the values data[ii + jj * n_k]; every value followed by a
+    space."""
+    p = str(tmp_path / "c.txt")
+    data = np.array([[1.0, 2.0, 3.0], [-4.5, 0.0, 6.25e-7]])  # [n_phi = 2, n_k = 3]
+    msf.write_correlograph_txt(p, data, np.arange(4) * 0.25, np.arange(3) * np.pi)
+    text = open(p).read()
+    assert text == ("0 0.0000000000e+00 3.1415926536e+00 \n"
+                    "0.0000000000e+00 1.0000000000e+00 -4.5000000000e+00 \n"
+                    "2.5000000000e-01 2.0000000000e+00 0.0000000000e+00 \n"
+                    "5.0000000000e-01 3.0000000000e+00 6.2500000000e-07 \n")
+    # no phi sectors: the corner and one boundary per k bin
+    msf.write_correlograph_txt(p, np.zeros((0, 2)), [0.0, 0.5, 1.0], [])
+    assert open(p).read() == "0 \n0.0000000000e+00 \n5.0000000000e-01 \n"
