@@ -165,6 +165,25 @@ int msf_probe_k(msf_ctx* ctx, const msf_grid* grid, const msf_optics* optics, in
 int msf_slice_step_batch(msf_ctx* ctx, const msf_specimen* spec, uint32_t slice, double* psi_k,
                          uint32_t count);
 
+/* Parity entry point for the ADF ring sum alone (func.cpp:2593-2641, called at
+ * adfstem.cpp:2245-2267): adf[p] = sum of |psi_p|^2 over the pixels with
+ * k^2 in (k1, k2] (k1 = inner/lambda, k2 = outer/lambda, squared with
+ * MSF_ADF_PHYSICAL), FP64 on the device, for count <= 65535 k-space waves
+ * psi_k (count*nx*ny complex doubles, reference layout). */
+int msf_adf_reduce(msf_ctx* ctx, const msf_grid* grid, const msf_detector* det,
+                   const double* psi_k, uint32_t count, double* adf);
+
+/* Parity entry point for the FEM ring sums and their accumulation
+ * (variance.cpp:64-300 integrate_out_phi_fftw; adfstem.cpp:2518-2522
+ * direct-beam zeroing, :2591-2599 accumulation, :2837-2856 V_omega), FP64 on
+ * the device, on count <= 65535 k-space waves taken as the positions of one
+ * raster in order.  radial: count*n_bins ring sums of |psi| (zero on
+ * direct-beam bins); counts: n_bins ring pixel counts; v_omega: n_bins values
+ * (msf_fem_bins).  Each output may be NULL. */
+int msf_fem_accumulate(msf_ctx* ctx, const msf_grid* grid, const msf_fem* fem, double alpha_max,
+                       const double* psi_k, uint32_t count, double* radial, int32_t* counts,
+                       double* v_omega);
+
 /* ------------------------------------------------------------ run */
 /* Number of V_omega bins (adfstem.cpp:947-972) and their lower edges
  * (n_bins doubles, may be NULL). */

@@ -87,6 +87,10 @@ hipError_t launch_fem_finalize(const double* state, int nbins, double* v, hipStr
 hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbins, int off_abs,
                            int off_sqr, const int32_t* counts, const uint8_t* zeroed,
                            double* vsum, hipStream_t s);
+// FP64 sums over pixel groups of count full-grid FP64 waves: out[count][groups]
+hipError_t launch_group_sums_f64(const double2* psi, int64_t plane, const int32_t* offsets,
+                                 const int32_t* pixels, const int32_t* kind, int groups,
+                                 int count, double* out, hipStream_t s);
 // correlograph sums of c and c^2 per (ring, phi sector) over a batch
 hipError_t launch_fem_correlograph(const double* partial, int groups, int batch, int nbins,
                                    int nphi, int off_abs, int off_corr, const int32_t* counts,

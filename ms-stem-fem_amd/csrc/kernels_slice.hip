@@ -663,6 +663,45 @@ hipError_t launch_fem_correlograph(const double* partial, int groups, int batch,
     return hipGetLastError();
 }
 
+// FP64 group sums on full-grid FP64 waves (parity entry points): one
+// workgroup per (group, wave), strided pixel loop and a fixed-order tree, so
+// results do not depend on scheduling.  kind 0 sums |psi|^2, kind 1 |psi|.
+__global__ void __launch_bounds__(256) group_sums_f64_kernel(const double2* psi, int64_t plane,
+                                                             const int32_t* offsets,
+                                                             const int32_t* pixels,
+                                                             const int32_t* kind, int groups,
+                                                             double* out)
+{
+    __shared__ double red[256];
+    const int g = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
+    const double2* w = psi + (int64_t)b * plane;
+    const bool amp = kind[g] == 1;
+    double acc = 0.0;
+    for (int k = offsets[g] + t; k < offsets[g + 1]; k += 256) {
+        const double2 v = w[pixels[k]];
+        const double i2 = v.x * v.x + v.y * v.y;
+        acc += amp ? sqrt(i2) : i2;
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[(int64_t)b * groups + g] = red[0];
+}
+
+hipError_t launch_group_sums_f64(const double2* psi, int64_t plane, const int32_t* offsets,
+                                 const int32_t* pixels, const int32_t* kind, int groups,
+                                 int count, double* out, hipStream_t s)
+{
+    if (groups <= 0 || count <= 0) return hipSuccess;
+    if (groups > 65535 || count > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_sums_f64_kernel, dim3((unsigned)groups, (unsigned)count), dim3(256),
+                       0, s, psi, plane, offsets, pixels, kind, groups, out);
+    return hipGetLastError();
+}
+
 // Per-pixel sums over positions of |psi| and |psi|^2 in raster order, FP64
 // (adfstem.cpp:2753-2781), on the compact plane of the exit waves.
 __global__ void pixel_moments_kernel(const float2* psi, int64_t plane, int batch, double* s1,

@@ -1166,6 +1166,114 @@ int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lowe
     return MSF_OK;
 }
 
+namespace {
+
+// Every pixel of the grid kept: the full [nx][ny] layout as its own compact plane.
+msf::RowRanges full_ranges(int nx, int ny)
+{
+    msf::RowRanges r;
+    r.ab.assign(2 * (size_t)nx, 0);
+    for (int i = 0; i < nx; ++i) r.ab[2 * i] = ny;
+    r.amax = ny;
+    r.bmax = 0;
+    return r;
+}
+
+// Group sums of `count` FP64 k-space waves on the device: out[count][G]
+int group_sums_host(msf_ctx* c, const msf::Groups& g, const msf::Domain& d, const double* psi_k,
+                    uint32_t count, std::vector<double>& out)
+{
+    const int G = (int)g.kind.size();
+    const size_t M = (size_t)d.nx * d.ny;
+    DevBuf w, off, pix, kind, part;
+    HIPCHK(c, w.alloc(M * count * sizeof(double2)));
+    HIPCHK(c, hipMemcpyAsync(w.p, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, upload(off, g.offsets, c->stream));
+    HIPCHK(c, upload(pix, g.pixels, c->stream));
+    HIPCHK(c, upload(kind, g.kind, c->stream));
+    HIPCHK(c, part.alloc((size_t)count * G * sizeof(double)));
+    HIPCHK(c, msf::launch_group_sums_f64(w.as<double2>(), (int64_t)M, off.as<int32_t>(),
+                                         pix.as<int32_t>(), kind.as<int32_t>(), G, (int)count,
+                                         part.as<double>(), c->stream));
+    out.resize((size_t)count * G);
+    HIPCHK(c, hipMemcpyAsync(out.data(), part.p, out.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MSF_OK;
+}
+
+} // namespace
+
+int msf_adf_reduce(msf_ctx* c, const msf_grid* grid, const msf_detector* det,
+                   const double* psi_k, uint32_t count, double* adf)
+{
+    if (!c) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
+    if (!det || !psi_k || !adf) return fail(c, MSF_ERR_INVALID, "NULL argument");
+    if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
+    if (int r = check_grid(c, grid)) return r;
+    if (det->inner < 0.0 || det->outer < 0.0)
+        return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
+    HIPCHK(c, hipSetDevice(c->device));
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    double k1 = det->inner / d.lambda, k2 = det->outer / d.lambda; // adfstem.cpp:2251-2252
+    if (det->threshold_mode == MSF_ADF_PHYSICAL) {
+        k1 = k1 * k1;
+        k2 = k2 * k2;
+    }
+    const msf::Groups g = msf::build_groups(d, full_ranges(d.nx, d.ny), d.ny, true, k1, k2,
+                                            false, 1.0, 0.0);
+    std::vector<double> part;
+    if (int r = group_sums_host(c, g, d, psi_k, count, part)) return r;
+    const size_t G = g.kind.size();
+    for (uint32_t p = 0; p < count; ++p) adf[p] = part[p * G];
+    return MSF_OK;
+}
+
+int msf_fem_accumulate(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, double alpha_max,
+                       const double* psi_k, uint32_t count, double* radial, int32_t* counts,
+                       double* v_omega)
+{
+    if (!c) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
+    if (!fem || !psi_k) return fail(c, MSF_ERR_INVALID, "NULL argument");
+    if (!(fem->dr > 0.0)) return fail(c, MSF_ERR_INVALID, "dr must be positive");
+    if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
+    if (int r = check_grid(c, grid)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    const msf::Groups g = msf::build_groups(d, full_ranges(d.nx, d.ny), d.ny, false, 0.0, 0.0,
+                                            true, fem->dr, alpha_max);
+    std::vector<double> part;
+    if (int r = group_sums_host(c, g, d, psi_k, count, part)) return r;
+    const int nb = g.nbins, G = (int)g.kind.size();
+    if (radial)
+        for (uint32_t p = 0; p < count; ++p)
+            for (int k = 0; k < nb; ++k)
+                radial[(size_t)p * nb + k] = g.zeroed[k] ? 0.0 : part[(size_t)p * G + g.fem_abs + k];
+    if (counts)
+        for (int k = 0; k < nb; ++k) counts[k] = g.counts[k];
+    if (v_omega && nb > 0) { // the production kernels on these sums (adfstem.cpp:2591-2599, :2837-2856)
+        DevBuf dp, dc, dz, st, v;
+        HIPCHK(c, upload(dp, part, c->stream));
+        HIPCHK(c, upload(dc, g.counts, c->stream));
+        HIPCHK(c, upload(dz, g.zeroed, c->stream));
+        HIPCHK(c, st.alloc((size_t)nb * 3 * sizeof(double)));
+        HIPCHK(c, v.alloc((size_t)nb * sizeof(double)));
+        HIPCHK(c, hipMemsetAsync(st.p, 0, (size_t)nb * 3 * sizeof(double), c->stream));
+        HIPCHK(c, msf::launch_fem_welford(dp.as<double>(), G, (int)count, nb, dc.as<int32_t>(),
+                                          dz.as<uint8_t>(), st.as<double>(), c->stream));
+        HIPCHK(c, msf::launch_fem_finalize(st.as<double>(), nb, v.as<double>(), c->stream));
+        HIPCHK(c, hipMemcpyAsync(v_omega, v.p, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return MSF_OK;
+}
+
 int msf_correlograph_bins(const msf_grid* grid, double dr, uint32_t* n_phi, double* k_edges,
                           double* phi_edges)
 {

@@ -114,7 +114,8 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_probe_realspace", "msf_fem_bins", "msf_run_stem", "msf_exit_waves",
            "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem",
            "msf_submit", "msf_poll", "msf_wait", "msf_specimen_propagator", "msf_probe_k",
-           "msf_slice_step_batch", "msf_correlograph_bins"]
+           "msf_slice_step_batch", "msf_correlograph_bins", "msf_adf_reduce",
+           "msf_fem_accumulate"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -167,6 +168,10 @@ def lib():
         L.msf_probe_k.argtypes = [vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int32, C.c_int32,
                                   _dp]
         L.msf_slice_step_batch.argtypes = [vp, vp, C.c_uint32, _dp, C.c_uint32]
+        L.msf_adf_reduce.argtypes = [vp, C.POINTER(Grid), C.POINTER(Detector), _dp, C.c_uint32,
+                                     _dp]
+        L.msf_fem_accumulate.argtypes = [vp, C.POINTER(Grid), C.POINTER(Fem), C.c_double, _dp,
+                                         C.c_uint32, _dp, C.POINTER(C.c_int32), _dp]
         L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
         L.msf_poll.argtypes = [vp, C.c_uint64]
         L.msf_wait.argtypes = [vp, C.c_uint64]
@@ -458,6 +463,29 @@ class Context:
         self._check(lib().msf_probe_k(self._h, C.byref(grid), C.byref(optics), sx, sy,
                                       _p(out.view(np.float64))), "msf_probe_k")
         return out
+
+    def adf_reduce(self, grid, detector, psi_k):
+        """msf_adf_reduce on k-space waves [count, nx, ny] (complex): ADF per wave."""
+        w = np.ascontiguousarray(psi_k, np.complex128).reshape(-1, grid.nx, grid.ny)
+        out = np.zeros(w.shape[0])
+        self._check(lib().msf_adf_reduce(self._h, C.byref(grid), C.byref(detector),
+                                         _p(w.view(np.float64)), w.shape[0], _p(out)),
+                    "msf_adf_reduce")
+        return out
+
+    def fem_accumulate(self, grid, fem, alpha_max, psi_k):
+        """msf_fem_accumulate on k-space waves [count, nx, ny] taken as one
+        raster: (ring sums [count, n_bins], counts [n_bins], V_omega [n_bins])."""
+        w = np.ascontiguousarray(psi_k, np.complex128).reshape(-1, grid.nx, grid.ny)
+        nb = fem_bins(grid, fem.dr).size
+        radial = np.zeros(max(w.shape[0] * nb, 1))
+        counts = np.zeros(max(nb, 1), np.int32)
+        v = np.zeros(max(nb, 1))
+        self._check(lib().msf_fem_accumulate(self._h, C.byref(grid), C.byref(fem), alpha_max,
+                                             _p(w.view(np.float64)), w.shape[0], _p(radial),
+                                             counts.ctypes.data_as(C.POINTER(C.c_int32)), _p(v)),
+                    "msf_fem_accumulate")
+        return radial[:w.shape[0] * nb].reshape(w.shape[0], nb), counts[:nb], v[:nb]
 
     def slice_step(self, spec, slice_index, psi_k):
         """msf_slice_step_batch on a copy of psi_k [count, nx, ny] (complex)."""

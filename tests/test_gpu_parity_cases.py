@@ -258,3 +258,42 @@ def test_single_step_entry_points_match_oracle(msf, ctx, O, mode):
             assert rel_l2(w_g, w_o) < 1e-10, n
     finally:
         ctx.configure()
+
+
+def test_reduction_entry_points_match_oracle(msf, ctx, O):
+    """msf_adf_reduce and msf_fem_accumulate (FP64 on the device) on random
+    k-space waves against orc_adf, orc_fem_radial and the V_omega sums of the
+    oracle, for both ADF threshold modes."""
+    rng = np.random.default_rng(17)
+    nx, ny, Lx, Ly = 64, 96, 9.0, 12.0
+    g = msf.make_grid(nx, ny, Lx, Ly, 6.0, 200e3)
+    w = rng.normal(size=(5, nx, ny)) + 1j * rng.normal(size=(5, nx, ny))
+    kx, ky, _, _ = O.domains(nx, ny, Lx, Ly)
+    lam = msf.wavelength(200e3)[0]
+    for mode in (msf.ADF_REFERENCE, msf.ADF_PHYSICAL):
+        det = msf.Detector(1, 0.03, 0.15, mode)
+        k1, k2 = 0.03 / lam, 0.15 / lam
+        if mode == msf.ADF_PHYSICAL:
+            k1, k2 = k1 * k1, k2 * k2
+        adf_g = ctx.adf_reduce(g, det, w)
+        adf_o = np.array([O.adf(wi, kx, ky, k1, k2) for wi in w])
+        assert (adf_o > 0).all()
+        np.testing.assert_allclose(adf_g, adf_o, rtol=1e-13)
+    fem = msf.Fem(1, 0.5)
+    alpha = 0.01
+    r_g, c_g, v_g = ctx.fem_accumulate(g, fem, alpha, w)
+    _, bwt = O.bwcutoffs(nx, ny, Lx, Ly)
+    nb, dk = O.fem_nbins(kx, ky, 0.5, bwt)
+    ro = [O.fem_radial(wi, kx, ky, dk, nb, lam, alpha) for wi in w]
+    r_o = np.stack([r for r, _ in ro])
+    assert r_g.shape == (5, nb)
+    np.testing.assert_array_equal(c_g, ro[0][1])
+    np.testing.assert_allclose(r_g, r_o, rtol=1e-13, atol=0)
+    assert (r_o[:, 0] == 0).all() and (r_o[:, -1] > 0).all()   # direct beam zeroed
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s1 = (r_o / c_g).sum(0)
+        s2 = (r_o ** 2 / (c_g * c_g)).sum(0)
+    v_o = O.fem_finalize(s1, s2, 5)
+    assert rel_l2(v_g, v_o) < 1e-9
+    with pytest.raises(msf.MsfError):
+        ctx.adf_reduce(g, msf.Detector(1, -0.1, 0.15, 0), w)
