@@ -839,6 +839,85 @@ int orc_correlograph(const double* mag2, long P, const double* kx, const double*
     return 0;
 }
 
+/* The detector MTF in two dimensions (adfstem.cpp:300-439).  The squared
+ * domain (0.5 rate d)^2 is formed as tem.cpp:276-285 does from the file's
+ * Nyquist units.  Per pixel: xxx = xperiod^2 kx / Nx, rsqr = xxx^2 + yyy^2;
+ * the last segment [d_k^2, d_k+1^2) holding rsqr interpolates the MTF
+ * linearly in sqrt(rsqr) and multiplies by sinc(xxx/rate) sinc(yyy/rate);
+ * pixels in no segment are 0.  Then BACKWARD FFT, bw_limit applied to that
+ * real-space array by k index (as written), FORWARD FFT, / NxNy. */
+int orc_mtf_2d(const double* mtf, const double* mtf_domain, int n, double rate, const double* kx,
+               const double* ky, int Nx, int Ny, double xperiod, double yperiod,
+               double bwcutoff_t, double* out)
+{
+    if (n < 1) return -1;
+    const size_t M = (size_t)Nx * Ny;
+    const double NxNy = (double)Nx * Ny;
+    double* dsq = (double*)malloc(sizeof(double) * n);
+    for (int k = 0; k < n; k++) dsq[k] = ref_pow2((0.5 * rate * mtf_domain[k]));
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const double xxx = ref_pow2(xperiod) * kx[i] / Nx;
+            const double yyy = ref_pow2(yperiod) * ky[j] / Ny;
+            const double rsqr = ref_pow2(xxx) + ref_pow2(yyy);
+            const size_t k = (size_t)j + (size_t)i * Ny;
+            out[2 * k] = 0;
+            out[2 * k + 1] = 0;
+            for (size_t idx = 0; idx < (size_t)n - 1; ++idx) {
+                if ((rsqr >= dsq[idx]) && (rsqr < dsq[idx + 1])) {
+                    const double domain1 = sqrt(dsq[idx]);
+                    const double domain2 = sqrt(dsq[idx + 1]);
+                    out[2 * k] = mtf[idx]
+                                 + (mtf[idx + 1] - mtf[idx]) * (sqrt(rsqr) - domain1)
+                                       / (domain2 - domain1);
+                    double sincarg = xxx / rate;
+                    const double sincx = (sincarg != 0) ? sin(sincarg) / sincarg : 1;
+                    sincarg = yyy / rate;
+                    const double sincy = (sincarg != 0) ? sin(sincarg) / sincarg : 1;
+                    out[2 * k] = out[2 * k] * sincx * sincy;
+                }
+            }
+        }
+    free(dsq);
+    if (orc_fft2d(out, Nx, Ny, +1)) return -1;
+    bw_limit(out, kx, Nx, ky, Ny, bwcutoff_t);
+    if (orc_fft2d(out, Nx, Ny, -1)) return -1;
+    for (size_t k = 0; k < 2 * M; k++) out[k] = out[k] / NxNy;
+    return 0;
+}
+
+/* adfstem.cpp:2272-2385: |psi| outside the aperture (lambda^2 k^2 >
+ * alpha_max^2, else 0; imaginary part 0), BACKWARD FFT, times the 2-D MTF /
+ * sqrt(NxNy) with the real part overwritten before the imaginary part is
+ * formed, FORWARD FFT, / sqrt(NxNy), bw_limit. */
+int orc_apply_mtf(double* psi, const double* mtf2d, const double* kx, const double* ky, int Nx,
+                  int Ny, double lambda, double alpha_max, double bwcutoff_t)
+{
+    const size_t M = (size_t)Nx * Ny;
+    const double NxNy = (double)Nx * Ny;
+    const double s = (Nx == Ny) ? (double)Nx : sqrt(NxNy);
+    const double lambda_sqr = lambda * lambda, alpha_max_sqr = alpha_max * alpha_max;
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            const size_t k = (size_t)j + (size_t)i * Ny;
+            const double ksqr = ref_pow2(kx[i]) + ref_pow2(ky[j]);
+            if (lambda_sqr * ksqr > alpha_max_sqr)
+                psi[2 * k] = sqrt((psi[2 * k] * psi[2 * k]) + (psi[2 * k + 1] * psi[2 * k + 1]));
+            else
+                psi[2 * k] = 0.0;
+            psi[2 * k + 1] = 0.0;
+        }
+    if (orc_fft2d(psi, Nx, Ny, +1)) return -1;
+    for (size_t k = 0; k < M; k++) {
+        psi[2 * k] = ((psi[2 * k]) * (mtf2d[2 * k]) - (psi[2 * k + 1]) * (mtf2d[2 * k + 1])) / s;
+        psi[2 * k + 1] = ((psi[2 * k]) * (mtf2d[2 * k + 1]) + (psi[2 * k + 1]) * (mtf2d[2 * k])) / s;
+    }
+    if (orc_fft2d(psi, Nx, Ny, -1)) return -1;
+    for (size_t k = 0; k < 2 * M; k++) psi[k] = psi[k] / s;
+    bw_limit(psi, kx, Nx, ky, Ny, bwcutoff_t);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* tem.cpp:944-1276: unique species, templates (scatterer_pap_LUT),
  * auto-slicing, per-slice propagator and transmission function. */
@@ -955,6 +1034,13 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
     double k1 = job->det_inner / lambda, k2 = job->det_outer / lambda;
     if (job->adf_mode == 1) { k1 = k1 * k1; k2 = k2 * k2; }
     int rc = 0;
+    double* mtf2d = NULL;
+    if (job->mtf && job->mtf_domain && job->mtf_n >= 1 && job->mtf_rate != 0) {
+        mtf2d = (double*)malloc(sizeof(double) * 2 * M);
+        if (orc_mtf_2d(job->mtf, job->mtf_domain, job->mtf_n, job->mtf_rate, kx, ky, Nx, Ny,
+                       job->xperiod, job->yperiod, bwt, mtf2d))
+            rc = -1;
+    }
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #pragma omp parallel
@@ -970,6 +1056,8 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                                            job->transmit_mode, psi);
             if (r) rc = r;
             if (job->adf && adf_image) adf_image[p] = orc_adf(psi, kx, ky, Nx, Ny, k1, k2);
+            if (mtf2d && orc_apply_mtf(psi, mtf2d, kx, ky, Nx, Ny, lambda, job->alpha_max, bwt))
+                rc = -1;
             if (job->fem)
                 orc_fem_radial(psi, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max,
                                radial + (size_t)p * nbins, counts + (size_t)p * nbins);
@@ -1001,7 +1089,7 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                              radial, counts, corr_avg, corr_sqr);
         free(mag2);
     }
-    free(kx); free(ky); free(xx); free(yy); free(probe); free(sx); free(sy);
+    free(kx); free(ky); free(xx); free(yy); free(probe); free(sx); free(sy); free(mtf2d);
     free(radial); free(counts);
     return rc;
 }

@@ -34,6 +34,12 @@ typedef struct orc_job {
     double dr;                        /* azimuthal binning size factor */
     int32_t transmit_mode;            /* 0: reference in-place update, 1: exact complex product */
     int32_t adf_mode;                 /* 0: reference k^2 vs angle/lambda, 1: |k| vs angle/lambda */
+    /* detector MTF (--mtf_file, --mtf_sampling_rate): active when mtf_n >= 1
+     * and mtf_rate != 0; mtf_domain in units of the Nyquist frequency as read */
+    const double* mtf;
+    const double* mtf_domain;
+    int32_t mtf_n;
+    double mtf_rate;
 } orc_job;
 
 /* ---- scalars and domains ---- */
@@ -125,6 +131,13 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                     double* omega_vimage, double* v_gt17, double* v_rbar, double* corr_avg,
                     double* corr_sqr, double* exit_psi);
 int orc_correlograph_nphi(double delta_k, int nbins);
+/* the 2-D detector MTF (adfstem.cpp:300-439, tem.cpp:276-285), complex [nx][ny] */
+int orc_mtf_2d(const double* mtf, const double* mtf_domain, int n, double rate, const double* kx,
+               const double* ky, int nx, int ny, double xperiod, double yperiod,
+               double bwcutoff_t, double* out);
+/* its application to one k-space exit wave in place (adfstem.cpp:2272-2385) */
+int orc_apply_mtf(double* psi, const double* mtf2d, const double* kx, const double* ky, int nx,
+                  int ny, double lambda, double alpha_max, double bwcutoff_t);
 int orc_correlograph(const double* mag2, long n_positions, const double* kx, const double* ky,
                      int nx, int ny, double delta_k, int nbins, double lambda, double alpha_max,
                      const double* radial1, const int32_t* counts1, double* avg, double* sqr);

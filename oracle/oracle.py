@@ -34,6 +34,8 @@ class Job(C.Structure):
         ("adf", C.c_int32), ("det_inner", C.c_double), ("det_outer", C.c_double),
         ("fem", C.c_int32), ("dr", C.c_double),
         ("transmit_mode", C.c_int32), ("adf_mode", C.c_int32),
+        ("mtf", C.c_void_p), ("mtf_domain", C.c_void_p), ("mtf_n", C.c_int32),
+        ("mtf_rate", C.c_double),
     ]
 
 
@@ -103,6 +105,10 @@ def lib():
         L.orc_adfstem_fem.argtypes = [C.POINTER(Job), C.POINTER(_Spec), _dp, C.c_int, _dp,
                                       C.c_int, C.c_int] + [C.c_void_p] * 10
         L.orc_correlograph_nphi.argtypes = [C.c_double, C.c_int]
+        L.orc_mtf_2d.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, _dp, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_apply_mtf.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, C.c_double]
         L.orc_correlograph.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                        C.c_int, C.c_double, C.c_double, _dp, _i32, _dp, _dp]
         L.orc_fem_modes.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, C.c_int, C.c_double,
@@ -204,7 +210,9 @@ def transmission(q, Z, atoms, zlist, templates, kx, ky, xx, yy, bwcutoff_t):
     return t
 
 
-def make_job(**kw):
+def make_job(mtf=None, mtf_domain=None, mtf_rate=0.0, **kw):
+    """orc_job; mtf / mtf_domain (equal lengths, Nyquist units) and mtf_rate
+    select the detector MTF.  The arrays are kept alive on the job."""
     j = Job()
     defaults = dict(zmin=0.0, xmin=0.0, ymin=0.0, min_slice=1.0, corrected=0, cs3=0.0,
                     cs5=0.0, defocus=0.0, alpha_max=0.01, adf=1, det_inner=0.04,
@@ -212,7 +220,36 @@ def make_job(**kw):
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(j, k, v)
+    if mtf is not None:
+        m = np.ascontiguousarray(mtf, np.float64)
+        d = np.ascontiguousarray(mtf_domain, np.float64)
+        assert m.size == d.size
+        j._mtf_keep = (m, d)
+        j.mtf, j.mtf_domain, j.mtf_n, j.mtf_rate = m.ctypes.data, d.ctypes.data, m.size, mtf_rate
     return j
+
+
+def mtf_2d(job, mtf, mtf_domain, rate):
+    """orc_mtf_2d on the job's grid: complex [nx, ny]."""
+    kx, ky, _, _ = domains(job.nx, job.ny, job.xperiod, job.yperiod, job.xmin, job.ymin)
+    _, bwt = bwcutoffs(job.nx, job.ny, job.xperiod, job.yperiod)
+    m = np.ascontiguousarray(mtf, np.float64)
+    d = np.ascontiguousarray(mtf_domain, np.float64)
+    out = np.zeros((job.nx, job.ny), np.complex128)
+    assert lib().orc_mtf_2d(m, d, m.size, rate, kx, ky, job.nx, job.ny, job.xperiod,
+                            job.yperiod, bwt, out.view(np.float64)) == 0
+    return out
+
+
+def apply_mtf(job, psi, mtf2d):
+    """orc_apply_mtf on a copy of one k-space wave [nx, ny]."""
+    kx, ky, _, _ = domains(job.nx, job.ny, job.xperiod, job.yperiod, job.xmin, job.ymin)
+    _, bwt = bwcutoffs(job.nx, job.ny, job.xperiod, job.yperiod)
+    lam = wavelength(job.voltage)[0]
+    w = np.ascontiguousarray(psi, np.complex128).copy()
+    assert lib().orc_apply_mtf(w.view(np.float64), np.ascontiguousarray(mtf2d).view(np.float64),
+                               kx, ky, job.nx, job.ny, lam, job.alpha_max, bwt) == 0
+    return w
 
 
 def probe(job):

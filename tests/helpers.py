@@ -24,7 +24,8 @@ def random_atoms(n, cell, seed, species=(14,)):
 class Case:
     def __init__(self, nx, ny, cell, voltage, q, Z, min_slice=1.0, cs3=0.0, cs5=0.0,
                  defocus=0.0, alpha_max=0.01, corrected=0, adf=True, inner=0.04, outer=0.2,
-                 fem=False, dr=1 / np.sqrt(2), transmit_mode=0, adf_mode=0):
+                 fem=False, dr=1 / np.sqrt(2), transmit_mode=0, adf_mode=0, mtf=None,
+                 mtf_domain=None, mtf_rate=0.0):
         self.__dict__.update(locals())
         del self.__dict__["self"]
 
@@ -35,7 +36,8 @@ class Case:
                           defocus=self.defocus, alpha_max=self.alpha_max,
                           adf=1 if self.adf else 0, det_inner=self.inner, det_outer=self.outer,
                           fem=1 if self.fem else 0, dr=self.dr,
-                          transmit_mode=self.transmit_mode, adf_mode=self.adf_mode)
+                          transmit_mode=self.transmit_mode, adf_mode=self.adf_mode,
+                          mtf=self.mtf, mtf_domain=self.mtf_domain, mtf_rate=self.mtf_rate)
 
     def grid(self, msf):
         return msf.make_grid(self.nx, self.ny, self.cell[0], self.cell[1], self.cell[2],
