@@ -67,10 +67,21 @@ typedef struct msf_detector {
 } msf_detector;
 
 /* FEM V_omega (d1): adfstem.cpp:947-1066, variance.cpp:64-300,
- * adfstem.cpp:2495-2601 and :2822-2866. */
+ * adfstem.cpp:2495-2601 and :2822-2866.
+ * Detector MTF (--mtf_file, --mtf_sampling_rate; adfstem.cpp:238-453,
+ * :2272-2385): when mtf and mtf_domain are set, mtf_n >= 1 and
+ * mtf_sampling_rate != 0, every exit wave is replaced, after the ADF sum and
+ * before the FEM statistics, by the MTF-filtered |psi| outside the aperture.
+ * mtf_domain is in units of the Nyquist frequency, as read from the file
+ * (msf_read_mtf); the arrays are read during the call only.  Zero-filled
+ * trailing fields (an initialiser naming only enabled and dr) leave it off. */
 typedef struct msf_fem {
     int32_t enabled;
     double dr;                         /* --dr azimuthal binning size factor (default 1/sqrt 2) */
+    const double* mtf;                 /* mtf_n MTF values (second line of the MTF file) */
+    const double* mtf_domain;          /* mtf_n spatial frequencies (first line) */
+    uint32_t mtf_n;
+    double mtf_sampling_rate;          /* --mtf_sampling_rate */
 } msf_fem;
 
 /* Transmission multiply of the slice loop (adfstem.cpp:1922-1952).  The
@@ -183,6 +194,18 @@ int msf_adf_reduce(msf_ctx* ctx, const msf_grid* grid, const msf_detector* det,
 int msf_fem_accumulate(msf_ctx* ctx, const msf_grid* grid, const msf_fem* fem, double alpha_max,
                        const double* psi_k, uint32_t count, double* radial, int32_t* counts,
                        double* v_omega);
+
+/* The 2-D detector MTF of fem's table on the grid (adfstem.cpp:300-439):
+ * nx*ny complex doubles, reference layout.  Parity entry point. */
+int msf_mtf_2d(msf_ctx* ctx, const msf_grid* grid, const msf_fem* fem, double* mtf2d);
+/* The MTF application of adfstem.cpp:2272-2385 on count <= 65535 k-space
+ * waves in place (FP64, reference layout): |psi| outside the aperture
+ * (lambda^2 k^2 > alpha_max^2), BACKWARD FFT, times the 2-D MTF / sqrt(NxNy)
+ * with the reference's in-place real/imaginary update, FORWARD FFT,
+ * / sqrt(NxNy), bw_limit.  Parity entry point; the CLI uses it for the
+ * final diffraction image of an MTF run. */
+int msf_apply_mtf(msf_ctx* ctx, const msf_grid* grid, const msf_fem* fem, double alpha_max,
+                  double* psi_k, uint32_t count);
 
 /* ------------------------------------------------------------ run */
 /* Number of V_omega bins (adfstem.cpp:947-972) and their lower edges

@@ -47,6 +47,24 @@ int msf_read_xyz(const char* path, msf_model* out);
 int msf_read_lammps_data(const char* path, msf_model* out);
 void msf_model_free(msf_model* m);
 
+/* A detector MTF table: n spatial frequencies (Nyquist units) and values.
+ * Owned by the library. */
+typedef struct msf_mtf_table {
+    uint32_t n;
+    double* domain;
+    double* mtf;
+} msf_mtf_table;
+
+/* read_mtf_file (tem_io.cpp:5722-5800): the first line holds the domain, the
+ * second the MTF values; commas count as spaces; each line is read with
+ * repeated stream extraction until its end, so trailing blanks repeat the
+ * line's last value.  As with every reader here a line counts only if a
+ * newline follows it.  Unequal lengths are an error, and so are an empty
+ * table (which the reference would index out of bounds) and a non-numeric
+ * entry (on which the reference's read loop never ends). */
+int msf_read_mtf(const char* path, msf_mtf_table* out);
+void msf_mtf_table_free(msf_mtf_table* t);
+
 /* write_tif_grayscale<uint16_t> (write_tif.hpp:143-335): data is a
  * row-major height x width array scaled from [min, max] to 0..65535
  * (logscale: log(x - min + 1) / log(max - min + 1)); rows are stored

@@ -51,7 +51,7 @@ struct Params {
     double dr = 0.70710678118654746; // 1/sqrt(2) (tem.cpp:178)
     double minslice = 1.0;           // (tem.cpp:180)
     unsigned dupe_x = 1, dupe_y = 1, dupe_z = 1;
-    std::string mtf_name;
+    std::vector<double> mtf, mtf_domain; // as read, appended per read (tem_io.cpp:5758,5776)
     double mtf_res = 0;
 };
 
@@ -59,6 +59,22 @@ std::string lower(std::string s)
 {
     std::transform(s.begin(), s.end(), s.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
     return s;
+}
+
+// read_mtf_file (tem_io.cpp:5722-5800), called where the key or flag is
+// parsed; the values are appended as the reference appends them
+bool read_mtf(const std::string& name, Params& p)
+{
+    msf_mtf_table t{};
+    if (msf_read_mtf(name.c_str(), &t) != MSF_OK) {
+        std::cout << msf_io_last_error() << std::endl;
+        std::cout << "Error, could not read MTF file : " << name << std::endl;
+        return false;
+    }
+    p.mtf_domain.insert(p.mtf_domain.end(), t.domain, t.domain + t.n);
+    p.mtf.insert(p.mtf.end(), t.mtf, t.mtf + t.n);
+    msf_mtf_table_free(&t);
+    return true;
 }
 
 // read_parameter_file (tem_io.cpp:4063-4500): one key per line, key case
@@ -97,7 +113,12 @@ bool read_parameter_file(const std::string& name, Flags& f, Params& p)
             if (p.det_inner > p.det_outer) std::swap(p.det_inner, p.det_outer);
             f.detector_angles = true;
         }
-        else if (key == "mtf_file") { s >> p.mtf_name; f.mtf_file = true; }
+        else if (key == "mtf_file") {
+            std::string name;
+            s >> name;
+            if (read_mtf(name, p)) f.mtf_file = true;
+            else f.fail = true;
+        }
         else if (key == "mtf_sampling_rate") {
             p.mtf_res = 0;
             if (!s.eof()) s >> p.mtf_res;
@@ -140,9 +161,11 @@ bool read_parameter_file(const std::string& name, Flags& f, Params& p)
     return true;
 }
 
-// read_cmdline_options (tem_io.cpp:5277-5700).  One deliberate difference:
+// read_cmdline_options (tem_io.cpp:5277-5700).  Two deliberate differences:
 // --detectorangles takes two values (the reference reads the first value
-// twice and then rejects the second as an unexpected argument).
+// twice and then rejects the second as an unexpected argument), and
+// --mtf_file consumes its file name (the reference reads the file but does
+// not advance past the name, which it then rejects as an unexpected argument).
 void read_cmdline(const std::vector<std::string>& args, Flags& f, Params& p)
 {
     auto num = [&](size_t i, auto& v) {
@@ -160,8 +183,21 @@ void read_cmdline(const std::vector<std::string>& args, Flags& f, Params& p)
             f.pf = true;
             i += 1;
         }
-        else if (a == "--mtf_file") { if (i + 1 < args.size()) p.mtf_name = args[i + 1]; f.mtf_file = true; i += 1; }
-        else if (a == "--mtf_sampling_rate") { num(i + 1, p.mtf_res); f.mtf_resolution = p.mtf_res != 0; i += 1; }
+        else if (a == "--mtf_file") {
+            const std::string name = i + 1 < args.size() ? args[i + 1] : std::string();
+            if (read_mtf(name, p)) f.mtf_file = true;
+            else f.fail = true;
+            i += 1;
+        }
+        else if (a == "--mtf_sampling_rate") {
+            p.mtf_res = 0;
+            if (i + 1 < args.size()) {
+                num(i + 1, p.mtf_res);
+                i += 1;
+            }
+            f.mtf_resolution = p.mtf_res > 0;
+            if (!f.mtf_resolution) std::cout << " Warning: attempted to use MTF sampling rate <= 0" << std::endl;
+        }
         else if (a == "--debug") f.debug = true;
         else if (a == "--output_prefix") { if (i + 1 < args.size()) p.prefix = args[i + 1]; f.o = true; i += 1; }
         else if (a == "-m") {
@@ -253,7 +289,7 @@ bool check_supported(const Flags& f)
     const Item items[] = {
         {f.correlograph_every, "--correlograph_everyimage / --correlograph_everytxt"},
         {f.bfctem_corrected || f.bfctem_uncorrected, "--bfctem*"},
-        {f.mtf_file, "--mtf_file"}, {f.pap_tif, "--paptif"},
+        {f.pap_tif, "--paptif"},
         {f.diffraction_output, "diffractionimages"},
         {f.netcdf_images || f.netcdf_variance, "--netcdf*"},
         {f.lammps_pre, "--lammps_* (molecular dynamics coupling)"}};
@@ -485,6 +521,8 @@ int main(int argc, char** argv)
                 std::printf("mode %s cs3 %.17g cs5 %.17g defocus %.17g alphamax %.17g\n",
                             m.suffix + 1, o.cs3, o.cs5, o.defocus, o.alpha_max);
             }
+        const bool mtf_on = f.mtf_file && f.mtf_resolution;
+        std::printf("mtf %zu %.17g\n", mtf_on ? p.mtf.size() : (size_t)0, mtf_on ? p.mtf_res : 0.0);
         std::printf("prefix %s\n", p.prefix.c_str());
         return EXIT_SUCCESS;
     }
@@ -510,6 +548,14 @@ int main(int argc, char** argv)
     const bool corr = f.fem && f.correlograph;
     fem.enabled = (f.fem && (f.d1 || f.d2 || f.d3 || f.d4 || f.gt17 || f.rva || corr)) ? 1 : 0;
     fem.dr = p.dr;
+    // detector MTF (adfstem.cpp:238, :2272): both the table and a sampling rate
+    const bool mtf = f.mtf_file && f.mtf_resolution;
+    if (mtf) {
+        fem.mtf = p.mtf.data();
+        fem.mtf_domain = p.mtf_domain.data();
+        fem.mtf_n = (uint32_t)p.mtf.size();
+        fem.mtf_sampling_rate = p.mtf_res;
+    }
     uint32_t nbins = 0, nphi = 0;
     std::vector<double> edges, k_edges, phi_edges;
     if (fem.enabled) {
@@ -593,6 +639,14 @@ int main(int argc, char** argv)
             const double rx = grid.xperiod * 1.0e-8, ry = grid.yperiod * 1.0e-8;
             for (double s : {1.0e-5, 1.0e-1, 1.0e0, 1.0e10, 1.0e20})
                 w |= write_diffraction(out + "_first", ex.data(), p.Nx, p.Ny, s, rx, ry);
+            if (mtf) { // the final wave has been through the MTF (adfstem.cpp:2272-2385)
+                const size_t M = (size_t)p.Nx * p.Ny;
+                std::vector<double> w64(2 * M);
+                for (size_t k = 0; k < 2 * M; ++k) w64[k] = ex[2 * M + k];
+                if (msf_apply_mtf(c, &grid, &fem, o.alpha_max, w64.data(), 1) != MSF_OK)
+                    return die("mtf", c);
+                for (size_t k = 0; k < 2 * M; ++k) ex[2 * M + k] = (float)w64[k];
+            }
             w |= write_diffraction(out + "_final", ex.data() + 2 * (size_t)p.Nx * p.Ny, p.Nx, p.Ny, 1.0, rx, ry);
             if (w) {
                 std::cerr << "Error writing images: " << msf_io_last_error() << std::endl;

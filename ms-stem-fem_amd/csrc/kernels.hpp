@@ -91,6 +91,16 @@ hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbi
 hipError_t launch_group_sums_f64(const double2* psi, int64_t plane, const int32_t* offsets,
                                  const int32_t* pixels, const int32_t* kind, int groups,
                                  int count, double* out, hipStream_t s);
+// detector MTF steps: |psi| outside the aperture from the compact FP32 planes
+// into full FP64 planes; the same in place on full FP64 planes; / s with the
+// band limit back into compact FP32 planes; a /= d
+hipError_t launch_mtf_magnitude(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
+                                const uint8_t* outside, int batch, double2* out, hipStream_t s);
+hipError_t launch_mtf_magnitude_f64(double2* a, int64_t m, int count, const uint8_t* outside,
+                                    hipStream_t s);
+hipError_t launch_mtf_compact(const double2* full, int nx, int ny, int ncp, int ka, int kb,
+                              const int2* keep, double s, int batch, float2* psi, hipStream_t st);
+hipError_t launch_div_f64(double2* a, int64_t n, double d, hipStream_t s);
 // correlograph sums of c and c^2 per (ring, phi sector) over a batch
 hipError_t launch_fem_correlograph(const double* partial, int groups, int batch, int nbins,
                                    int nphi, int off_abs, int off_corr, const int32_t* counts,

@@ -702,6 +702,107 @@ hipError_t launch_group_sums_f64(const double2* psi, int64_t plane, const int32_
     return hipGetLastError();
 }
 
+// Detector MTF, first step (adfstem.cpp:2279-2292): |psi| of the compact
+// FP32 exit waves outside the aperture (outside[i*ny+j] != 0), else 0, into
+// full-grid FP64 planes with zero imaginary part.
+__global__ void mtf_magnitude_kernel(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
+                                     const uint8_t* outside, int64_t total, double2* out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int j = (int)(idx % ny);
+    const int64_t r = idx / ny; // b * nx + i
+    const int i = (int)(r % nx);
+    double v = 0.0;
+    if (outside[(int64_t)i * ny + j] && (j < ka || j >= ny - kb)) {
+        const int jc = (j < ka) ? j : j - ny + ka + kb;
+        const float2 w = psi[r * ncp + jc];
+        const double re = w.x, im = w.y;
+        v = sqrt((re * re) + (im * im));
+    }
+    out[idx] = double2{v, 0.0};
+}
+
+hipError_t launch_mtf_magnitude(const float2* psi, int nx, int ny, int ncp, int ka, int kb,
+                                const uint8_t* outside, int batch, double2* out, hipStream_t s)
+{
+    const int64_t n = (int64_t)batch * nx * ny;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mtf_magnitude_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       psi, nx, ny, ncp, ka, kb, outside, n, out);
+    return hipGetLastError();
+}
+
+// the same on full-grid FP64 waves in place
+__global__ void mtf_magnitude_f64_kernel(double2* a, int64_t m, int64_t total,
+                                         const uint8_t* outside)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const double2 w = a[idx];
+    const double v = outside[idx % m] ? sqrt((w.x * w.x) + (w.y * w.y)) : 0.0;
+    a[idx] = double2{v, 0.0};
+}
+
+hipError_t launch_mtf_magnitude_f64(double2* a, int64_t m, int count, const uint8_t* outside,
+                                    hipStream_t s)
+{
+    const int64_t n = m * count;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mtf_magnitude_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       s, a, m, n, outside);
+    return hipGetLastError();
+}
+
+// Detector MTF, last step (adfstem.cpp:2369-2383): / sqrt(NxNy), bw_limit
+// (keep ranges per row), back into the compact FP32 planes; the padding
+// columns are written as zero.
+__global__ void mtf_compact_kernel(const double2* full, int nx, int ny, int ncp, int ka, int kb,
+                                   const int2* keep, double s, int64_t total, float2* psi)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int c = (int)(idx % ncp);
+    const int64_t r = idx / ncp; // b * nx + i
+    const int i = (int)(r % nx);
+    float2 v{0.f, 0.f};
+    if (c < ka + kb) {
+        const int j = (c < ka) ? c : c + ny - ka - kb;
+        const int2 k = keep[i];
+        if (j < k.x || j >= ny - k.y) {
+            const double2 w = full[r * ny + j];
+            v = float2{(float)(w.x / s), (float)(w.y / s)};
+        }
+    }
+    psi[idx] = v;
+}
+
+hipError_t launch_mtf_compact(const double2* full, int nx, int ny, int ncp, int ka, int kb,
+                              const int2* keep, double s, int batch, float2* psi, hipStream_t st)
+{
+    const int64_t n = (int64_t)batch * nx * ncp;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mtf_compact_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       full, nx, ny, ncp, ka, kb, keep, s, n, psi);
+    return hipGetLastError();
+}
+
+// a /= d on both parts (the reference divides, it does not scale by 1/d)
+__global__ void div_f64_kernel(double2* a, int64_t n, double d)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const double2 w = a[idx];
+    a[idx] = double2{w.x / d, w.y / d};
+}
+
+hipError_t launch_div_f64(double2* a, int64_t n, double d, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(div_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, d);
+    return hipGetLastError();
+}
+
 // Per-pixel sums over positions of |psi| and |psi|^2 in raster order, FP64
 // (adfstem.cpp:2753-2781), on the compact plane of the exit waves.
 __global__ void pixel_moments_kernel(const float2* psi, int64_t plane, int batch, double* s1,

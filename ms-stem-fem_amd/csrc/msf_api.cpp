@@ -85,6 +85,12 @@ struct RunPlan {
     DevBuf psi, part, segp, state, femv, adf, shift, exit;
     DevBuf vsum, pm1, pm2;                    // Vbar_r sums; per-pixel |psi|, |psi|^2 sums
     DevBuf csum, csqr;                        // correlograph sums of c and c^2
+    // detector MTF: the table it was built from, the 2-D MTF [nx][ny], the
+    // outside-aperture map and the full-grid FP64 work planes [B][nx][ny]
+    bool mtf_on = false;
+    std::vector<double> mtf_v, mtf_d;
+    double mtf_rate = 0.0;
+    DevBuf mtf2d, mtf_out, mtf_full;
     // the job's launch sequence as an instantiated HIP graph, replayed while
     // every pointer and size it was captured with is unchanged
     hipGraphExec_t graph = nullptr;
@@ -115,6 +121,7 @@ struct ServeJob {
     msf_fem fem{};
     double* adf = nullptr;
     msf_fem_outputs fo{};
+    std::vector<double> mtf, mtf_dom; // copies of the MTF table (fem.mtf* point here)
 };
 
 struct ServeState {
@@ -234,6 +241,42 @@ std::vector<int2> to_int2(const msf::RowRanges& r)
     std::vector<int2> v(r.ab.size() / 2);
     for (size_t i = 0; i < v.size(); ++i) v[i] = int2{r.ab[2 * i], r.ab[2 * i + 1]};
     return v;
+}
+
+// The 2-D detector MTF (adfstem.cpp:300-439): the table sampled on the grid
+// on the host in FP64, then on the device BACKWARD FFT, bw_limit applied to
+// that real-space array by k index (as the reference does), FORWARD FFT,
+// / NxNy.  out: [nx][ny] double2.
+int build_mtf2d(msf_ctx* c, const msf::Domain& d, double xperiod, double yperiod,
+                const msf_fem* fem, DevBuf& out)
+{
+    const std::vector<double> m = msf::mtf_table_2d(d, xperiod, yperiod, fem->mtf,
+                                                    fem->mtf_domain, (int)fem->mtf_n,
+                                                    fem->mtf_sampling_rate);
+    std::vector<double2> h(m.size());
+    for (size_t k = 0; k < m.size(); ++k) h[k] = double2{m[k], 0.0};
+    const double2 *twx, *twy;
+    if (int r = twiddles(c, d.nx, nullptr, &twx)) return r;
+    if (int r = twiddles(c, d.ny, nullptr, &twy)) return r;
+    DevBuf keep;
+    HIPCHK(c, upload(out, h, c->stream));
+    HIPCHK(c, upload(keep, to_int2(msf::bwlimit_keep_ranges(d)), c->stream));
+    double2* v = out.as<double2>();
+    HIPCHK(c, msf::fft2d_f64(v, d.nx, d.ny, 1, true, twx, twy, c->stream));
+    HIPCHK(c, msf::launch_apply_keep(v, d.nx, d.ny, 1, keep.as<int2>(), c->stream));
+    HIPCHK(c, msf::fft2d_f64(v, d.nx, d.ny, 1, false, twx, twy, c->stream));
+    HIPCHK(c, msf::launch_div_f64(v, (int64_t)m.size(), d.nxny, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream)); // keep and h are released on return
+    return MSF_OK;
+}
+
+int check_mtf(msf_ctx* c, const msf_fem* fem)
+{
+    if (!fem || !fem->mtf || !fem->mtf_domain || fem->mtf_n < 1)
+        return fail(c, MSF_ERR_INVALID, "no MTF table");
+    if (fem->mtf_sampling_rate == 0.0)
+        return fail(c, MSF_ERR_INVALID, "mtf_sampling_rate must be non-zero");
+    return MSF_OK;
 }
 
 // H(k) -> BACKWARD FFT; leaves h = IFFT(H) (unnormalised) and its squared norm
@@ -358,6 +401,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const bool want_pix = fem_on && (out.v_re || out.omega_vimage || out.v_gt17 || out.v_rbar);
     const bool want_corr = fem_on && (out.corr_avg || out.corr_sqr);
     const bool want_fem = want_d1 || want_d2 || want_pix || want_corr;
+    // the detector MTF changes only what the FEM statistics see
+    const bool want_mtf = want_fem && fem->mtf && fem->mtf_domain && fem->mtf_n >= 1 &&
+                          fem->mtf_sampling_rate != 0.0;
     if (want_adf) {
         if (det->inner < 0.0 || det->outer < 0.0)
             return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
@@ -380,15 +426,36 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const int64_t plane = (int64_t)nx * ncp;
 
     RunPlan& pl = c->plan;
-    msf_detector dkey{};
-    msf_fem fkey{};
-    if (want_adf) dkey = *det;
-    if (want_fem) fkey = *fem;
+    // cache keys built member by member (no padding bytes from the caller);
+    // the MTF table is compared by content below
+    msf_detector dkey;
+    msf_fem fkey;
+    std::memset(&dkey, 0, sizeof dkey);
+    std::memset(&fkey, 0, sizeof fkey);
+    if (want_adf) {
+        dkey.enabled = det->enabled;
+        dkey.inner = det->inner;
+        dkey.outer = det->outer;
+        dkey.threshold_mode = det->threshold_mode;
+    }
+    if (want_fem) {
+        fkey.enabled = fem->enabled;
+        fkey.dr = fem->dr;
+    }
+    if (want_mtf) {
+        fkey.mtf_n = fem->mtf_n;
+        fkey.mtf_sampling_rate = fem->mtf_sampling_rate;
+    }
+    bool same_mtf = pl.mtf_on == want_mtf;
+    if (same_mtf && want_mtf)
+        same_mtf = std::equal(pl.mtf_v.begin(), pl.mtf_v.end(), fem->mtf) &&
+                   std::equal(pl.mtf_d.begin(), pl.mtf_d.end(), fem->mtf_domain) &&
+                   pl.mtf_v.size() == fem->mtf_n;
     const bool same_optics = pl.valid && pl.spec_id == sp->id &&
                              std::memcmp(&pl.optics, o, sizeof(msf_optics)) == 0;
     const bool same_groups = same_optics && std::memcmp(&pl.det, &dkey, sizeof dkey) == 0 &&
                              std::memcmp(&pl.fem, &fkey, sizeof fkey) == 0 &&
-                             pl.fem_sqr == want_d2 && pl.corr == want_corr;
+                             pl.fem_sqr == want_d2 && pl.corr == want_corr && same_mtf;
     if (!same_optics || !same_groups) pl.valid = false;
 
     if (!same_groups) {
@@ -418,6 +485,22 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, upload(pl.zero, g.zeroed, c->stream));
         pl.cap_b = 0; // partial-sum buffers depend on G
         pl.hg = g;
+        pl.mtf_on = false;
+        pl.mtf_v.clear();
+        pl.mtf_d.clear();
+        if (want_mtf) {
+            // the MTF's bw_limit must keep exactly the compact layout's pixels
+            if (msf::bwlimit_keep_ranges(d).ab != keep.ab)
+                return fail(c, MSF_ERR_UNSUPPORTED,
+                            "MTF band limit differs from the slice band limit on this grid");
+            if (int r = build_mtf2d(c, d, sp->grid.xperiod, sp->grid.yperiod, fem, pl.mtf2d))
+                return r;
+            HIPCHK(c, upload(pl.mtf_out, msf::outside_aperture(d, o->alpha_max), c->stream));
+            pl.mtf_v.assign(fem->mtf, fem->mtf + fem->mtf_n);
+            pl.mtf_d.assign(fem->mtf_domain, fem->mtf_domain + fem->mtf_n);
+            pl.mtf_rate = fem->mtf_sampling_rate;
+            pl.mtf_on = true;
+        }
     }
     const int G = pl.G;
     const int nbins = pl.nbins;
@@ -495,6 +578,13 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, pl.state.alloc((size_t)std::max(nbins, 1) * 3 * sizeof(double)));
         HIPCHK(c, pl.femv.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
         HIPCHK(c, pl.vsum.alloc((size_t)std::max(nbins, 1) * sizeof(double)));
+    }
+    if (want_mtf && pl.mtf_full.bytes < (size_t)B * M * sizeof(double2))
+        HIPCHK(c, pl.mtf_full.alloc((size_t)B * M * sizeof(double2)));
+    const double2 *twx64 = nullptr, *twy64 = nullptr;
+    if (want_mtf) {
+        if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
+        if (int r = twiddles(c, ny, nullptr, &twy64)) return r;
     }
     if (want_corr && pl.csum.bytes < ncorr * sizeof(double)) {
         HIPCHK(c, pl.csum.alloc(ncorr * sizeof(double)));
@@ -603,6 +693,28 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         if (want_adf)
             HIPCHK(c, msf::launch_collect_adf(dpart.as<double>(), G, nb, dadf.as<double>() + b0,
                                               c->stream));
+        if (want_mtf) {
+            // adfstem.cpp:2272-2385 after the ADF sum: |psi| outside the
+            // aperture, BACKWARD FFT, x MTF / s (in-place update), FORWARD
+            // FFT, / s, bw_limit; then the group sums again for the FEM
+            double2* full = pl.mtf_full.as<double2>();
+            HIPCHK(c, msf::launch_mtf_magnitude(dpsi.as<float2>(), nx, ny, ncp, ka, kb,
+                                                pl.mtf_out.as<uint8_t>(), nb, full, c->stream));
+            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, true, twx64, twy64, c->stream));
+            HIPCHK(c, msf::launch_transmit_f64(full, pl.mtf2d.as<double2>(), M, nb, 0, d.sqrt_nxny,
+                                               c->stream));
+            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, false, twx64, twy64, c->stream));
+            HIPCHK(c, msf::launch_mtf_compact(full, nx, ny, ncp, ka, kb, dkeep.as<int2>(),
+                                              d.sqrt_nxny, nb, dpsi.as<float2>(), c->stream));
+            HIPCHK(c, msf::launch_reduce_segments(dpsi.as<float2>(), plane,
+                                                  pl.seg_pix.as<int32_t>(),
+                                                  pl.tile_off.as<int32_t>(),
+                                                  pl.seg_kind.as<int32_t>(), pl.n_tiles, nb,
+                                                  pl.segp.as<double>(), c->stream));
+            HIPCHK(c, msf::launch_combine_segments(pl.segp.as<double>(), pl.n_tiles * 64,
+                                                   pl.seg_of_group.as<int32_t>(), G, nb,
+                                                   dpart.as<double>(), c->stream));
+        }
         if (want_d1)
             HIPCHK(c, msf::launch_fem_welford(dpart.as<double>(), G, nb, nbins, dcnt.as<int32_t>(),
                                               dzero.as<uint8_t>(), dstate.as<double>(),
@@ -643,7 +755,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             (uint64_t)want_d1 | (uint64_t)want_d2 << 1 | (uint64_t)want_pix << 2 |
                 (uint64_t)want_corr << 3,
             (uint64_t)(uintptr_t)pl.csum.p, (uint64_t)(uintptr_t)pl.csqr.p, (uint64_t)nphi,
-            (uint64_t)pl.hg.corr,
+            (uint64_t)pl.hg.corr, (uint64_t)want_mtf, (uint64_t)(uintptr_t)pl.mtf2d.p,
+            (uint64_t)(uintptr_t)pl.mtf_out.p, (uint64_t)(uintptr_t)pl.mtf_full.p,
+            (uint64_t)(uintptr_t)twx64, (uint64_t)(uintptr_t)twy64,
             (uint64_t)(uintptr_t)pl.vsum.p, (uint64_t)(uintptr_t)pl.pm1.p,
             (uint64_t)(uintptr_t)pl.pm2.p, (uint64_t)pl.hg.fem_abs, (uint64_t)pl.hg.fem_sqr,
             (uint64_t)plane, (uint64_t)(uintptr_t)dpsi.p, (uint64_t)(uintptr_t)dpart.p,
@@ -1274,6 +1388,58 @@ int msf_fem_accumulate(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, dou
     return MSF_OK;
 }
 
+int msf_mtf_2d(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, double* mtf2d)
+{
+    if (!c) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
+    if (!mtf2d) return fail(c, MSF_ERR_INVALID, "NULL argument");
+    if (int r = check_grid(c, grid)) return r;
+    if (int r = check_mtf(c, fem)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    DevBuf m;
+    if (int r = build_mtf2d(c, d, grid->xperiod, grid->yperiod, fem, m)) return r;
+    HIPCHK(c, hipMemcpy(mtf2d, m.p, (size_t)d.nx * d.ny * sizeof(double2), hipMemcpyDeviceToHost));
+    return MSF_OK;
+}
+
+int msf_apply_mtf(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, double alpha_max,
+                  double* psi_k, uint32_t count)
+{
+    if (!c) return MSF_ERR_INVALID;
+    MSF_LOCK(c);
+    if (!psi_k) return fail(c, MSF_ERR_INVALID, "NULL argument");
+    if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
+    if (int r = check_grid(c, grid)) return r;
+    if (int r = check_mtf(c, fem)) return r;
+    HIPCHK(c, hipSetDevice(c->device));
+    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+    const int nx = d.nx, ny = d.ny;
+    const size_t M = (size_t)nx * ny;
+    DevBuf m, w, out, keep;
+    if (int r = build_mtf2d(c, d, grid->xperiod, grid->yperiod, fem, m)) return r;
+    const double2 *twx, *twy;
+    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+    HIPCHK(c, upload(out, msf::outside_aperture(d, alpha_max), c->stream));
+    HIPCHK(c, upload(keep, to_int2(msf::bwlimit_keep_ranges(d)), c->stream));
+    HIPCHK(c, w.alloc(M * count * sizeof(double2)));
+    double2* v = w.as<double2>();
+    HIPCHK(c, hipMemcpyAsync(v, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, msf::launch_mtf_magnitude_f64(v, (int64_t)M, (int)count, out.as<uint8_t>(), c->stream));
+    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, true, twx, twy, c->stream));
+    HIPCHK(c, msf::launch_transmit_f64(v, m.as<double2>(), (int64_t)M, (int)count, 0, d.sqrt_nxny,
+                                       c->stream));
+    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, false, twx, twy, c->stream));
+    HIPCHK(c, msf::launch_div_f64(v, (int64_t)(M * count), d.sqrt_nxny, c->stream));
+    HIPCHK(c, msf::launch_apply_keep(v, nx, ny, (int)count, keep.as<int2>(), c->stream));
+    HIPCHK(c, hipMemcpyAsync(psi_k, v, M * count * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MSF_OK;
+}
+
 int msf_correlograph_bins(const msf_grid* grid, double dr, uint32_t* n_phi, double* k_edges,
                           double* phi_edges)
 {
@@ -1409,6 +1575,10 @@ void serve_loop(msf_ctx* c)
             msf_specimen* sp = nullptr;
             rc = msf_specimen_build(c, &job.grid, job.q.data(), job.Z.data(),
                                     (uint32_t)job.Z.size(), job.min_slice, &sp);
+            if (!job.mtf.empty()) {
+                job.fem.mtf = job.mtf.data();
+                job.fem.mtf_domain = job.mtf_dom.data();
+            }
             if (rc == MSF_OK)
                 rc = msf_run_stem_fem(c, sp, &job.optics, &job.det, &job.fem, job.xp.data(),
                                       (uint32_t)job.xp.size(), job.yp.data(),
@@ -1443,6 +1613,12 @@ int msf_submit(msf_ctx* c, const msf_job* j, uint64_t* id)
     job.yp.assign(j->y_p, j->y_p + j->n_yp);
     job.det = j->detector;
     job.fem = j->fem;
+    if (j->fem.mtf && j->fem.mtf_domain && j->fem.mtf_n) {
+        job.mtf.assign(j->fem.mtf, j->fem.mtf + j->fem.mtf_n);
+        job.mtf_dom.assign(j->fem.mtf_domain, j->fem.mtf_domain + j->fem.mtf_n);
+    }
+    job.fem.mtf = nullptr;
+    job.fem.mtf_domain = nullptr;
     job.adf = j->adf_image;
     job.fo = j->fem_out;
     {

@@ -176,6 +176,46 @@ int correlograph_nphi(double delta_k, int nbins)
     return count > 0 ? (int)count - 1 : 0;
 }
 
+std::vector<double> mtf_table_2d(const Domain& d, double xperiod, double yperiod,
+                                 const double* mtf, const double* domain, int n, double rate)
+{
+    std::vector<double> dsq((size_t)n);
+    for (int k = 0; k < n; ++k) dsq[k] = ref_pow2((0.5 * rate * domain[k])); // tem.cpp:283-284
+    std::vector<double> m((size_t)d.nx * d.ny, 0.0);
+    for (int i = 0; i < d.nx; ++i)
+        for (int j = 0; j < d.ny; ++j) {
+            const double xxx = ref_pow2(xperiod) * d.kx[i] / d.nx;
+            const double yyy = ref_pow2(yperiod) * d.ky[j] / d.ny;
+            const double rsqr = ref_pow2(xxx) + ref_pow2(yyy);
+            double& v = m[(size_t)i * d.ny + j];
+            // every segment holding rsqr overwrites (no early exit, as written)
+            for (size_t idx = 0; idx + 1 < (size_t)n; ++idx) {
+                if ((rsqr >= dsq[idx]) && (rsqr < dsq[idx + 1])) {
+                    const double d1 = std::sqrt(dsq[idx]), d2 = std::sqrt(dsq[idx + 1]);
+                    v = mtf[idx] + (mtf[idx + 1] - mtf[idx]) * (std::sqrt(rsqr) - d1) / (d2 - d1);
+                    double arg = xxx / rate;
+                    const double sx = (arg != 0) ? std::sin(arg) / arg : 1;
+                    arg = yyy / rate;
+                    const double sy = (arg != 0) ? std::sin(arg) / arg : 1;
+                    v = v * sx * sy;
+                }
+            }
+        }
+    return m;
+}
+
+std::vector<uint8_t> outside_aperture(const Domain& d, double alpha_max)
+{
+    const double l2 = d.lambda * d.lambda, a2 = alpha_max * alpha_max; // adfstem.cpp:95-96
+    std::vector<uint8_t> o((size_t)d.nx * d.ny);
+    for (int i = 0; i < d.nx; ++i)
+        for (int j = 0; j < d.ny; ++j) {
+            const double ksqr = ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]);
+            o[(size_t)i * d.ny + j] = (l2 * ksqr > a2) ? 1 : 0;
+        }
+    return o;
+}
+
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
                     double k2, bool fem, double dr, double alpha_max, bool fem_sqr,
                     bool correlograph)

@@ -121,6 +121,14 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
 // adfstem.cpp:969-985: the correlograph's phi sector count for nbins rings of width delta_k
 int correlograph_nphi(double delta_k, int nbins);
 
+// The detector MTF sampled on the grid, real [nx][ny] (adfstem.cpp:300-376;
+// the squared domain (0.5 rate d)^2 of tem.cpp:276-285 formed from the
+// file's Nyquist units).  n >= 1.
+std::vector<double> mtf_table_2d(const Domain& d, double xperiod, double yperiod,
+                                 const double* mtf, const double* domain, int n, double rate);
+// adfstem.cpp:2279-2290: 1 where lambda^2 k^2 > alpha_max^2 (kept as |psi|), else 0
+std::vector<uint8_t> outside_aperture(const Domain& d, double alpha_max);
+
 // Groups cut into segments of at most `seg` pixels, 64 segments per wavefront
 // tile, pixel offsets interleaved [tile][k][lane] (-1 pads short segments) so
 // that a wavefront's index loads are coalesced.  Segments of one group are

@@ -205,6 +205,59 @@ void msf_model_free(msf_model* m)
     m->n_atoms = 0;
 }
 
+void msf_mtf_table_free(msf_mtf_table* t)
+{
+    if (!t) return;
+    std::free(t->domain);
+    std::free(t->mtf);
+    t->domain = nullptr;
+    t->mtf = nullptr;
+    t->n = 0;
+}
+
+int msf_read_mtf(const char* path, msf_mtf_table* out)
+{
+    if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
+    *out = msf_mtf_table{};
+    std::ifstream in(path);
+    if (!in.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening MTF file: ") + path);
+    std::vector<double> rows[2]; // domain, mtf
+    for (std::vector<double>& v : rows) {
+        std::string line;
+        if (!(std::getline(in, line) && in.good())) break;
+        std::replace(line.begin(), line.end(), ',', ' ');
+        std::istringstream ls(line);
+        // one value per line, as declared by the reference before its loop: an
+        // extraction that finds only trailing blanks leaves it unchanged, so the
+        // last value repeats (for a blank line the reference reads an
+        // uninitialised double; 0 here)
+        double value = 0.0;
+        while (!ls.eof()) {
+            ls >> value;
+            if (ls.fail() && !ls.eof()) // the reference's loop never ends here
+                return io_fail(MSF_ERR_INVALID, std::string("non-numeric entry in MTF file: ") +
+                                                    path);
+            v.push_back(value);
+        }
+    }
+    if (rows[0].size() != rows[1].size())
+        return io_fail(MSF_ERR_INVALID, std::string("Error reading MTF file: ") + path +
+                                            "\n MTF (1st line) and MTF domain (2nd line) do not "
+                                            "have the same number of elements.");
+    if (rows[0].empty()) return io_fail(MSF_ERR_INVALID, std::string("empty MTF table: ") + path);
+    const size_t n = rows[0].size();
+    out->domain = (double*)std::malloc(n * sizeof(double));
+    out->mtf = (double*)std::malloc(n * sizeof(double));
+    if (!out->domain || !out->mtf) {
+        msf_mtf_table_free(out);
+        return io_fail(MSF_ERR_INVALID, "out of memory");
+    }
+    std::copy(rows[0].begin(), rows[0].end(), out->domain);
+    std::copy(rows[1].begin(), rows[1].end(), out->mtf);
+    out->n = (uint32_t)n;
+    return MSF_OK;
+}
+
 int msf_read_xyz(const char* path, msf_model* out)
 {
     if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");

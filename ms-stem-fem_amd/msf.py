@@ -50,7 +50,30 @@ class Detector(C.Structure):
 
 
 class Fem(C.Structure):
-    _fields_ = [("enabled", C.c_int32), ("dr", C.c_double)]
+    _fields_ = [("enabled", C.c_int32), ("dr", C.c_double), ("mtf", C.c_void_p),
+                ("mtf_domain", C.c_void_p), ("mtf_n", C.c_uint32),
+                ("mtf_sampling_rate", C.c_double)]
+
+
+def make_fem(dr, mtf=None, mtf_domain=None, sampling_rate=0.0, enabled=1):
+    """msf_fem with an optional detector MTF table (values and spatial
+    frequencies in Nyquist units, as in the MTF file); the arrays are kept
+    alive on the returned struct."""
+    f = Fem(enabled, dr)
+    if mtf is not None:
+        m = np.ascontiguousarray(mtf, np.float64)
+        d = np.ascontiguousarray(mtf_domain, np.float64)
+        if m.size != d.size:
+            raise ValueError("mtf and mtf_domain differ in length")
+        f._keep = (m, d)
+        f.mtf, f.mtf_domain, f.mtf_n, f.mtf_sampling_rate = (m.ctypes.data, d.ctypes.data,
+                                                              m.size, sampling_rate)
+    return f
+
+
+class MtfTable(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("domain", C.POINTER(C.c_double)),
+                ("mtf", C.POINTER(C.c_double))]
 
 
 class Model(C.Structure):
@@ -105,7 +128,8 @@ class Stats(C.Structure):
 
 IO_EXPORTS = ["msf_io_last_error", "msf_element_z", "msf_guess_z", "msf_read_xyz",
               "msf_read_lammps_data", "msf_model_free", "msf_write_tiff16",
-              "msf_write_variance_txt", "msf_write_f64", "msf_write_correlograph_txt"]
+              "msf_write_variance_txt", "msf_write_f64", "msf_write_correlograph_txt",
+              "msf_read_mtf", "msf_mtf_table_free"]
 
 EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf_get_stats",
            "msf_supported_length", "msf_wavelength", "msf_scherzer_defocus",
@@ -115,7 +139,7 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem",
            "msf_submit", "msf_poll", "msf_wait", "msf_specimen_propagator", "msf_probe_k",
            "msf_slice_step_batch", "msf_correlograph_bins", "msf_adf_reduce",
-           "msf_fem_accumulate"]
+           "msf_fem_accumulate", "msf_mtf_2d", "msf_apply_mtf"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -172,6 +196,11 @@ def lib():
                                      _dp]
         L.msf_fem_accumulate.argtypes = [vp, C.POINTER(Grid), C.POINTER(Fem), C.c_double, _dp,
                                          C.c_uint32, _dp, C.POINTER(C.c_int32), _dp]
+        L.msf_mtf_2d.argtypes = [vp, C.POINTER(Grid), C.POINTER(Fem), _dp]
+        L.msf_apply_mtf.argtypes = [vp, C.POINTER(Grid), C.POINTER(Fem), C.c_double, _dp,
+                                    C.c_uint32]
+        L.msf_read_mtf.argtypes = [C.c_char_p, C.POINTER(MtfTable)]
+        L.msf_mtf_table_free.argtypes = [C.POINTER(MtfTable)]
         L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
         L.msf_poll.argtypes = [vp, C.c_uint64]
         L.msf_wait.argtypes = [vp, C.c_uint64]
@@ -262,6 +291,17 @@ def read_model(path):
         return q, Z, (m.xperiod, m.yperiod, m.zperiod)
     finally:
         lib().msf_model_free(C.byref(m))
+
+
+def read_mtf(path):
+    """msf_read_mtf: (domain [n], mtf [n]) as read from an MTF file."""
+    t = MtfTable()
+    _io_check(lib().msf_read_mtf(os.fsencode(path), C.byref(t)), "read " + path)
+    try:
+        return (np.ctypeslib.as_array(t.domain, shape=(t.n,)).copy(),
+                np.ctypeslib.as_array(t.mtf, shape=(t.n,)).copy())
+    finally:
+        lib().msf_mtf_table_free(C.byref(t))
 
 
 def write_tiff16(path, data, vmin=None, vmax=None, logscale=False, resolution_unit=3,
@@ -486,6 +526,20 @@ class Context:
                                              counts.ctypes.data_as(C.POINTER(C.c_int32)), _p(v)),
                     "msf_fem_accumulate")
         return radial[:w.shape[0] * nb].reshape(w.shape[0], nb), counts[:nb], v[:nb]
+
+    def mtf_2d(self, grid, fem):
+        """msf_mtf_2d: the 2-D detector MTF [nx, ny] (complex)."""
+        out = np.zeros((grid.nx, grid.ny), np.complex128)
+        self._check(lib().msf_mtf_2d(self._h, C.byref(grid), C.byref(fem),
+                                     _p(out.view(np.float64))), "msf_mtf_2d")
+        return out
+
+    def apply_mtf(self, grid, fem, alpha_max, psi_k):
+        """msf_apply_mtf on a copy of k-space waves [count, nx, ny] (complex)."""
+        w = np.ascontiguousarray(psi_k, np.complex128).reshape(-1, grid.nx, grid.ny).copy()
+        self._check(lib().msf_apply_mtf(self._h, C.byref(grid), C.byref(fem), alpha_max,
+                                        _p(w.view(np.float64)), w.shape[0]), "msf_apply_mtf")
+        return w.reshape(np.shape(psi_k))
 
     def slice_step(self, spec, slice_index, psi_k):
         """msf_slice_step_batch on a copy of psi_k [count, nx, ny] (complex)."""

@@ -50,7 +50,8 @@ class Case:
         return msf.Detector(1 if self.adf else 0, self.inner, self.outer, self.adf_mode)
 
     def femcfg(self, msf):
-        return msf.Fem(1 if self.fem else 0, self.dr)
+        return msf.make_fem(self.dr, self.mtf, self.mtf_domain, self.mtf_rate,
+                            enabled=1 if self.fem else 0)
 
     def run_oracle(self, O, x_p, y_p, exit_waves=False, threads=0):
         job = self.oracle_job(O)

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol(msf):
         assert hasattr(lib, n), n
     assert sorted(msf.EXPORTS) == names
     io = _declared("msf_io.h")
-    assert len(io) == 10
+    assert len(io) == 12
     for n in io:
         assert hasattr(lib, n), n
     assert sorted(msf.IO_EXPORTS) == io

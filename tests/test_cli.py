@@ -172,3 +172,27 @@ def test_correlograph_flags(tmp_path, model):
     for flag in ("--correlograph_everyimage", "--correlograph_everytxt"):
         rc, _, out = run(base + [flag], tmp_path)
         assert rc == 1 and "not supported" in out, flag
+
+
+def test_mtf_flags(tmp_path, model):
+    (tmp_path / "mtf.txt").write_text("0, 0.5, 1\n1, 0.5, 0.25\n")
+    base = ["-m", "64", "64", "300000", "-a", model, "--output_prefix", "o", "--cs3", "0",
+            "--defocus", "0", "--alphamax", "0.01", "--plan_only", "--adfstemuncorrfem"]
+    rc, plan, out = run(base + ["--mtf_file", "mtf.txt", "--mtf_sampling_rate", "8"], tmp_path)
+    assert rc == 0, out
+    assert plan["mtf"] == ["3 8"]
+    # a file without a sampling rate (or with a rate <= 0) lacks a parameter
+    rc, _, out = run(base + ["--mtf_file", "mtf.txt"], tmp_path)
+    assert rc == 1 and "lacking required parameters" in out
+    rc, _, out = run(base + ["--mtf_file", "mtf.txt", "--mtf_sampling_rate", "-1"], tmp_path)
+    assert rc == 1 and "sampling rate <= 0" in out
+    rc, _, out = run(base + ["--mtf_file", "nothere.txt", "--mtf_sampling_rate", "8"], tmp_path)
+    assert rc == 1 and "could not read MTF file" in out
+    # parameter-file keys; a negative rate is accepted there (tem_io.cpp:4249-4258)
+    pf = tmp_path / "p.txt"
+    pf.write_text("mtf_file mtf.txt\nmtf_sampling_rate -2.5\n")
+    rc, plan, out = run(["--parameter_file", str(pf)] + base, tmp_path)
+    assert rc == 0, out
+    assert plan["mtf"] == ["3 -2.5"]
+    rc, plan, _ = run(base, tmp_path)
+    assert plan["mtf"] == ["0 0"]

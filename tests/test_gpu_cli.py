@@ -153,3 +153,45 @@ def test_cli_correlograph_run(tmp_path, O):
     cl = _tiff(f"{pre}_correlograph_variance_logscale.tif").astype(float)
     tl = np.log(var - var.min() + 1) / np.log(var.max() - var.min() + 1)
     assert np.abs(cl - np.where(tl > 0, np.floor(tl * 65535.0 + 0.5), 0)).max() <= 2
+
+
+def test_cli_mtf_run(tmp_path, O):
+    """--mtf_file / --mtf_sampling_rate: V_omega of the MTF-filtered waves and
+    the final diffraction image (taken after the MTF) against the oracle; the
+    ADF image and first diffraction are taken before the MTF."""
+    rng = np.random.default_rng(31)
+    n_at = 45
+    q = rng.random((n_at, 3)) * [10.0, 10.0, 6.0]
+    names = rng.choice(["Si", "Cu"], n_at)
+    lines = [f"{n_at}", "10 10 6"] + [f"{e} {x:.6f} {y:.6f} {z:.6f}" for e, (x, y, z) in zip(names, q)]
+    (tmp_path / "m.xyz").write_text("\n".join(lines) + "\n")
+    dom = np.linspace(0.0, 1.0, 6)
+    tab = 1.0 / (1.0 + (dom / 0.5) ** 2)
+    (tmp_path / "mtf.txt").write_text(", ".join(f"{v:.17g}" for v in dom) + "\n" +
+                                      ", ".join(f"{v:.17g}" for v in tab) + "\n")
+    args = [CLI, "-m", "128", "128", "300000", "-a", "m.xyz", "--output_prefix", "run",
+            "--adfstemuncorrfem", "--cs3", "1.0e6", "--defocus", "25", "--alphamax", "0.01",
+            "--rasterspacing", "2.5", "--minslice", "1.5", "--dr", "0.5", "--images",
+            "--mtf_file", "mtf.txt", "--mtf_sampling_rate", "9"]
+    print("cli:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    qq = np.array([[float(v) for v in l.split()[1:]] for l in lines[2:]])
+    Z = np.array([14 if e == "Si" else 29 for e in names], np.uint32)
+    c = Case(nx=128, ny=128, cell=(10.0, 10.0, 6.0), voltage=300e3, q=qq, Z=Z, min_slice=1.5,
+             cs3=1.0e6, defocus=25.0, alpha_max=0.01, inner=0.040, outer=0.200, fem=True, dr=0.5,
+             mtf=tab, mtf_domain=dom, mtf_rate=9.0)
+    xp = np.arange(4) * 2.5
+    _, (img_o, v_o, ex_o) = c.run_oracle(O, xp, xp, exit_waves=True)
+    pre = tmp_path / "run_adfstem_uncorr"
+    raw = open(f"{pre}_stem_image_output.f64", "rb").read()
+    img = np.frombuffer(raw.split(b"\n", 1)[1], "<f8").reshape(4, 4)
+    assert rel_l2(img, img_o) < 1e-4
+    txt = np.loadtxt(f"{pre}_fem_V_omega.txt")
+    assert rel_l2(txt[:, 1], v_o) < 1e-4
+    d = _tiff(f"{pre}_final_diffraction_scalefactor_1.tif").astype(float)
+    m = np.fft.fftshift(np.log(1.0 + np.abs(ex_o[15]) ** 2))
+    want = np.floor((m - m.min()) * (65535.0 / (m.max() - m.min())) + 0.5)
+    bad = np.abs(d - want) > 2
+    assert bad.mean() < 1e-3, bad.sum()

@@ -240,3 +240,35 @@ def test_correlograph_txt_format(tmp_path):
     # no phi sectors: the corner and one boundary per k bin
     msf.write_correlograph_txt(p, np.zeros((0, 2)), [0.0, 0.5, 1.0], [])
     assert open(p).read() == "0 \n0.0000000000e+00 \n5.0000000000e-01 \n"
+
+
+def test_mtf_reader(tmp_path):
+    """read_mtf_file (tem_io.cpp:5722-5800): domain on the first line, values
+    on the second, commas as blanks; trailing blanks repeat a line's last
+    value (the stream extraction that finds only blanks leaves its variable
+    as it was)."""
+    p = tmp_path / "m.txt"
+    p.write_text("0, 0.5, 1.0\n1.0, 0.6, 0.2\n")
+    d, m = msf.read_mtf(str(p))
+    np.testing.assert_array_equal(d, [0.0, 0.5, 1.0])
+    np.testing.assert_array_equal(m, [1.0, 0.6, 0.2])
+    p.write_text("0 0.5 \n1 0.6 \n")
+    d, m = msf.read_mtf(str(p))
+    np.testing.assert_array_equal(d, [0.0, 0.5, 0.5])
+    np.testing.assert_array_equal(m, [1.0, 0.6, 0.6])
+    p.write_text("0 0.25 1\n1e0\t0.5 2.5e-1\nignored line\n")
+    d, m = msf.read_mtf(str(p))
+    np.testing.assert_array_equal(m, [1.0, 0.5, 0.25])
+
+
+@pytest.mark.parametrize("text", ["0 0.5\n1 0.6",          # second line lacks its newline
+                                  "0 0.5 1\n1 0.6\n",       # unequal lengths
+                                  "0 x\n1 0.6\n",           # the reference would loop forever
+                                  ""])
+def test_mtf_reader_rejects(tmp_path, text):
+    p = tmp_path / "m.txt"
+    p.write_text(text)
+    with pytest.raises(msf.MsfError):
+        msf.read_mtf(str(p))
+    with pytest.raises(msf.MsfError):
+        msf.read_mtf(str(tmp_path / "missing.txt"))
