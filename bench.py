@@ -199,7 +199,7 @@ def main():
     plain = None
     if not args.no_plain:
         n = min(args.plain_positions, P)
-        ctx.configure(batch=1, graph=False)   # one position per launch, stream launches
+        ctx.configure(batch=1, graph=False, chains=1)   # one position per launch, stream launches
         xs = np.resize(xp, n)
         ctx.adfstem(spec, optics, xs[:1], yp[:1], fem=fem)
         t0 = time.perf_counter()
@@ -233,7 +233,8 @@ def main():
         "config": {"workload": "cuzr_fem_512: 3500 Cu/Zr atoms, 40 A cube, 512x512, 200 kV, "
                                "Cs3 1.2 mm Scherzer, 2 mrad, 32x32 raster, V_omega",
                    "grid": 512, "atoms": int(Z.size), "slices": int(S), "positions": int(P),
-                   "batch": int(st["batch"]), "seed": 2001},
+                   "batch": int(st["batch"]), "chains": int(st["chains"]),
+                   "graph": int(st["graph"]), "seed": 2001},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": plain,

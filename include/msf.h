@@ -100,6 +100,11 @@ typedef struct msf_config {
     int32_t col_tile;                  /* columns per column-pass workgroup: 8, 16; 0 = automatic */
     int32_t no_graph;                  /* 1: launch on the stream; 0: capture the job's launch
                                           sequence once per shape as a HIP graph and replay it */
+    int32_t chains;                    /* batch chains in flight: 1, or 2 (alternate batches on
+                                          two streams, FEM statistics still in raster order,
+                                          results bit-identical); 0 = automatic (two chains on
+                                          the streams for launches of >= 4M pixels, else one
+                                          chain as a graph) */
 } msf_config;
 
 /* Timings of the last msf_run_stem() on the context. */
@@ -115,6 +120,7 @@ typedef struct msf_stats {
     uint64_t row_pass_count, col_pass_count;
     double row_pass_bytes, col_pass_bytes; /* compulsory HBM bytes of one launch */
     int32_t graph;                     /* 0: stream launches; 1: graph captured; 2: graph replayed */
+    int32_t chains;                    /* batch chains in flight (msf_config.chains) */
 } msf_stats;
 
 /* ------------------------------------------------------------ context */

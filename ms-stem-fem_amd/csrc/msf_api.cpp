@@ -83,6 +83,7 @@ struct RunPlan {
     msf::Groups hg;                           // host copy for the per-pixel FEM modes
     int64_t cap_b = 0, cap_p = 0;
     DevBuf psi, part, segp, state, femv, adf, shift, exit;
+    DevBuf psi2, part2, segp2, mtf_full2;     // the second batch chain's work buffers
     DevBuf vsum, pm1, pm2;                    // Vbar_r sums; per-pixel |psi|, |psi|^2 sums
     DevBuf csum, csqr;                        // correlograph sums of c and c^2
     // detector MTF: the table it was built from, the 2-D MTF [nx][ny], the
@@ -141,8 +142,11 @@ struct msf_ctx {
     std::unique_ptr<ServeState> serve;
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream and the events that order two batch chains (created on first use)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fem[2] = {nullptr, nullptr};
     std::string err;
-    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0};
+    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0, 0};
     msf_stats stats{};
     std::map<int, DevBuf> tw32, tw64, twnat;
     RunPlan plan;
@@ -561,13 +565,40 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     B = std::max<int64_t>(1, std::min<int64_t>(B, P));
 
     // work buffers, grown as needed and kept for the next run
+    // Two chains of batches in flight on two streams: the row passes (VALU
+    // bound) of one chain overlap the column passes (memory bound) of the
+    // other.  Automatic: two chains, launched on the streams, once a launch
+    // covers >= 4M pixels (512^2: +4 % over one chain, graph or not; the
+    // runtime replays a two-branch graph's branches one after the other);
+    // below that one chain replayed as a graph, where launch cost matters.
+    // Profiled and exit-wave runs use one chain.
+    const bool prof = c->cfg.profile_kernels != 0;
+    const int64_t n_batches = (P + B - 1) / B;
+    const bool big_launch = B * plane >= (4ll << 20);
+    const int chains_req = c->cfg.chains ? c->cfg.chains : (big_launch ? 2 : 1);
+    const int chains = (chains_req == 2 && n_batches >= 2 && !prof && !out.exit) ? 2 : 1;
+    const bool auto_eager = c->cfg.chains == 0 && chains == 2;
     if (B > pl.cap_b || pl.psi.bytes < (size_t)B * plane * sizeof(float2)) {
         HIPCHK(c, pl.psi.alloc((size_t)B * plane * sizeof(float2)));
         HIPCHK(c, pl.part.alloc((size_t)B * G * sizeof(double)));
         HIPCHK(c, pl.segp.alloc((size_t)B * std::max(pl.n_tiles, 1) * 64 * sizeof(double)));
         HIPCHK(c, hipMemsetAsync(pl.psi.p, 0, pl.psi.bytes, c->stream));
+        pl.psi2.release();
+        pl.mtf_full.release();
+        pl.mtf_full2.release();
         pl.cap_b = B;
         pl.exit.release();
+    }
+    if (chains == 2 && pl.psi2.bytes < pl.psi.bytes) {
+        HIPCHK(c, pl.psi2.alloc(pl.psi.bytes));
+        HIPCHK(c, pl.part2.alloc(pl.part.bytes));
+        HIPCHK(c, pl.segp2.alloc(pl.segp.bytes));
+        HIPCHK(c, hipMemsetAsync(pl.psi2.p, 0, pl.psi2.bytes, c->stream));
+    }
+    if (chains == 2 && !c->stream2) {
+        HIPCHK(c, hipStreamCreate(&c->stream2)); // default priority, as the first stream
+        for (hipEvent_t* e : {&c->ev_fork, &c->ev_join, &c->ev_fem[0], &c->ev_fem[1]})
+            HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     if (P > pl.cap_p) {
         HIPCHK(c, pl.adf.alloc((size_t)P * sizeof(double)));
@@ -581,6 +612,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     }
     if (want_mtf && pl.mtf_full.bytes < (size_t)B * M * sizeof(double2))
         HIPCHK(c, pl.mtf_full.alloc((size_t)B * M * sizeof(double2)));
+    if (want_mtf && chains == 2 && pl.mtf_full2.bytes < (size_t)B * M * sizeof(double2))
+        HIPCHK(c, pl.mtf_full2.alloc((size_t)B * M * sizeof(double2)));
     const double2 *twx64 = nullptr, *twy64 = nullptr;
     if (want_mtf) {
         if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
@@ -616,8 +649,6 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
            &dU0 = pl.U0, &dPsi0 = pl.Psi0;
 
     // events for per-kernel timing
-    const bool prof = c->cfg.profile_kernels != 0;
-    const int64_t n_batches = (P + B - 1) / B;
     enum { K_ROW0, K_ROW, K_COL, K_COLL, K_RED, K_NUM };
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_kind;
@@ -662,82 +693,94 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     // every launch of the job (all batches, reductions, FEM statistics);
     // issued directly or captured once into a graph
     auto enqueue = [&]() -> int {
-    for (int64_t b0 = 0; b0 < P; b0 += B) {
+    hipStream_t streams[2] = {c->stream, c->stream2};
+    if (chains == 2) { // fork: the second stream starts after everything issued so far
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    }
+    int64_t k = 0;
+    for (int64_t b0 = 0; b0 < P; b0 += B, ++k) {
         const int nb = (int)std::min<int64_t>(B, P - b0);
-        ra.psi = dpsi.as<float2>();
+        const int ch = (int)(k % chains);
+        hipStream_t s = streams[ch];
+        float2* bpsi = (ch ? pl.psi2 : dpsi).as<float2>();
+        double* bpart = (ch ? pl.part2 : dpart).as<double>();
+        double* bsegp = (ch ? pl.segp2 : pl.segp).as<double>();
+        ra.psi = bpsi;
         ra.batch = nb;
         ra.shift = dshift.as<int2>() + b0;
-        ca.psi = dpsi.as<float2>();
+        ca.psi = bpsi;
         ca.batch = nb;
         ca.shift = dshift.as<int2>() + b0;
         for (int n = 0; n < S; ++n) {
             ra.t = t32 + (int64_t)n * M;
             ra.u0 = dU0.as<float2>() + (int64_t)n * 2 * M;
             HIPCHK(c, ev_begin(n == 0 ? K_ROW0 : K_ROW));
-            HIPCHK(c, msf::launch_row_pass(ny, n == 0, tmode, ra, c->stream));
+            HIPCHK(c, msf::launch_row_pass(ny, n == 0, tmode, ra, s));
             HIPCHK(c, ev_end());
             ca.px = px + (int64_t)n * nx;
             ca.py = py + (int64_t)n * ny;
             HIPCHK(c, ev_begin(n == S - 1 ? K_COLL : K_COL));
-            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, ca, c->stream));
+            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, ca, s));
             HIPCHK(c, ev_end());
         }
         HIPCHK(c, ev_begin(K_RED));
-        HIPCHK(c, msf::launch_reduce_segments(dpsi.as<float2>(), plane, pl.seg_pix.as<int32_t>(),
+        HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix.as<int32_t>(),
                                               pl.tile_off.as<int32_t>(), pl.seg_kind.as<int32_t>(),
-                                              pl.n_tiles, nb, pl.segp.as<double>(), c->stream));
-        HIPCHK(c, msf::launch_combine_segments(pl.segp.as<double>(), pl.n_tiles * 64,
-                                               pl.seg_of_group.as<int32_t>(), G, nb,
-                                               dpart.as<double>(), c->stream));
+                                              pl.n_tiles, nb, bsegp, s));
+        HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64,
+                                               pl.seg_of_group.as<int32_t>(), G, nb, bpart, s));
         HIPCHK(c, ev_end());
         if (want_adf)
-            HIPCHK(c, msf::launch_collect_adf(dpart.as<double>(), G, nb, dadf.as<double>() + b0,
-                                              c->stream));
+            HIPCHK(c, msf::launch_collect_adf(bpart, G, nb, dadf.as<double>() + b0, s));
         if (want_mtf) {
             // adfstem.cpp:2272-2385 after the ADF sum: |psi| outside the
             // aperture, BACKWARD FFT, x MTF / s (in-place update), FORWARD
             // FFT, / s, bw_limit; then the group sums again for the FEM
-            double2* full = pl.mtf_full.as<double2>();
-            HIPCHK(c, msf::launch_mtf_magnitude(dpsi.as<float2>(), nx, ny, ncp, ka, kb,
-                                                pl.mtf_out.as<uint8_t>(), nb, full, c->stream));
-            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, true, twx64, twy64, c->stream));
+            double2* full = (ch ? pl.mtf_full2 : pl.mtf_full).as<double2>();
+            HIPCHK(c, msf::launch_mtf_magnitude(bpsi, nx, ny, ncp, ka, kb,
+                                                pl.mtf_out.as<uint8_t>(), nb, full, s));
+            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, true, twx64, twy64, s));
             HIPCHK(c, msf::launch_transmit_f64(full, pl.mtf2d.as<double2>(), M, nb, 0, d.sqrt_nxny,
-                                               c->stream));
-            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, false, twx64, twy64, c->stream));
+                                               s));
+            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, false, twx64, twy64, s));
             HIPCHK(c, msf::launch_mtf_compact(full, nx, ny, ncp, ka, kb, dkeep.as<int2>(),
-                                              d.sqrt_nxny, nb, dpsi.as<float2>(), c->stream));
-            HIPCHK(c, msf::launch_reduce_segments(dpsi.as<float2>(), plane,
-                                                  pl.seg_pix.as<int32_t>(),
+                                              d.sqrt_nxny, nb, bpsi, s));
+            HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix.as<int32_t>(),
                                                   pl.tile_off.as<int32_t>(),
-                                                  pl.seg_kind.as<int32_t>(), pl.n_tiles, nb,
-                                                  pl.segp.as<double>(), c->stream));
-            HIPCHK(c, msf::launch_combine_segments(pl.segp.as<double>(), pl.n_tiles * 64,
-                                                   pl.seg_of_group.as<int32_t>(), G, nb,
-                                                   dpart.as<double>(), c->stream));
+                                                  pl.seg_kind.as<int32_t>(), pl.n_tiles, nb, bsegp,
+                                                  s));
+            HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64,
+                                                   pl.seg_of_group.as<int32_t>(), G, nb, bpart, s));
         }
+        // the FEM accumulators are shared: batch k updates them after batch k-1
+        if (chains == 2 && k > 0) HIPCHK(c, hipStreamWaitEvent(s, c->ev_fem[(k - 1) % 2], 0));
         if (want_d1)
-            HIPCHK(c, msf::launch_fem_welford(dpart.as<double>(), G, nb, nbins, dcnt.as<int32_t>(),
-                                              dzero.as<uint8_t>(), dstate.as<double>(),
-                                              c->stream));
+            HIPCHK(c, msf::launch_fem_welford(bpart, G, nb, nbins, dcnt.as<int32_t>(),
+                                              dzero.as<uint8_t>(), dstate.as<double>(), s));
         if (want_d2)
-            HIPCHK(c, msf::launch_fem_vbar(dpart.as<double>(), G, nb, nbins, pl.hg.fem_abs,
-                                           pl.hg.fem_sqr, dcnt.as<int32_t>(), dzero.as<uint8_t>(),
-                                           pl.vsum.as<double>(), c->stream));
+            HIPCHK(c, msf::launch_fem_vbar(bpart, G, nb, nbins, pl.hg.fem_abs, pl.hg.fem_sqr,
+                                           dcnt.as<int32_t>(), dzero.as<uint8_t>(),
+                                           pl.vsum.as<double>(), s));
         if (want_corr)
-            HIPCHK(c, msf::launch_fem_correlograph(dpart.as<double>(), G, nb, nbins, nphi,
-                                                   pl.hg.fem_abs, pl.hg.corr, dcnt.as<int32_t>(),
+            HIPCHK(c, msf::launch_fem_correlograph(bpart, G, nb, nbins, nphi, pl.hg.fem_abs,
+                                                   pl.hg.corr, dcnt.as<int32_t>(),
                                                    dzero.as<uint8_t>(), pl.csum.as<double>(),
-                                                   pl.csqr.as<double>(), c->stream));
+                                                   pl.csqr.as<double>(), s));
         if (want_pix)
-            HIPCHK(c, msf::launch_pixel_moments(dpsi.as<float2>(), plane, nb, pl.pm1.as<double>(),
-                                                pl.pm2.as<double>(), c->stream));
+            HIPCHK(c, msf::launch_pixel_moments(bpsi, plane, nb, pl.pm1.as<double>(),
+                                                pl.pm2.as<double>(), s));
+        if (chains == 2) HIPCHK(c, hipEventRecord(c->ev_fem[k % 2], s));
         if (out.exit) {
-            HIPCHK(c, msf::launch_expand_exit(dpsi.as<float2>(), nx, ny, ncp, ka, kb, nb,
-                                              dexit.as<float2>(), c->stream));
+            HIPCHK(c, msf::launch_expand_exit(bpsi, nx, ny, ncp, ka, kb, nb, dexit.as<float2>(), s));
             HIPCHK(c, hipMemcpyAsync(out.exit + (size_t)b0 * nx * ny * 2, dexit.p,
                                      (size_t)nb * nx * ny * sizeof(float2), hipMemcpyDeviceToHost,
-                                     c->stream));
+                                     s));
         }
+    }
+    if (chains == 2) { // join: the first stream waits for everything on the second
+        HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     if (want_d1)
         HIPCHK(c, msf::launch_fem_finalize(dstate.as<double>(), nbins, dfemv.as<double>(), c->stream));
@@ -749,13 +792,15 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     // re-captures.  Profiled runs and exit-wave runs (host copies per batch)
     // launch directly.
     int graph_state = 0;
-    if (!prof && !out.exit && !c->cfg.no_graph) {
+    if (!prof && !out.exit && !c->cfg.no_graph && !auto_eager) {
         const std::vector<uint64_t> sig = {
             sp->id, (uint64_t)P, (uint64_t)B, (uint64_t)tmode, (uint64_t)want_adf,
             (uint64_t)want_d1 | (uint64_t)want_d2 << 1 | (uint64_t)want_pix << 2 |
                 (uint64_t)want_corr << 3,
             (uint64_t)(uintptr_t)pl.csum.p, (uint64_t)(uintptr_t)pl.csqr.p, (uint64_t)nphi,
             (uint64_t)pl.hg.corr, (uint64_t)want_mtf, (uint64_t)(uintptr_t)pl.mtf2d.p,
+            (uint64_t)chains, (uint64_t)(uintptr_t)pl.psi2.p, (uint64_t)(uintptr_t)pl.part2.p,
+            (uint64_t)(uintptr_t)pl.segp2.p, (uint64_t)(uintptr_t)pl.mtf_full2.p,
             (uint64_t)(uintptr_t)pl.mtf_out.p, (uint64_t)(uintptr_t)pl.mtf_full.p,
             (uint64_t)(uintptr_t)twx64, (uint64_t)(uintptr_t)twy64,
             (uint64_t)(uintptr_t)pl.vsum.p, (uint64_t)(uintptr_t)pl.pm1.p,
@@ -832,6 +877,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     st.row_pass_bytes = 2.0 * (double)B * nx * (ka + kb) * sizeof(float2);
     st.col_pass_bytes = st.row_pass_bytes;
     st.graph = graph_state;
+    st.chains = chains;
     if (prof) {
         double sum[K_NUM] = {0}, cnt[K_NUM] = {0};
         for (size_t k = 0; k < ev_kind.size(); ++k) {
@@ -904,6 +950,12 @@ int msf_destroy(msf_ctx* c)
         c->tw32.clear();
         c->tw64.clear();
         c->twnat.clear();
+        for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_fem[0], c->ev_fem[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (c->stream2) {
+            (void)hipStreamSynchronize(c->stream2);
+            (void)hipStreamDestroy(c->stream2);
+        }
         (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -920,6 +972,7 @@ int msf_set_config(msf_ctx* c, const msf_config* cfg)
         return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
     if (cfg->col_tile != 0 && cfg->col_tile != 8 && cfg->col_tile != 16)
         return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 8 or 16");
+    if (cfg->chains < 0 || cfg->chains > 2) return fail(c, MSF_ERR_INVALID, "chains must be 0, 1 or 2");
     c->cfg = *cfg;
     return MSF_OK;
 }

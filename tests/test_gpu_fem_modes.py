@@ -50,9 +50,11 @@ def test_fem_modes_do_not_depend_on_batch(msf, ctx):
     spec = c.gpu_specimen(msf, ctx)
     outs = []
     try:
-        for b, graph in ((0, True), (1, False), (7, True), (7, False)):
-            ctx.configure(batch=b, graph=graph)
-            outs.append(ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf))[1])
+        for b, graph, ch in ((0, True, 1), (1, False, 1), (7, True, 1), (7, False, 1),
+                             (7, True, 2), (7, False, 2), (4, True, 2)):
+            ctx.configure(batch=b, graph=graph, chains=ch)
+            outs.append(ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf),
+                                        correlograph=True)[1])
     finally:
         ctx.configure()
     for o in outs[1:]:

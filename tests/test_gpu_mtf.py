@@ -82,8 +82,8 @@ def test_mtf_runs_do_not_depend_on_batch_graph_or_cache(msf, ctx):
     again = run(c)
     outs = [again]
     try:
-        for b, graph in ((1, False), (7, True), (7, False)):
-            ctx.configure(batch=b, graph=graph)
+        for b, graph, ch in ((1, False, 1), (7, True, 1), (7, False, 1), (7, True, 2), (5, False, 2)):
+            ctx.configure(batch=b, graph=graph, chains=ch)
             outs.append(run(c))
     finally:
         ctx.configure()

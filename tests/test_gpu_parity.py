@@ -103,8 +103,8 @@ def test_batch_composition_does_not_change_results(msf, ctx, O):
     yp = np.linspace(0.0, 11.0, 5)
     spec_g = c.gpu_specimen(msf, ctx)
     outs = []
-    for b in (0, 1, 6, 35):
-        ctx.configure(batch=b)
+    for b, ch in ((0, 1), (1, 1), (6, 1), (35, 1), (6, 2), (1, 2)):
+        ctx.configure(batch=b, chains=ch)
         outs.append(ctx.adfstem(spec_g, c.optics(msf), xp, yp, detector=c.detector(msf),
                                 fem=c.femcfg(msf)))
     ctx.configure()

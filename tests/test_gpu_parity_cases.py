@@ -177,8 +177,8 @@ def test_graph_replay_matches_stream_launches(msf, ctx):
     rasters = [(np.linspace(0.0, 11.0, 6), np.linspace(0.5, 11.5, 5)),
                (np.linspace(1.3, 10.0, 6), np.linspace(0.1, 9.5, 5))]
 
-    def run(xp, yp, graph):
-        ctx.configure(batch=8, graph=graph)
+    def run(xp, yp, graph, chains=1):
+        ctx.configure(batch=8, graph=graph, chains=chains)
         out = ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
                           fem=c.femcfg(msf))
         return out, ctx.stats()["graph"]
@@ -192,6 +192,18 @@ def test_graph_replay_matches_stream_launches(msf, ctx):
                 assert st_g == (1 if rep == 0 and xp is rasters[0][0] else 2)
                 np.testing.assert_array_equal(img_g, img_e)
                 np.testing.assert_array_equal(v_g, v_e)
+        # two batch chains in flight (a graph with two branches), replayed
+        for rep in range(3):
+            for xp, yp in rasters:
+                (img_e, v_e), _ = run(xp, yp, graph=False)
+                (img_2, v_2), st_2 = run(xp, yp, graph=True, chains=2)
+                assert st_2 == (1 if rep == 0 and xp is rasters[0][0] else 2)
+                np.testing.assert_array_equal(img_2, img_e)
+                np.testing.assert_array_equal(v_2, v_e)
+                (img_3, v_3), st_3 = run(xp, yp, graph=False, chains=2)
+                assert st_3 == 0
+                np.testing.assert_array_equal(img_3, img_e)
+                np.testing.assert_array_equal(v_3, v_e)
     finally:
         ctx.configure()
 
