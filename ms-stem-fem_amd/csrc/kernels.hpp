@@ -62,9 +62,12 @@ int col_tile(int nx);
 bool row_prunable(int ny, int ka, int kb);
 bool col_prunable(int nx, const int32_t* keep_ab);
 
+// ev0/ev1 (optional): events the dispatch itself records at the kernel's
+// start and end, for profiling without marker overhead
 hipError_t launch_row_pass(int ny, bool first, int transmit_mode, const RowPassArgs& a,
-                           hipStream_t s);
-hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s);
+                           hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s,
+                           hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // per (position, segment) FP64 sums over interleaved pixel tiles:
 // kind 0 -> |psi|^2, 1 -> |psi|; then per (position, group) the fixed-order

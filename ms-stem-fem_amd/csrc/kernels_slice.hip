@@ -13,6 +13,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "fft_engine.hpp"
 #include "kernels.hpp"
 
@@ -381,14 +383,22 @@ int col_tile(int nx)
     }
 }
 
+// With ev0/ev1 the dispatch packet itself stamps the kernel's start and end
+// (hipExtLaunchKernelGGL), so profiled durations carry no marker overhead.
 template <int NY>
-static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStream_t s)
+static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
+                             hipEvent_t ev0, hipEvent_t ev1)
 {
     constexpr int RPW = 256 / Plan<NY>::T;
     const int64_t groups = (a.batch + RPW - 1) / RPW;
     const dim3 grid((unsigned)(groups * a.nx));
     const dim3 block(RPW * Plan<NY>::T);
-#define MSF_ROW(F, M, PR) hipLaunchKernelGGL((row_pass_kernel<NY, F, M, PR>), grid, block, 0, s, a)
+#define MSF_ROW(F, M, PR)                                                                          \
+    do {                                                                                           \
+        if (ev0) hipExtLaunchKernelGGL((row_pass_kernel<NY, F, M, PR>), grid, block, 0, s, ev0,    \
+                                       ev1, 0, a);                                                 \
+        else hipLaunchKernelGGL((row_pass_kernel<NY, F, M, PR>), grid, block, 0, s, a);           \
+    } while (0)
     if (a.prune) {
         if (first) { if (tmode == 0) MSF_ROW(true, 0, true); else MSF_ROW(true, 1, true); }
         else { if (tmode == 0) MSF_ROW(false, 0, true); else MSF_ROW(false, 1, true); }
@@ -400,10 +410,11 @@ static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, hipStream_t s)
+hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, hipStream_t s,
+                           hipEvent_t ev0, hipEvent_t ev1)
 {
     switch (ny) {
-#define MSF_CASE(N) case N: return row_pass_n<N>(first, tmode, a, s);
+#define MSF_CASE(N) case N: return row_pass_n<N>(first, tmode, a, s, ev0, ev1);
         MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
         MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
 #undef MSF_CASE
@@ -413,18 +424,26 @@ hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, 
 }
 
 template <int NX, int W>
-static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s)
+static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s, hipEvent_t ev0,
+                             hipEvent_t ev1)
 {
     if (a.ncp % W) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((int64_t)a.batch * (a.ncp / W)));
     const dim3 block(W * Plan<NX>::T);
+#define MSF_COL(L, PR)                                                                             \
+    do {                                                                                           \
+        if (ev0) hipExtLaunchKernelGGL((col_pass_kernel<NX, L, W, PR>), grid, block, 0, s, ev0,    \
+                                       ev1, 0, a);                                                 \
+        else hipLaunchKernelGGL((col_pass_kernel<NX, L, W, PR>), grid, block, 0, s, a);           \
+    } while (0)
     if (a.prune) {
-        if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true, W, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((col_pass_kernel<NX, false, W, true>), grid, block, 0, s, a);
+        if (last) MSF_COL(true, true);
+        else MSF_COL(false, true);
     } else {
-        if (last) hipLaunchKernelGGL((col_pass_kernel<NX, true, W, false>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((col_pass_kernel<NX, false, W, false>), grid, block, 0, s, a);
+        if (last) MSF_COL(true, false);
+        else MSF_COL(false, false);
     }
+#undef MSF_COL
     return hipGetLastError();
 }
 
@@ -435,20 +454,22 @@ template <int NX, int W> constexpr bool col_fits()
 }
 
 template <int NX>
-static hipError_t col_pass_n(bool last, const ColPassArgs& a, hipStream_t s)
+static hipError_t col_pass_n(bool last, const ColPassArgs& a, hipStream_t s, hipEvent_t ev0,
+                             hipEvent_t ev1)
 {
     int w = ColTile<NX>::W;
     if (a.tile == 8 || a.tile == 16) w = a.tile;
     if constexpr (col_fits<NX, 16>()) {
-        if (w == 16) return col_pass_w<NX, 16>(last, a, s);
+        if (w == 16) return col_pass_w<NX, 16>(last, a, s, ev0, ev1);
     }
-    return col_pass_w<NX, 8>(last, a, s);
+    return col_pass_w<NX, 8>(last, a, s, ev0, ev1);
 }
 
-hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s)
+hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s,
+                           hipEvent_t ev0, hipEvent_t ev1)
 {
     switch (nx) {
-#define MSF_CASE(N) case N: return col_pass_n<N>(last, a, s);
+#define MSF_CASE(N) case N: return col_pass_n<N>(last, a, s, ev0, ev1);
         MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
         MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
 #undef MSF_CASE

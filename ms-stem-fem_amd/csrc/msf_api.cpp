@@ -664,6 +664,19 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         return hipEventRecord(a, c->stream);
     };
     auto ev_end = [&]() -> hipError_t { return prof ? hipEventRecord(ev.back(), c->stream) : hipSuccess; };
+    // an event pair stamped by the kernel's own dispatch (slice passes)
+    hipEvent_t ka0 = nullptr, ka1 = nullptr;
+    auto ev_kernel = [&](int kind) -> hipError_t {
+        ka0 = ka1 = nullptr;
+        if (!prof) return hipSuccess;
+        hipError_t e = hipEventCreate(&ka0);
+        if (e == hipSuccess) e = hipEventCreate(&ka1);
+        if (e != hipSuccess) return e;
+        ev.push_back(ka0);
+        ev.push_back(ka1);
+        ev_kind.push_back(kind);
+        return hipSuccess;
+    };
 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const double t_loop = now_ms();
@@ -715,14 +728,12 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         for (int n = 0; n < S; ++n) {
             ra.t = t32 + (int64_t)n * M;
             ra.u0 = dU0.as<float2>() + (int64_t)n * 2 * M;
-            HIPCHK(c, ev_begin(n == 0 ? K_ROW0 : K_ROW));
-            HIPCHK(c, msf::launch_row_pass(ny, n == 0, tmode, ra, s));
-            HIPCHK(c, ev_end());
+            HIPCHK(c, ev_kernel(n == 0 ? K_ROW0 : K_ROW));
+            HIPCHK(c, msf::launch_row_pass(ny, n == 0, tmode, ra, s, ka0, ka1));
             ca.px = px + (int64_t)n * nx;
             ca.py = py + (int64_t)n * ny;
-            HIPCHK(c, ev_begin(n == S - 1 ? K_COLL : K_COL));
-            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, ca, s));
-            HIPCHK(c, ev_end());
+            HIPCHK(c, ev_kernel(n == S - 1 ? K_COLL : K_COL));
+            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, ca, s, ka0, ka1));
         }
         HIPCHK(c, ev_begin(K_RED));
         HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix.as<int32_t>(),
