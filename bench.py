@@ -177,12 +177,26 @@ def main():
     # HBM traffic of one full-batch launch from the committed counter
     # summary (FETCH_SIZE/WRITE_SIZE per position, copy-calibrated)
     traffic = None
+    lds = None
     pmc = os.path.join(ROOT, "profiles", f"{dom}_cuzr512_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
-            per_pos = json.load(f).get("hbm_bytes_per_position")
+            prof = json.load(f)
+        per_pos = prof.get("hbm_bytes_per_position")
         if per_pos:
             traffic = float(per_pos) * st["batch"]
+        # LDS bytes of one full launch (SQ_INSTS_LDS_*_BANDWIDTH counters) over
+        # this run's event-timed launch duration, against the LDS peak rates
+        pl = prof.get("lds")
+        if pl and us > 0:
+            rd = pl["read_bytes_per_launch"] / (us * 1e-6) / 1e12
+            wr = pl["write_bytes_per_launch"] / (us * 1e-6) / 1e12
+            lds = {"read_tbs": round(rd, 2), "write_tbs": round(wr, 2),
+                   "peak_read_tbs": pl["peak_read_tbs_published"],
+                   "peak_write_tbs": pl["peak_write_tbs_published"],
+                   "frac": round(rd / pl["peak_read_tbs_published"]
+                                 + wr / pl["peak_write_tbs_published"], 4),
+                   "busy_frac_counters": pl["busy_frac"]}
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "bytes_per_launch": nbytes, "avg_us": round(us, 2),
@@ -191,6 +205,7 @@ def main():
                    "last_col_pass": round(pst["last_col_pass_us"], 2),
                    "reduce": round(pst["reduce_us"], 2)}}
 
+    roofline["lds"] = lds
     copy_gbs = round(copy_bandwidth(), 1)
     roofline["copy_gbs"] = copy_gbs
     roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
