@@ -310,12 +310,18 @@ typedef struct msf_job {
     msf_fem_outputs fem_out;           /* optional FEM outputs */
 } msf_job;
 
-/* No reference counterpart: the reference runs one job per process.  A
- * context owns one worker thread (started by the first submission) that
- * runs queued jobs in submission order on the context's stream, each as
- * msf_specimen_build + msf_run_stem_fem + msf_specimen_free.  Direct calls
- * on the same context wait for the job in progress.  msf_destroy finishes
- * the queue before it returns. */
+/* No reference counterpart: the reference runs one job per process.
+ * Continuous batching: the first submission starts the context's serving
+ * engine, `slots` workers each with its own internal context (stream, run
+ * plan).  A free worker takes the job at the head of the queue once the
+ * head job's estimated device footprint (msf_job_footprint) fits beside the
+ * jobs already running within the memory limit, so jobs that arrive while
+ * others run start as slots and memory free up and run concurrently on the
+ * device.  A job larger than the limit runs when nothing else does.  Each
+ * job runs as msf_specimen_build + msf_run_stem_fem + msf_specimen_free with
+ * the submitting context's msf_config at submission; its results do not
+ * depend on what runs beside it.  Direct calls on the context do not wait
+ * for served jobs.  msf_destroy finishes the queue before it returns. */
 int msf_submit(msf_ctx* ctx, const msf_job* job, uint64_t* id);
 /* MSF_OK or the job's error code once finished, MSF_PENDING while queued or
  * running, MSF_ERR_INVALID for an unknown id */
@@ -323,6 +329,40 @@ int msf_poll(msf_ctx* ctx, uint64_t id);
 /* blocks until the job has finished; returns its status (the error text is
  * then available from msf_last_error) and forgets the job */
 int msf_wait(msf_ctx* ctx, uint64_t id);
+
+/* Timings of a served job: queue_ms from submission to start, run_ms from
+ * start to finish (specimen build, run, release), its admitted footprint and
+ * the worker slot that ran it. */
+typedef struct msf_job_info {
+    double queue_ms;
+    double run_ms;
+    uint64_t footprint;
+    uint32_t slot;
+} msf_job_info;
+/* msf_wait that also returns the job's timings (info may be NULL) */
+int msf_wait_info(msf_ctx* ctx, uint64_t id, msf_job_info* info);
+
+/* Serving engine settings, before the first msf_submit on the context:
+ * slots = jobs run at once (0: 3, at most 16); mem_limit = device bytes the
+ * running jobs may claim together by their estimated footprints (0: 80 % of
+ * the device memory free when the engine starts). */
+int msf_serve_configure(msf_ctx* ctx, uint32_t slots, uint64_t mem_limit);
+
+typedef struct msf_serve_stats {
+    uint32_t slots;
+    uint32_t running;                  /* jobs running now */
+    uint32_t max_running;              /* most jobs that ran at once */
+    uint64_t queued;                   /* jobs waiting for admission */
+    uint64_t jobs_done;
+    uint64_t mem_limit;
+    uint64_t admitted_bytes;           /* footprints of the running jobs */
+    uint64_t peak_admitted_bytes;
+} msf_serve_stats;
+int msf_serve_get_stats(msf_ctx* ctx, msf_serve_stats* out);
+
+/* The admission estimate of a job's device footprint in bytes (specimen
+ * stacks, vacuum waves, batch buffers, set-up temporaries, +10 %); host only. */
+uint64_t msf_job_footprint(const msf_job* job);
 
 #ifdef __cplusplus
 }

@@ -123,23 +123,42 @@ struct ServeJob {
     double* adf = nullptr;
     msf_fem_outputs fo{};
     std::vector<double> mtf, mtf_dom; // copies of the MTF table (fem.mtf* point here)
+    msf_config cfg{};                 // the submitting context's settings at submission
+    uint64_t footprint = 0;           // estimated device bytes (admission)
+    double t_submit = 0;
 };
 
+struct ServeDone {
+    int rc = MSF_OK;
+    std::string msg;
+    msf_job_info info{};
+};
+
+// Continuous batching: `slots` workers, each with its own internal context
+// (stream, run plan), take jobs from the head of the queue as soon as a slot
+// is free and the head job's estimated footprint fits beside the jobs
+// already running (a job runs alone if it exceeds the limit by itself).
 struct ServeState {
-    std::thread worker;
+    std::vector<std::thread> workers;
+    std::vector<msf_ctx*> slot_ctx;
     std::mutex m;
     std::condition_variable work_cv, done_cv;
     std::deque<ServeJob> queue;
     std::set<uint64_t> pending;                              // queued or running
-    std::map<uint64_t, std::pair<int, std::string>> done;    // status, message
+    std::map<uint64_t, ServeDone> done;
     uint64_t next_id = 1;
     bool stop = false;
+    uint32_t slots = 0;
+    uint64_t limit = 0, admitted = 0, peak_admitted = 0, jobs_done = 0;
+    uint32_t running = 0, max_running = 0;
 };
 
 struct msf_ctx {
     std::recursive_mutex mu;    // serialises device work of the context
-    std::mutex serve_mu;        // guards the creation of the serving worker
+    std::mutex serve_mu;        // guards the creation of the serving engine
     std::unique_ptr<ServeState> serve;
+    uint32_t serve_slots = 0;   // msf_serve_configure (0: automatic)
+    uint64_t serve_limit = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     // second stream and the events that order two batch chains (created on first use)
@@ -827,25 +846,36 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             (uint64_t)(uintptr_t)twy_nat, (uint64_t)ncp, (uint64_t)G, (uint64_t)nbins,
             (uint64_t)pl.n_tiles, (uint64_t)c->cfg.col_tile};
         graph_state = 2;
+        bool captured = true;
         if (!pl.graph || pl.graph_sig != sig) {
             pl.drop_graph();
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             const int r = enqueue();
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (r != MSF_OK) {
+            if (r != MSF_OK || e != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
-                return r;
+                // A capture can be invalidated from outside, e.g. by another
+                // thread's allocation while it runs; nothing captured has
+                // executed, so one chain simply launches on the stream.
+                if (chains != 1) return r != MSF_OK ? r : fail(c, MSF_ERR_DEVICE, hipGetErrorString(e));
+                (void)hipGetLastError();
+                captured = false;
+            } else {
+                const hipError_t ei = hipGraphInstantiate(&pl.graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) pl.graph = nullptr;
+                HIPCHK(c, ei);
+                pl.graph_sig = sig;
+                graph_state = 1;
             }
-            HIPCHK(c, e);
-            const hipError_t ei = hipGraphInstantiate(&pl.graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ei != hipSuccess) pl.graph = nullptr;
-            HIPCHK(c, ei);
-            pl.graph_sig = sig;
-            graph_state = 1;
         }
-        HIPCHK(c, hipGraphLaunch(pl.graph, c->stream));
+        if (captured) {
+            HIPCHK(c, hipGraphLaunch(pl.graph, c->stream));
+        } else {
+            graph_state = 0;
+            if (int r = enqueue()) return r;
+        }
     } else {
         if (int r = enqueue()) return r;
     }
@@ -937,13 +967,15 @@ int msf_destroy(msf_ctx* c)
 {
     if (!c) return MSF_ERR_INVALID;
     std::unique_lock<std::mutex> serve_lk(c->serve_mu);
-    if (c->serve) { // finish the queued jobs, then stop the worker
+    if (c->serve) { // finish the queued jobs, then stop the workers
         {
             std::lock_guard<std::mutex> lk(c->serve->m);
             c->serve->stop = true;
         }
         c->serve->work_cv.notify_all();
-        if (c->serve->worker.joinable()) c->serve->worker.join();
+        for (std::thread& w : c->serve->workers)
+            if (w.joinable()) w.join();
+        for (msf_ctx* sc : c->serve->slot_ctx) msf_destroy(sc);
         c->serve.reset();
     }
     serve_lk.unlock();
@@ -1620,43 +1652,121 @@ int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, cons
 // ---------------------------------------------------------------- serving
 namespace {
 
-void serve_loop(msf_ctx* c)
+double ms_since(double t0) { return now_ms() - t0; }
+
+// Device bytes a job claims while it runs (specimen, vacuum waves, batch
+// buffers of both chains, set-up temporaries), over-estimated by 10 %.
+uint64_t job_footprint(const ServeJob& j)
+{
+    const uint64_t M = (uint64_t)j.grid.nx * j.grid.ny;
+    const msf::Slices sl = msf::autoslice(j.q.data(), (uint32_t)j.Z.size(), j.grid.zmin,
+                                          j.grid.zperiod, j.min_slice);
+    const uint64_t S = std::max<uint64_t>(1, sl.lower.size());
+    const uint64_t P = (uint64_t)j.xp.size() * j.yp.size();
+    uint64_t wave = j.cfg.batch ? (uint64_t)j.cfg.batch * M * 8 : (112ull << 20);
+    wave = std::min<uint64_t>(wave, P * M * 8 + (16ull << 20));
+    uint64_t b = 40 * M * S                     // t (FP32 + FP64) and u0 stacks
+                 + 64 * M                       // probe, vacuum wave, per-pixel sums
+                 + 2 * wave                     // two chains of wave buffers
+                 + (256ull << 20);              // species maps of the set-up
+    if (j.mtf.size() && j.fem.mtf_sampling_rate != 0.0)
+        b += 2 * std::min<uint64_t>(P, (wave / 8 / std::max<uint64_t>(M, 1)) + 16) * M * 16;
+    return b + b / 10 + (16ull << 20);
+}
+
+// a context's cached run plan and its buffers, released between served jobs
+void reset_plan(msf_ctx* c)
+{
+    c->plan.~RunPlan();
+    new (&c->plan) RunPlan();
+}
+
+void serve_worker(msf_ctx* c, uint32_t slot)
 {
     ServeState& st = *c->serve;
+    msf_ctx* sc = st.slot_ctx[slot];
     for (;;) {
         ServeJob job;
+        double t_start = 0;
         {
             std::unique_lock<std::mutex> lk(st.m);
-            st.work_cv.wait(lk, [&] { return st.stop || !st.queue.empty(); });
+            st.work_cv.wait(lk, [&] {
+                if (st.queue.empty()) return st.stop;
+                return st.running == 0 || st.admitted + st.queue.front().footprint <= st.limit;
+            });
             if (st.queue.empty()) return; // stop requested and nothing left
             job = std::move(st.queue.front());
             st.queue.pop_front();
+            st.admitted += job.footprint;
+            st.running += 1;
+            st.max_running = std::max(st.max_running, st.running);
+            st.peak_admitted = std::max(st.peak_admitted, st.admitted);
+            t_start = now_ms();
         }
-        int rc;
-        std::string msg;
+        st.work_cv.notify_all(); // the next head job may fit as well
+        ServeDone d;
         {
-            MSF_LOCK(c);
             msf_specimen* sp = nullptr;
-            rc = msf_specimen_build(c, &job.grid, job.q.data(), job.Z.data(),
-                                    (uint32_t)job.Z.size(), job.min_slice, &sp);
+            d.rc = msf_set_config(sc, &job.cfg);
+            if (d.rc == MSF_OK)
+                d.rc = msf_specimen_build(sc, &job.grid, job.q.data(), job.Z.data(),
+                                          (uint32_t)job.Z.size(), job.min_slice, &sp);
             if (!job.mtf.empty()) {
                 job.fem.mtf = job.mtf.data();
                 job.fem.mtf_domain = job.mtf_dom.data();
             }
-            if (rc == MSF_OK)
-                rc = msf_run_stem_fem(c, sp, &job.optics, &job.det, &job.fem, job.xp.data(),
-                                      (uint32_t)job.xp.size(), job.yp.data(),
-                                      (uint32_t)job.yp.size(), job.adf, &job.fo);
-            if (rc != MSF_OK) msg = c->err;
+            if (d.rc == MSF_OK)
+                d.rc = msf_run_stem_fem(sc, sp, &job.optics, &job.det, &job.fem, job.xp.data(),
+                                        (uint32_t)job.xp.size(), job.yp.data(),
+                                        (uint32_t)job.yp.size(), job.adf, &job.fo);
+            if (d.rc != MSF_OK) d.msg = sc->err;
             if (sp) msf_specimen_free(sp);
+            std::lock_guard<std::recursive_mutex> lk(sc->mu);
+            reset_plan(sc);
         }
+        d.info.queue_ms = t_start - job.t_submit;
+        d.info.run_ms = ms_since(t_start);
+        d.info.footprint = job.footprint;
+        d.info.slot = slot;
         {
             std::lock_guard<std::mutex> lk(st.m);
+            st.admitted -= job.footprint;
+            st.running -= 1;
+            st.jobs_done += 1;
             st.pending.erase(job.id);
-            st.done[job.id] = {rc, msg};
+            st.done[job.id] = std::move(d);
         }
+        st.work_cv.notify_all();
         st.done_cv.notify_all();
     }
+}
+
+// starts the engine on first use: slot contexts on the same device, workers
+int start_serving(msf_ctx* c)
+{
+    std::lock_guard<std::mutex> lk(c->serve_mu);
+    if (c->serve) return MSF_OK;
+    std::unique_ptr<ServeState> st(new ServeState);
+    st->slots = c->serve_slots ? c->serve_slots : 3;
+    st->limit = c->serve_limit;
+    if (!st->limit) {
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            return MSF_ERR_DEVICE;
+        st->limit = (uint64_t)(0.8 * (double)free_b);
+    }
+    for (uint32_t k = 0; k < st->slots; ++k) {
+        msf_ctx* sc = nullptr;
+        if (int r = msf_create(&sc, c->device)) {
+            for (msf_ctx* x : st->slot_ctx) msf_destroy(x);
+            return r;
+        }
+        st->slot_ctx.push_back(sc);
+    }
+    c->serve = std::move(st);
+    for (uint32_t k = 0; k < c->serve->slots; ++k)
+        c->serve->workers.emplace_back(serve_worker, c, k);
+    return MSF_OK;
 }
 
 } // namespace
@@ -1686,12 +1796,16 @@ int msf_submit(msf_ctx* c, const msf_job* j, uint64_t* id)
     job.adf = j->adf_image;
     job.fo = j->fem_out;
     {
-        std::lock_guard<std::mutex> lk(c->serve_mu);
-        if (!c->serve) {
-            c->serve.reset(new ServeState);
-            c->serve->worker = std::thread(serve_loop, c);
-        }
+        MSF_LOCK(c);
+        job.cfg = c->cfg;
+        job.cfg.profile_kernels = 0;
+        // served jobs launch on their streams: a graph capture on one worker
+        // would be invalidated by another worker's allocations
+        job.cfg.no_graph = 1;
     }
+    job.footprint = job_footprint(job);
+    job.t_submit = now_ms();
+    if (int r = start_serving(c)) return r;
     ServeState& st = *c->serve;
     {
         std::lock_guard<std::mutex> lk(st.m);
@@ -1710,15 +1824,15 @@ int msf_poll(msf_ctx* c, uint64_t id)
     ServeState& st = *c->serve;
     std::lock_guard<std::mutex> lk(st.m);
     auto it = st.done.find(id);
-    if (it != st.done.end()) return it->second.first;
+    if (it != st.done.end()) return it->second.rc;
     return st.pending.count(id) ? MSF_PENDING : MSF_ERR_INVALID;
 }
 
-int msf_wait(msf_ctx* c, uint64_t id)
+int msf_wait_info(msf_ctx* c, uint64_t id, msf_job_info* info)
 {
     if (!c || !c->serve) return MSF_ERR_INVALID;
     ServeState& st = *c->serve;
-    std::pair<int, std::string> r;
+    ServeDone r;
     {
         std::unique_lock<std::mutex> lk(st.m);
         if (!st.done.count(id) && !st.pending.count(id)) return MSF_ERR_INVALID;
@@ -1726,11 +1840,59 @@ int msf_wait(msf_ctx* c, uint64_t id)
         r = st.done[id];
         st.done.erase(id);
     }
-    if (r.first != MSF_OK) {
+    if (info) *info = r.info;
+    if (r.rc != MSF_OK) {
         MSF_LOCK(c);
-        c->err = "job " + std::to_string(id) + ": " + r.second;
+        c->err = "job " + std::to_string(id) + ": " + r.msg;
     }
-    return r.first;
+    return r.rc;
+}
+
+int msf_wait(msf_ctx* c, uint64_t id) { return msf_wait_info(c, id, nullptr); }
+
+int msf_serve_configure(msf_ctx* c, uint32_t slots, uint64_t mem_limit)
+{
+    if (!c) return MSF_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(c->serve_mu);
+    if (c->serve) return fail(c, MSF_ERR_INVALID, "the serving engine has already started");
+    if (slots > 16) return fail(c, MSF_ERR_INVALID, "slots must be 0..16");
+    c->serve_slots = slots;
+    c->serve_limit = mem_limit;
+    return MSF_OK;
+}
+
+int msf_serve_get_stats(msf_ctx* c, msf_serve_stats* out)
+{
+    if (!c || !out) return MSF_ERR_INVALID;
+    *out = msf_serve_stats{};
+    std::lock_guard<std::mutex> lk0(c->serve_mu);
+    if (!c->serve) return MSF_OK;
+    ServeState& st = *c->serve;
+    std::lock_guard<std::mutex> lk(st.m);
+    out->slots = st.slots;
+    out->running = st.running;
+    out->max_running = st.max_running;
+    out->queued = st.queue.size();
+    out->jobs_done = st.jobs_done;
+    out->mem_limit = st.limit;
+    out->admitted_bytes = st.admitted;
+    out->peak_admitted_bytes = st.peak_admitted;
+    return MSF_OK;
+}
+
+uint64_t msf_job_footprint(const msf_job* j)
+{
+    if (!j || (j->n_atoms && (!j->q_xyz || !j->Z))) return 0;
+    ServeJob job;
+    job.grid = j->grid;
+    job.q.assign(j->q_xyz, j->q_xyz + 3 * (size_t)j->n_atoms);
+    job.Z.assign(j->Z, j->Z + j->n_atoms);
+    job.min_slice = j->min_slice;
+    job.xp.assign(j->x_p, j->x_p + j->n_xp);
+    job.yp.assign(j->y_p, j->y_p + j->n_yp);
+    job.fem = j->fem;
+    if (j->fem.mtf && j->fem.mtf_n) job.mtf.assign(j->fem.mtf, j->fem.mtf + j->fem.mtf_n);
+    return job_footprint(job);
 }
 
 } // extern "C"

@@ -106,6 +106,17 @@ class Ticket:
         self.id, self.img, self.outs, self._keep = job_id, img, outs, keep
 
 
+class JobInfo(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("footprint", C.c_uint64),
+                ("slot", C.c_uint32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("running", C.c_uint32), ("max_running", C.c_uint32),
+                ("queued", C.c_uint64), ("jobs_done", C.c_uint64), ("mem_limit", C.c_uint64),
+                ("admitted_bytes", C.c_uint64), ("peak_admitted_bytes", C.c_uint64)]
+
+
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
                 ("profile_kernels", C.c_int32), ("col_tile", C.c_int32),
@@ -139,7 +150,8 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_scherzer_cs3", "msf_probe_shifts", "msf_exit_waves_shifted", "msf_run_stem_fem",
            "msf_submit", "msf_poll", "msf_wait", "msf_specimen_propagator", "msf_probe_k",
            "msf_slice_step_batch", "msf_correlograph_bins", "msf_adf_reduce",
-           "msf_fem_accumulate", "msf_mtf_2d", "msf_apply_mtf"]
+           "msf_fem_accumulate", "msf_mtf_2d", "msf_apply_mtf", "msf_wait_info",
+           "msf_serve_configure", "msf_serve_get_stats", "msf_job_footprint"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -204,6 +216,11 @@ def lib():
         L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
         L.msf_poll.argtypes = [vp, C.c_uint64]
         L.msf_wait.argtypes = [vp, C.c_uint64]
+        L.msf_wait_info.argtypes = [vp, C.c_uint64, C.POINTER(JobInfo)]
+        L.msf_serve_configure.argtypes = [vp, C.c_uint32, C.c_uint64]
+        L.msf_serve_get_stats.argtypes = [vp, C.POINTER(ServeStats)]
+        L.msf_job_footprint.argtypes = [C.POINTER(Job)]
+        L.msf_job_footprint.restype = C.c_uint64
         L.msf_scherzer_cs3.argtypes = [C.c_double, C.c_double]
         L.msf_scherzer_cs3.restype = C.c_double
         # host-side readers and writers (include/msf_io.h)
@@ -346,6 +363,18 @@ def fem_bins(grid, dr):
     return edges[:n.value]
 
 
+def job_footprint(grid, q, Z, min_slice, x_p, y_p, fem=None):
+    """msf_job_footprint: the serving engine's admission estimate in bytes."""
+    q = np.ascontiguousarray(q, np.float64).reshape(-1)
+    Z = np.ascontiguousarray(Z, np.uint32)
+    x_p = np.ascontiguousarray(x_p, np.float64)
+    y_p = np.ascontiguousarray(y_p, np.float64)
+    job = Job(grid, q.ctypes.data, Z.ctypes.data, Z.size, min_slice, Optics(), x_p.ctypes.data,
+              x_p.size, y_p.ctypes.data, y_p.size, Detector(), fem if fem is not None else Fem(),
+              None, FemOutputs())
+    return lib().msf_job_footprint(C.byref(job))
+
+
 def correlograph_bins(grid, dr):
     """msf_correlograph_bins: (n_phi, k boundaries [n_bins + 1], phi boundaries [n_phi + 1])."""
     n = C.c_uint32()
@@ -467,7 +496,7 @@ class Context:
     def submit(self, grid, q, Z, min_slice, optics, x_p, y_p, detector=None, fem=None,
                modes=("v_omega",)):
         """msf_submit: queue a whole job (specimen build + run) on the
-        context's worker thread; returns a Ticket for poll() / wait()."""
+        context's serving engine; returns a Ticket for poll() / wait()."""
         q = np.ascontiguousarray(q, np.float64).reshape(-1)
         Z = np.ascontiguousarray(Z, np.uint32)
         x_p = np.ascontiguousarray(x_p, np.float64)
@@ -485,18 +514,34 @@ class Context:
                   img.ctypes.data if img is not None else None, fo)
         jid = C.c_uint64()
         self._check(lib().msf_submit(self._h, C.byref(job), C.byref(jid)), "msf_submit")
-        return Ticket(jid.value, img, outs, (q, Z, x_p, y_p, nb))
+        t = Ticket(jid.value, img, outs, (q, Z, x_p, y_p, nb))
+        t.footprint = lib().msf_job_footprint(C.byref(job))
+        return t
 
     def poll(self, ticket):
         return lib().msf_poll(self._h, ticket.id)
 
-    def wait(self, ticket):
-        """Returns (adf image [n_xp, n_yp] or None, {mode: values})."""
-        self._check(lib().msf_wait(self._h, ticket.id), "msf_wait")
+    def wait(self, ticket, info=False):
+        """Returns (adf image [n_xp, n_yp] or None, {mode: values}), and with
+        info the job's timings (msf_wait_info) as a third element."""
+        ji = JobInfo()
+        self._check(lib().msf_wait_info(self._h, ticket.id, C.byref(ji)), "msf_wait")
         img = ticket.img
         nx, ny, nb = ticket._keep[2].size, ticket._keep[3].size, ticket._keep[4]
-        return ((img.reshape(nx, ny) if img is not None else None),
-                {m: v[:nb] for m, v in ticket.outs.items()})
+        res = ((img.reshape(nx, ny) if img is not None else None),
+               {m: v[:nb] for m, v in ticket.outs.items()})
+        if info:
+            return res + ({k: getattr(ji, k) for k, _ in JobInfo._fields_},)
+        return res
+
+    def serve_configure(self, slots=0, mem_limit=0):
+        """msf_serve_configure (before the first submit)."""
+        self._check(lib().msf_serve_configure(self._h, slots, mem_limit), "msf_serve_configure")
+
+    def serve_stats(self):
+        st = ServeStats()
+        self._check(lib().msf_serve_get_stats(self._h, C.byref(st)), "msf_serve_get_stats")
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_}
 
     def probe_k(self, grid, optics, sx, sy):
         out = np.zeros((grid.nx, grid.ny), np.complex128)

@@ -82,3 +82,70 @@ def test_destroy_finishes_the_queue(msf):
                    detector=c.detector(msf))
     own.close()                                   # joins the worker after the job ran
     assert np.isfinite(t.img).all() and t.img.max() > 0
+
+
+def _submit_all(ctx, msf, jobs):
+    return [ctx.submit(c.grid(msf), c.q, c.Z, c.min_slice, c.optics(msf), xp, xp,
+                       detector=c.detector(msf), fem=c.femcfg(msf), modes=("v_omega", "v_re"))
+            for c, xp in jobs]
+
+
+def test_continuous_batching_overlaps_jobs_and_matches_alone_runs(msf):
+    """Jobs submitted together run at the same time on several slots, and
+    each result is bit-identical to the same job served alone."""
+    jobs = _jobs(9, seed=3100)
+    mix = msf.Context(0)
+    alone = msf.Context(0)
+    try:
+        mix.serve_configure(slots=3)
+        alone.serve_configure(slots=1)
+        got = [mix.wait(t, info=True) for t in _submit_all(mix, msf, jobs)]
+        st = mix.serve_stats()
+        print("mixed:", st, flush=True)
+        assert st["jobs_done"] == 9 and st["running"] == 0 and st["queued"] == 0
+        assert st["max_running"] >= 2                      # the work overlapped
+        assert len({info["slot"] for _, _, info in got}) >= 2
+        for (c, xp), (img, fem, info) in zip(jobs, got):
+            t = _submit_all(alone, msf, [(c, xp)])[0]
+            img_a, fem_a = alone.wait(t)
+            np.testing.assert_array_equal(img, img_a)
+            for m in fem:
+                np.testing.assert_array_equal(fem[m], fem_a[m])
+            assert info["run_ms"] > 0 and info["queue_ms"] >= 0
+        assert alone.serve_stats()["max_running"] == 1
+    finally:
+        mix.close()
+        alone.close()
+
+
+def test_admission_by_footprint(msf):
+    """With a memory limit of about two jobs' footprints, at most two run at
+    once and the admitted footprints never exceed the limit; with a limit
+    below any job, jobs run one at a time."""
+    jobs = _jobs(6, seed=3200)
+    fps = [msf.job_footprint(c.grid(msf), c.q, c.Z, c.min_slice, xp, xp, fem=c.femcfg(msf))
+           for c, xp in jobs]
+    assert min(fps) > 0
+    limit = int(2.05 * max(fps))
+    ctx2 = msf.Context(0)
+    ctx1 = msf.Context(0)
+    try:
+        ctx2.serve_configure(slots=4, mem_limit=limit)
+        ts = _submit_all(ctx2, msf, jobs)
+        assert [t.footprint for t in ts] == fps
+        for t in ts:
+            ctx2.wait(t)
+        st = ctx2.serve_stats()
+        print("limited:", st, flush=True)
+        assert st["max_running"] <= 2 and st["peak_admitted_bytes"] <= limit
+        ctx1.serve_configure(slots=4, mem_limit=1)
+        for t in _submit_all(ctx1, msf, jobs[:3]):
+            img, _ = ctx1.wait(t)
+            assert np.isfinite(img).all()
+        st1 = ctx1.serve_stats()
+        assert st1["max_running"] == 1 and st1["jobs_done"] == 3
+        with pytest.raises(msf.MsfError):
+            ctx1.serve_configure(slots=2)               # only before the first submission
+    finally:
+        ctx2.close()
+        ctx1.close()
