@@ -109,6 +109,48 @@ def copy_bandwidth(nbytes=1 << 30, reps=5):
     return 2.0 * nbytes / (best * 1e-3) / 1e9
 
 
+def serving_sample(msf, SP, n_jobs, rate, slots, seed=5000):
+    """A bounded serving run: n_jobs of a seeded mixed stream submitted at
+    their Poisson arrival times to a context with `slots` serving slots;
+    per-job latency = queue + run time (msf_wait_info)."""
+    jobs = SP.serving_mix(n_jobs, seed=seed, rate=rate)
+    ctx = msf.Context(0)
+    try:
+        ctx.serve_configure(slots=slots)
+        tickets = []
+        work = 0
+        t0 = time.perf_counter()
+        for j in jobs:
+            delay = j["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            grid = msf.make_grid(j["nx"], j["ny"], j["cell"][0], j["cell"][1], j["cell"][2],
+                                 j["voltage"])
+            optics = msf.Optics(0, 0.0, 0.0, 0.0, j["alpha_max"])
+            det = msf.Detector(1, j["det_inner"], j["det_outer"], msf.ADF_REFERENCE)
+            tickets.append(ctx.submit(grid, j["q"], j["Z"], j["min_slice"], optics, j["x_p"],
+                                      j["y_p"], detector=det, fem=msf.Fem(1, j["dr"])))
+            work += j["x_p"].size * j["y_p"].size * j["slices"] * (j["nx"] * j["ny"]) / 512 ** 2
+        lat = []
+        for t in tickets:
+            _, _, info = ctx.wait(t, info=True)
+            lat.append(info["queue_ms"] + info["run_ms"])
+        span = time.perf_counter() - t0
+        st = ctx.serve_stats()
+    finally:
+        ctx.close()
+    lat = np.array(lat)
+    return {"jobs": n_jobs, "arrival_rate_per_s": rate, "slots": slots,
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
+                           "p99": round(float(np.percentile(lat, 99)), 1)},
+            "throughput_jobs_per_s": round(n_jobs / span, 2),
+            "throughput_512eq_props_per_s": round(work / span, 1),
+            "makespan_s": round(span, 2), "max_running": st["max_running"],
+            "peak_admitted_gib": round(st["peak_admitted_bytes"] / 2 ** 30, 2),
+            "mix": "N 256/512/1024 (0.5/0.35/0.15), 10-50 slices, rasters 16-64 per side, "
+                   "Si or Cu/Zr, Poisson arrivals; 512eq = probe-slices x (N/512)^2"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +161,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=24,
+                    help="jobs of the bounded serving sample (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run hipMemcpy device-to-device of this size first (counter scale)")
     args = ap.parse_args()
@@ -225,6 +269,13 @@ def main():
                  "speedup_batched_vs_plain": round(value / (n * S / dt), 2)}
         log(f"plain: {plain['value']:.4g} props/s")
 
+    serving = None
+    if args.serving_jobs > 0:
+        log(f"serving sample: {args.serving_jobs} jobs, Poisson 4/s, 3 slots")
+        serving = serving_sample(msf, SP, args.serving_jobs, 4.0, 3)
+        log(f"serving: p50 {serving['latency_ms']['p50']} ms, p99 {serving['latency_ms']['p99']} ms, "
+            f"{serving['throughput_jobs_per_s']} jobs/s, max {serving['max_running']} running")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -253,6 +304,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": plain,
+        "serving": serving,
         "job_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                            "p99": round(float(np.percentile(lat, 99)), 3)},
         "slice_loop_ms": round(st["slice_loop_ms"], 3),

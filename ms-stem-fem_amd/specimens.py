@@ -135,3 +135,33 @@ def config_cuzr_512(seed=2001):
                det_outer=0.200, dr=0.1414213562)
     xp = raster(32, 40.0)
     return job, q, Z, xp, xp.copy()
+
+
+def serving_mix(n_jobs, seed=5000, rate=4.0):
+    """Seeded job stream in the manner of a serving workload: Poisson
+    arrivals at `rate` jobs/s; N in {256, 512, 1024} with weights
+    0.5/0.35/0.15 at 0.078125 A per pixel; slices U{10..50} from a cell depth
+    of 2 A per slice (minslice 2.0); raster side U{16..64} over the cell;
+    amorphous Si (0.05 A^-3) or Cu/Zr (p = 0.5); 300 kV, 20 mrad, ADF
+    60-200 mrad and FEM V_omega.  Atom positions are uniform without the
+    minimum separation: the device work depends only on grid, slices and
+    raster.  Returns a list of dicts (arrival time t in s)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for k in range(n_jobs):
+        t += float(rng.exponential(1.0 / rate))
+        N = int(rng.choice([256, 512, 1024], p=[0.5, 0.35, 0.15]))
+        L = N * 0.078125
+        S = int(rng.integers(10, 51))
+        depth = 2.0 * S
+        side = int(rng.integers(16, 65))
+        cuzr = bool(rng.random() < 0.5)
+        n_at = int(round((0.05 if not cuzr else 3500 / 40.0 ** 3) * L * L * depth))
+        q = rng.random((n_at, 3)) * [L, L, depth]
+        Z = (np.where(rng.random(n_at) < 0.5, 29, 40) if cuzr else np.full(n_at, 14)).astype(np.uint32)
+        jobs.append(dict(t=t, nx=N, ny=N, cell=(L, L, depth), q=q, Z=Z, slices=S,
+                         x_p=raster(side, L), y_p=raster(side, L), voltage=300e3,
+                         alpha_max=0.020, det_inner=0.060, det_outer=0.200, dr=0.1414213562,
+                         min_slice=2.0))
+    return jobs
