@@ -228,7 +228,7 @@ def main():
             prof = json.load(f)
         per_pos = prof.get("hbm_bytes_per_position")
         if per_pos:
-            traffic = float(per_pos) * st["batch"]
+            traffic = float(per_pos) * pst["batch"]   # the profiled launches' batch
         # LDS bytes of one full launch (SQ_INSTS_LDS_*_BANDWIDTH counters) over
         # this run's event-timed launch duration, against the LDS peak rates
         pl = prof.get("lds")
@@ -300,7 +300,8 @@ def main():
                                "Cs3 1.2 mm Scherzer, 2 mrad, 32x32 raster, V_omega",
                    "grid": 512, "atoms": int(Z.size), "slices": int(S), "positions": int(P),
                    "batch": int(st["batch"]), "chains": int(st["chains"]),
-                   "graph": int(st["graph"]), "seed": 2001},
+                   "col_tile": int(st["col_tile"]), "graph": int(st["graph"]),
+                   "autotuned": int(st["autotuned"]), "seed": 2001},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": plain,

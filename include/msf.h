@@ -105,6 +105,10 @@ typedef struct msf_config {
                                           results bit-identical); 0 = automatic (two chains on
                                           the streams for launches of >= 4M pixels, else one
                                           chain as a graph) */
+    int32_t no_autotune;               /* 0: when batch, col_tile and chains are all automatic,
+                                          time the candidates on the job's own specimen once per
+                                          shape (nx, ny, kept columns, positions up to 256) and
+                                          keep the fastest; 1: use the fixed rules */
 } msf_config;
 
 /* Timings of the last msf_run_stem() on the context. */
@@ -121,6 +125,8 @@ typedef struct msf_stats {
     double row_pass_bytes, col_pass_bytes; /* compulsory HBM bytes of one launch */
     int32_t graph;                     /* 0: stream launches; 1: graph captured; 2: graph replayed */
     int32_t chains;                    /* batch chains in flight (msf_config.chains) */
+    int32_t col_tile;                  /* columns per column-pass workgroup used */
+    int32_t autotuned;                 /* 0: fixed rules; 1: tuned in this call; 2: tuned earlier */
 } msf_stats;
 
 /* ------------------------------------------------------------ context */

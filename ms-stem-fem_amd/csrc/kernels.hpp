@@ -57,6 +57,8 @@ bool slice_lengths_ok(int n);
 bool stage_twiddles(int n, std::vector<double>& re_im);
 // the column count of the compact wave is padded to a multiple of this
 int col_tile(int nx);
+// the column-pass tile width used when none is configured
+int col_tile_default(int nx);
 // true when the kept set leaves [3n/8, 5n/8) empty along the row (ka, kb) and
 // every row i in [3nx/8, 5nx/8) fully masked (keep ranges)
 bool row_prunable(int ny, int ka, int kb);

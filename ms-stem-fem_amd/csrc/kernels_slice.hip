@@ -383,6 +383,18 @@ int col_tile(int nx)
     }
 }
 
+int col_tile_default(int nx)
+{
+    switch (nx) {
+#define MSF_CASE(N) case N: return ColTile<N>::W;
+        MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
+        MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
+#undef MSF_CASE
+    default:
+        return 0;
+    }
+}
+
 // With ev0/ev1 the dispatch packet itself stamps the kernel's start and end
 // (hipExtLaunchKernelGGL), so profiled durations carry no marker overhead.
 template <int NY>

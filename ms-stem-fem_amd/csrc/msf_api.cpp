@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -165,9 +166,11 @@ struct msf_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fem[2] = {nullptr, nullptr};
     std::string err;
-    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0, 0};
+    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0, 0, 0};
     msf_stats stats{};
     std::map<int, DevBuf> tw32, tw64, twnat;
+    // autotuned launch shape per (nx, ny, ncp, min(P, 256)): batch, column tile, chains
+    std::map<std::array<int, 4>, std::array<int, 3>> tuned;
     RunPlan plan;
     std::set<msf_specimen*> specimens; // live specimens built on this context
 };
@@ -582,6 +585,125 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         if (B > 16 && B % 16) B = std::min<int64_t>(bmax, B + 16 - B % 16);
     }
     B = std::max<int64_t>(1, std::min<int64_t>(B, P));
+    int col_tile = c->cfg.col_tile;
+    int chains_cfg = c->cfg.chains;
+
+    // Autotuner: with batch, column tile and chains all automatic, time a few
+    // slice steps of this job's own specimen and shifts for each candidate
+    // (batch depth, column tile, one or two chains) on trial buffers once per
+    // shape, and keep the fastest per position-slice (the faster of two
+    // timings each) if it beats the rule above by more than 3 %.  Every
+    // candidate gives bit-identical results.
+    int tuned_state = 0;
+    const bool prof0 = c->cfg.profile_kernels != 0;
+    if (!c->cfg.no_autotune && c->cfg.batch == 0 && c->cfg.col_tile == 0 && c->cfg.chains == 0 &&
+        !prof0 && !out.exit && P >= 2) {
+        const std::array<int, 4> key = {nx, ny, ncp, (int)std::min<int64_t>(P, 256)};
+        auto it = c->tuned.find(key);
+        if (it == c->tuned.end()) {
+            const int64_t per_pos = plane * (int64_t)sizeof(float2);
+            const int64_t bcap = std::max<int64_t>(1, (384ll << 20) / (2 * per_pos));
+            std::vector<int64_t> bs;
+            for (int64_t b : {16, 32, 48, 64, 96, 128})
+                if (b <= P && b <= bcap) bs.push_back(b);
+            if (std::find(bs.begin(), bs.end(), B) == bs.end()) bs.push_back(B);
+            const int64_t bmx = *std::max_element(bs.begin(), bs.end());
+            const int64_t ns = std::min<int64_t>(P, 2 * bmx);
+            DevBuf tpsi, tsh;
+            HIPCHK(c, tpsi.alloc((size_t)2 * bmx * per_pos));
+            HIPCHK(c, hipMemsetAsync(tpsi.p, 0, tpsi.bytes, c->stream));
+            std::vector<int2> sh(shifts.begin(), shifts.begin() + ns);
+            HIPCHK(c, upload(tsh, sh, c->stream));
+            if (!c->stream2) {
+                HIPCHK(c, hipStreamCreate(&c->stream2));
+                for (hipEvent_t* e : {&c->ev_fork, &c->ev_join, &c->ev_fem[0], &c->ev_fem[1]})
+                    HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+            }
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            HIPCHK(c, hipEventCreate(&e0));
+            HIPCHK(c, hipEventCreate(&e1));
+            msf::RowPassArgs tr{};
+            tr.nx = nx; tr.ncp = ncp; tr.ka = ka; tr.kb = kb; tr.tw = twy;
+            tr.inv_s = (float)(1.0 / d.sqrt_nxny);
+            tr.prune = msf::row_prunable(ny, ka, kb) ? 1 : 0;
+            msf::ColPassArgs tc{};
+            tc.ny = ny; tc.ncp = ncp; tc.ka = ka; tc.kb = kb; tc.keep = sp->dkeep.as<int2>();
+            tc.tw = twx; tc.psi0 = pl.Psi0.as<float2>(); tc.twx_nat = twx_nat; tc.twy = twy_nat;
+            tc.prune = msf::col_prunable(nx, keep.ab.data()) ? 1 : 0;
+            const int T = 3;
+            // one trial: T steady-state slice steps on `ch` chains of b positions
+            auto trial = [&](int64_t b, int w, int ch, float& ms) -> int {
+                hipStream_t st[2] = {c->stream, c->stream2};
+                auto steps = [&](int count) -> int {
+                    if (ch == 2) {
+                        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+                        HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+                    }
+                    for (int k = 0; k < ch; ++k) {
+                        tr.psi = tc.psi = tpsi.as<float2>() + (int64_t)k * bmx * plane;
+                        tr.shift = tc.shift = tsh.as<int2>() + k * b;
+                        tr.batch = tc.batch = (int)b;
+                        tc.tile = w;
+                        for (int n = 1; n <= count; ++n) {
+                            const int sl = n % S;
+                            tr.t = sp->t32.as<float2>() + (int64_t)sl * M;
+                            tr.u0 = pl.U0.as<float2>() + (int64_t)sl * 2 * M;
+                            tc.px = sp->px.as<float2>() + (int64_t)sl * nx;
+                            tc.py = sp->py.as<float2>() + (int64_t)sl * ny;
+                            HIPCHK(c, msf::launch_row_pass(ny, false, tmode, tr, st[k]));
+                            HIPCHK(c, msf::launch_col_pass(nx, false, tc, st[k]));
+                        }
+                    }
+                    if (ch == 2) {
+                        HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
+                        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+                    }
+                    return MSF_OK;
+                };
+                if (int r = steps(1)) return r; // warm-up
+                ms = 0.f;
+                for (int rep = 0; rep < 2; ++rep) { // the faster of two timings
+                    HIPCHK(c, hipEventRecord(e0, c->stream));
+                    if (int r = steps(T)) return r;
+                    HIPCHK(c, hipEventRecord(e1, c->stream));
+                    HIPCHK(c, hipEventSynchronize(e1));
+                    float t = 0.f;
+                    HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+                    if (rep == 0 || t < ms) ms = t;
+                }
+                return MSF_OK;
+            };
+            const int tile0 = msf::col_tile_default(nx);
+            const int def_ch = (B * plane >= (4ll << 20) && (P + B - 1) / B >= 2) ? 2 : 1;
+            double best = 0.0, def_cost = 0.0;
+            std::array<int, 3> pick = {(int)B, tile0, def_ch};
+            int rc = MSF_OK;
+            for (int64_t b : bs)
+                for (int w : {8, 16})
+                    for (int ch : {1, 2}) {
+                        if (ch == 2 && 2 * b > ns) continue;
+                        float ms = 0.f;
+                        if ((rc = trial(b, w, ch, ms))) break;
+                        const double cost = ms / ((double)b * ch * T);
+                        if (b == B && w == tile0 && ch == def_ch) def_cost = cost;
+                        if (best == 0.0 || cost < best) {
+                            best = cost;
+                            pick = {(int)b, w, ch};
+                        }
+                    }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (rc) return rc;
+            if (def_cost > 0.0 && best > def_cost / 1.03) pick = {(int)B, tile0, def_ch};
+            it = c->tuned.emplace(key, pick).first;
+            tuned_state = 1;
+        } else {
+            tuned_state = 2;
+        }
+        B = std::min<int64_t>(it->second[0], P);
+        col_tile = it->second[1];
+        chains_cfg = it->second[2];
+    }
 
     // work buffers, grown as needed and kept for the next run
     // Two chains of batches in flight on two streams: the row passes (VALU
@@ -594,7 +716,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const bool prof = c->cfg.profile_kernels != 0;
     const int64_t n_batches = (P + B - 1) / B;
     const bool big_launch = B * plane >= (4ll << 20);
-    const int chains_req = c->cfg.chains ? c->cfg.chains : (big_launch ? 2 : 1);
+    const int chains_req = chains_cfg ? chains_cfg : (big_launch ? 2 : 1);
     const int chains = (chains_req == 2 && n_batches >= 2 && !prof && !out.exit) ? 2 : 1;
     const bool auto_eager = c->cfg.chains == 0 && chains == 2;
     if (B > pl.cap_b || pl.psi.bytes < (size_t)B * plane * sizeof(float2)) {
@@ -717,7 +839,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ca.psi0 = dPsi0.as<float2>();
     ca.twx_nat = twx_nat;
     ca.twy = twy_nat;
-    ca.tile = c->cfg.col_tile;
+    ca.tile = col_tile;
     ca.prune = msf::col_prunable(nx, keep.ab.data()) ? 1 : 0;
     const float2* t32 = sp->t32.as<float2>();
     const float2* px = sp->px.as<float2>();
@@ -844,7 +966,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             (uint64_t)(uintptr_t)dcnt.p, (uint64_t)(uintptr_t)dzero.p,
             (uint64_t)(uintptr_t)twx, (uint64_t)(uintptr_t)twy, (uint64_t)(uintptr_t)twx_nat,
             (uint64_t)(uintptr_t)twy_nat, (uint64_t)ncp, (uint64_t)G, (uint64_t)nbins,
-            (uint64_t)pl.n_tiles, (uint64_t)c->cfg.col_tile};
+            (uint64_t)pl.n_tiles, (uint64_t)col_tile};
         graph_state = 2;
         bool captured = true;
         if (!pl.graph || pl.graph_sig != sig) {
@@ -919,6 +1041,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     st.col_pass_bytes = st.row_pass_bytes;
     st.graph = graph_state;
     st.chains = chains;
+    st.col_tile = col_tile ? col_tile : msf::col_tile_default(nx);
+    st.autotuned = tuned_state;
     if (prof) {
         double sum[K_NUM] = {0}, cnt[K_NUM] = {0};
         for (size_t k = 0; k < ev_kind.size(); ++k) {

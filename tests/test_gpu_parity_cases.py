@@ -309,3 +309,38 @@ def test_reduction_entry_points_match_oracle(msf, ctx, O):
     assert rel_l2(v_g, v_o) < 1e-9
     with pytest.raises(msf.MsfError):
         ctx.adf_reduce(g, msf.Detector(1, -0.1, 0.15, 0), w)
+
+
+def test_autotuned_launch_shape_gives_identical_results(msf, ctx):
+    """The autotuner times batch depths, column tiles and one or two chains
+    on the job itself once per shape; whatever it picks, ADF and V_omega are
+    bit-identical to the fixed rules and to explicit settings."""
+    q, Z = random_atoms(80, (12.0, 12.0, 6.0), seed=23, species=(14, 29))
+    c = Case(nx=128, ny=128, cell=(12.0, 12.0, 6.0), voltage=300e3, q=q, Z=Z, min_slice=1.5,
+             alpha_max=0.02, inner=0.03, outer=0.2, fem=True, dr=0.7)
+    spec = c.gpu_specimen(msf, ctx)
+    xp = np.linspace(0.0, 11.5, 12)
+    yp = np.linspace(0.2, 11.0, 10)
+    run = lambda: ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
+                              fem=c.femcfg(msf))
+    try:
+        ctx.configure(autotune=False)
+        img0, v0 = run()
+        assert ctx.stats()["autotuned"] == 0
+        ctx.configure()
+        img1, v1 = run()
+        st = ctx.stats()
+        print("tuned:", st["batch"], st["col_tile"], st["chains"], flush=True)
+        assert st["autotuned"] == 1 and st["col_tile"] in (8, 16)
+        img2, v2 = run()
+        assert ctx.stats()["autotuned"] == 2                 # kept per shape
+        outs = [(img1, v1), (img2, v2)]
+        for tile in (8, 16):
+            ctx.configure(col_tile=tile)
+            outs.append(run())
+            assert ctx.stats()["col_tile"] == tile
+        for img, v in outs:
+            np.testing.assert_array_equal(img, img0)
+            np.testing.assert_array_equal(v, v0)
+    finally:
+        ctx.configure()
