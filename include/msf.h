@@ -105,10 +105,10 @@ typedef struct msf_config {
                                           results bit-identical); 0 = automatic (two chains on
                                           the streams for launches of >= 4M pixels, else one
                                           chain as a graph) */
-    int32_t no_autotune;               /* 0: when batch, col_tile and chains are all automatic,
+    int32_t autotune;                  /* 1: when batch, col_tile and chains are all automatic,
                                           time the candidates on the job's own specimen once per
                                           shape (nx, ny, kept columns, positions up to 256) and
-                                          keep the fastest; 1: use the fixed rules */
+                                          keep the fastest; 0 (default): the fixed rules */
 } msf_config;
 
 /* Timings of the last msf_run_stem() on the context. */

@@ -588,25 +588,28 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     int col_tile = c->cfg.col_tile;
     int chains_cfg = c->cfg.chains;
 
-    // Autotuner: with batch, column tile and chains all automatic, time a few
-    // slice steps of this job's own specimen and shifts for each candidate
-    // (batch depth, column tile, one or two chains) on trial buffers once per
-    // shape, and keep the fastest per position-slice (the faster of two
-    // timings each) if it beats the rule above by more than 3 %.  Every
-    // candidate gives bit-identical results.
+    // Autotuner (opt-in): with batch, column tile and chains all automatic, time one
+    // step through every slice of this job's own specimen and shifts for each
+    // candidate (batch depth around the rule's, column tile, one or two
+    // chains) on trial buffers once per shape, and keep the fastest per
+    // position-slice (the faster of two timings each) if it beats the rule
+    // above by more than 3 %.  Every candidate gives bit-identical results.
     int tuned_state = 0;
     const bool prof0 = c->cfg.profile_kernels != 0;
-    if (!c->cfg.no_autotune && c->cfg.batch == 0 && c->cfg.col_tile == 0 && c->cfg.chains == 0 &&
+    if (c->cfg.autotune && c->cfg.batch == 0 && c->cfg.col_tile == 0 && c->cfg.chains == 0 &&
         !prof0 && !out.exit && P >= 2) {
         const std::array<int, 4> key = {nx, ny, ncp, (int)std::min<int64_t>(P, 256)};
         auto it = c->tuned.find(key);
         if (it == c->tuned.end()) {
             const int64_t per_pos = plane * (int64_t)sizeof(float2);
             const int64_t bcap = std::max<int64_t>(1, (384ll << 20) / (2 * per_pos));
-            std::vector<int64_t> bs;
-            for (int64_t b : {16, 32, 48, 64, 96, 128})
-                if (b <= P && b <= bcap) bs.push_back(b);
-            if (std::find(bs.begin(), bs.end(), B) == bs.end()) bs.push_back(B);
+            // around the rule's depth: B/2, B, 3B/2, 2B in whole groups of 16
+            std::vector<int64_t> bs = {B};
+            for (int64_t b : {B / 2, (3 * B) / 2, 2 * B}) {
+                if (B >= 16) b = std::max<int64_t>(16, b - b % 16);
+                if (b >= 1 && b <= P && b <= bcap && std::find(bs.begin(), bs.end(), b) == bs.end())
+                    bs.push_back(b);
+            }
             const int64_t bmx = *std::max_element(bs.begin(), bs.end());
             const int64_t ns = std::min<int64_t>(P, 2 * bmx);
             DevBuf tpsi, tsh;
@@ -630,8 +633,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             tc.ny = ny; tc.ncp = ncp; tc.ka = ka; tc.kb = kb; tc.keep = sp->dkeep.as<int2>();
             tc.tw = twx; tc.psi0 = pl.Psi0.as<float2>(); tc.twx_nat = twx_nat; tc.twy = twy_nat;
             tc.prune = msf::col_prunable(nx, keep.ab.data()) ? 1 : 0;
-            const int T = 3;
-            // one trial: T steady-state slice steps on `ch` chains of b positions
+            // one trial: a steady-state step through every slice of the stack
+            // (so t and u0 stream as in the job) on `ch` chains of b positions
+            const int T = S;
             auto trial = [&](int64_t b, int w, int ch, float& ms) -> int {
                 hipStream_t st[2] = {c->stream, c->stream2};
                 auto steps = [&](int count) -> int {
@@ -678,6 +682,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             double best = 0.0, def_cost = 0.0;
             std::array<int, 3> pick = {(int)B, tile0, def_ch};
             int rc = MSF_OK;
+            {
+                float ms = 0.f; // bring clocks and caches to steady state first
+                rc = trial(B, tile0, 1, ms);
+            }
             for (int64_t b : bs)
                 for (int w : {8, 16})
                     for (int ch : {1, 2}) {

@@ -120,7 +120,7 @@ class ServeStats(C.Structure):
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
                 ("profile_kernels", C.c_int32), ("col_tile", C.c_int32),
-                ("no_graph", C.c_int32), ("chains", C.c_int32), ("no_autotune", C.c_int32)]
+                ("no_graph", C.c_int32), ("chains", C.c_int32), ("autotune", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -426,9 +426,9 @@ class Context:
             raise MsfError(rc, f"{what}: {lib().msf_last_error(self._h).decode()}")
 
     def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False,
-                  col_tile=0, graph=True, chains=0, autotune=True):
+                  col_tile=0, graph=True, chains=0, autotune=False):
         cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0, col_tile,
-                     0 if graph else 1, chains, 0 if autotune else 1)
+                     0 if graph else 1, chains, 1 if autotune else 0)
         self._check(lib().msf_set_config(self._h, C.byref(cfg)), "msf_set_config")
 
     def stats(self):
