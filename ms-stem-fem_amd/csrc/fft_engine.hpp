@@ -247,7 +247,7 @@ MSF_PLAN(256, 8, 8, 8, 4)
 MSF_PLAN(384, 12, 6, 4, 4, 4)
 MSF_PLAN(512, 8, 8, 8, 8)
 MSF_PLAN(768, 24, 6, 8, 4, 4)
-MSF_PLAN(1024, 16, 8, 8, 4, 4)
+MSF_PLAN(1024, 8, 8, 8, 4, 4)
 #undef MSF_PLAN
 
 // entries of the concatenated stage tables: sum (R-1) NS over stages 1.. = N - R_0

@@ -130,7 +130,7 @@ row_pass_kernel(RowPassArgs a)
                 x[p] = (lo || up) ? v : float2{0.f, 0.f};
             }
         }
-        P::template run<true, float2, true>(x, lds, s_tw, t);
+        P::template run<true, float2, (T <= 64)>(x, lds, s_tw, t);
     }
 
     const int2 sh = a.shift[b];
@@ -158,7 +158,7 @@ row_pass_kernel(RowPassArgs a)
         }
     }
 
-    P::template run<false, float2, true>(x, lds, s_tw, t);
+    P::template run<false, float2, (T <= 64)>(x, lds, s_tw, t);
 
     if (active) {
 #pragma unroll
@@ -201,25 +201,44 @@ template <int NX, int W> __host__ __device__ constexpr int col_len()
     return base + ((want - base % 16) + 16) % 16;
 }
 
+// Lean tiles: when the twiddle, propagator and keep tables would keep two
+// workgroups from sharing a CU's 160 KB of LDS (1024 rows: a 74 KB tile),
+// the tables are read from global memory (L2-resident, shared by every
+// workgroup) and only the tile stays in LDS.
+template <int NX, int W> __host__ __device__ constexpr bool col_lean()
+{
+    return 2 * ((size_t)W * col_len<NX, W>() * 8 + (size_t)stage_tw_len<NX>() * 8 +
+                (size_t)NX * 16) > 160 * 1024;
+}
+
 template <int NX, bool LAST, int W, bool PRUNE>
 __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 {
     using P = Plan<NX>;
     constexpr int E = P::E, T = P::T;
     constexpr int LEN = col_len<NX, W>();
-    static_assert(64 % T == 0, "a column transform must fit in one wavefront");
-    __shared__ float2 s_tw[stage_tw_len<NX>()];
+    constexpr bool LEAN = col_lean<NX, W>();
+    // a column transform lies in one wavefront (wave-local exchanges) or
+    // spans whole wavefronts (workgroup barriers)
+    static_assert(64 % T == 0 || T % 64 == 0, "a column transform must align with wavefronts");
+    constexpr bool WAVE = T <= 64;
+    __shared__ float2 s_tw[LEAN ? 1 : stage_tw_len<NX>()];
     __shared__ float2 s_nat[LAST ? NX : 1]; // roots W_nx^e for the exit phase ramp
-    __shared__ float2 s_px[NX];
-    __shared__ int2 s_keep[NX];
+    __shared__ float2 s_px[LEAN ? 1 : NX];
+    __shared__ int2 s_keep[LEAN ? 1 : NX];
     __shared__ float2 s_buf[W * LEN];
 
     for (int e = threadIdx.x; e < NX; e += blockDim.x) {
-        if (e < stage_tw_len<NX>()) s_tw[e] = a.tw[e];
+        if constexpr (!LEAN) {
+            if (e < stage_tw_len<NX>()) s_tw[e] = a.tw[e];
+            s_px[e] = a.px[e];
+            s_keep[e] = a.keep[e];
+        }
         if constexpr (LAST) s_nat[e] = a.twx_nat[e];
-        s_px[e] = a.px[e];
-        s_keep[e] = a.keep[e];
     }
+    const float2* tw = LEAN ? a.tw : s_tw;
+    const float2* tpx = LEAN ? a.px : s_px;
+    const int2* tkeep = LEAN ? a.keep : s_keep;
 
     // neighbouring tiles share 128-byte lines (W = 8 reads 64 B per row): keep
     // them on one XCD, next to each other in dispatch order
@@ -257,7 +276,7 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 #pragma unroll
     for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
 
-    P::template run<false, float2, true>(x, lds, s_tw, t);
+    P::template run<false, float2, WAVE>(x, lds, tw, t);
 
     // band limit (bw_limit + propagate's k^2 test, decided on the host) and
     // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
@@ -269,16 +288,16 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
             continue;
         }
         const int i = t + T * p;
-        const int2 kr = s_keep[i];
+        const int2 kr = tkeep[i];
         const bool keep = (j < kr.x) || (j >= a.ny - kr.y);
-        const float2 pxi = s_px[i];
+        const float2 pxi = tpx[i];
         const float2 pr = float2{pxi.x * pyj.x - pxi.y * pyj.y, pxi.x * pyj.y + pxi.y * pyj.x};
         const float2 f = x[p];
         x[p] = keep ? float2{pr.x * f.x - pr.y * f.y, pr.x * f.y + pr.y * f.x} : float2{0.f, 0.f};
     }
 
     if constexpr (!LAST) {
-        P::template run<true, float2, true>(x, lds, s_tw, t);
+        P::template run<true, float2, WAVE>(x, lds, tw, t);
     } else if (active) {
         // add the vacuum exit wave, shifted: psi0(k) e^{-2 pi i (i sx / nx + j sy / ny)}
         const int2 sh = a.shift[b];
@@ -295,7 +314,7 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
         }
     }
 
-    lds_sync<true>();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int p = 0; p < E; ++p) lds[lds_pad(t + T * p)] = x[p];
     __syncthreads();
@@ -461,8 +480,9 @@ static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s, hip
 
 template <int NX, int W> constexpr bool col_fits()
 {
-    return (size_t)W * col_len<NX, W>() * 8 + (size_t)NX * 24 <= 160 * 1024 &&
-           W * Plan<NX>::T <= 1024;
+    const size_t tables = col_lean<NX, W>() ? (size_t)NX * 8  // s_nat of the last slice
+                                             : (size_t)NX * 24 + (size_t)stage_tw_len<NX>() * 8;
+    return (size_t)W * col_len<NX, W>() * 8 + tables <= 160 * 1024 && W * Plan<NX>::T <= 1024;
 }
 
 template <int NX>
