@@ -264,7 +264,10 @@ int msf_probe_shifts(const msf_grid* grid, const double* x_p, uint32_t n_xp, con
  *       ii + jj * n_bins for ring ii and phi sector jj; per position
  *       c = count_ii * S_ii,jj / S_ii - 1 (S: sums of |psi| over the sector
  *       and the ring) where that ratio is >= 1, else 0; the mean of c and
- *       the sum of c^2 over positions. */
+ *       the sum of c^2 over positions.
+ *   correlograph_each   (--correlograph_everyimage / _everytxt, adfstem.cpp:2655-2685):
+ *       n_xp * n_yp * n_bins * n_phi doubles, c of every position in raster
+ *       order, [p][ii + jj * n_bins]; computed with the correlograph. */
 typedef struct msf_fem_outputs {
     double* v_omega;
     double* vbar_r;
@@ -274,6 +277,7 @@ typedef struct msf_fem_outputs {
     double* v_rbar;
     double* correlograph_avg;
     double* correlograph_sqr_sum;
+    double* correlograph_each;
 } msf_fem_outputs;
 
 /* msf_run_stem with every FEM mode above; fem->enabled must be set for any

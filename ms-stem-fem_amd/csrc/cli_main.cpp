@@ -35,7 +35,7 @@ struct Flags {
     bool bfctem_corrected = false, bfctem_uncorrected = false;
     bool fem = false, d1 = false, d2 = false, d3 = false, d4 = false, gt17 = false, rva = false;
     bool correlograph = false, image_output = false, diffraction_output = false;
-    bool correlograph_variance = false, correlograph_every = false;
+    bool correlograph_variance = false, correlograph_everyimage = false, correlograph_everytxt = false;
     bool netcdf_images = false, netcdf_variance = false, pap_tif = false, dupe = false;
     bool mtf_file = false, mtf_resolution = false;
     bool lammps_pre = false, lammps_steps = false, lammps_samples = false;
@@ -147,8 +147,8 @@ bool read_parameter_file(const std::string& name, Flags& f, Params& p)
         else if (key == "netcdfvariance") f.netcdf_variance = true;
         else if (key == "correlograph") f.correlograph = true;
         else if (key == "correlograph_variance") f.correlograph = f.correlograph_variance = true;
-        else if (key == "correlograph_everyimage" || key == "correlograph_everytxt")
-            f.correlograph = f.correlograph_every = true;
+        else if (key == "correlograph_everyimage") f.correlograph = f.correlograph_everyimage = true;
+        else if (key == "correlograph_everytxt") f.correlograph = f.correlograph_everytxt = true;
         else if (key == "lammps_pretem_file_name") f.lammps_pre = true;
         else if (key == "lammps_tem_steps") f.lammps_steps = true;
         else if (key == "lammps_tem_samples") f.lammps_samples = true;
@@ -248,8 +248,8 @@ void read_cmdline(const std::vector<std::string>& args, Flags& f, Params& p)
         else if (a == "--netcdfvariance") f.netcdf_variance = true;
         else if (a == "--correlograph") f.correlograph = true;
         else if (a == "--correlograph_variance") f.correlograph = f.correlograph_variance = true;
-        else if (a == "--correlograph_everyimage" || a == "--correlograph_everytxt")
-            f.correlograph = f.correlograph_every = true;
+        else if (a == "--correlograph_everyimage") f.correlograph = f.correlograph_everyimage = true;
+        else if (a == "--correlograph_everytxt") f.correlograph = f.correlograph_everytxt = true;
         else if (a == "--lammps_preTEM_file_name") { f.lammps_pre = true; i += 1; }
         else if (a == "--lammps_TEM_steps") { f.lammps_steps = true; i += 1; }
         else if (a == "--lammps_TEM_samples") { f.lammps_samples = true; i += 1; }
@@ -287,7 +287,6 @@ bool check_supported(const Flags& f)
 {
     struct Item { bool on; const char* what; };
     const Item items[] = {
-        {f.correlograph_every, "--correlograph_everyimage / --correlograph_everytxt"},
         {f.bfctem_corrected || f.bfctem_uncorrected, "--bfctem*"},
         {f.pap_tif, "--paptif"},
         {f.diffraction_output, "diffractionimages"},
@@ -344,21 +343,30 @@ int write_diffraction(const std::string& prefix, const float* psi, long nx, long
 // variance name) and log and linear images of the variance over the k bins
 // from the one holding 0.2 up to, not including, the one holding 1.0 1/A.
 // Empty images are skipped (the reference reads element 0 of an empty array).
+// output_correlograph_image (write_tif.cpp:1607-1654): min-max 16-bit TIFF,
+// phi along x (w columns), k along y (h rows); the resolution tags come from
+// the full bin counts nk and nphi (adfstem.cpp:2651-2654)
+int write_correlograph_image(const double* v, uint32_t w, uint32_t h, const std::string& name,
+                             int lg, uint32_t nk, uint32_t nphi)
+{
+    if ((size_t)w * h == 0) return MSF_OK;
+    const double kres = (nk / (1.0 - 0.2)) * 1.0e8;
+    const double phires = (double)(nphi / 360);  // integer division, as in the reference
+    double mn = v[0], mx = v[0];
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        if (v[i] > mx) mx = v[i];
+        if (v[i] < mn) mn = v[i];
+    }
+    return msf_write_tiff16(name.c_str(), v, w, h, mn, mx, lg, 3, phires, kres);
+}
+
 int write_correlograph(const std::string& out, const std::vector<double>& avg,
                        const std::vector<double>& sqr, const std::vector<double>& k_edges,
                        const std::vector<double>& phi_edges, uint32_t nk, uint32_t nphi,
                        size_t P, bool variance)
 {
-    const double kres = (nk / (1.0 - 0.2)) * 1.0e8; // adfstem.cpp:2651-2654
-    const double phires = (double)(nphi / 360);      // integer division, as in the reference
     auto image = [&](const double* v, uint32_t w, uint32_t h, const std::string& name, int lg) {
-        if ((size_t)w * h == 0) return MSF_OK;
-        double mn = v[0], mx = v[0];
-        for (size_t i = 0; i < (size_t)w * h; ++i) {
-            if (v[i] > mx) mx = v[i];
-            if (v[i] < mn) mn = v[i];
-        }
-        return msf_write_tiff16(name.c_str(), v, w, h, mn, mx, lg, 3, phires, kres);
+        return write_correlograph_image(v, w, h, name, lg, nk, nphi);
     };
     int w = msf_write_correlograph_txt((out + "_correlograph_avg.txt").c_str(), avg.data(),
                                        k_edges.data(), nk, phi_edges.data(), nphi);
@@ -596,8 +604,11 @@ int main(int argc, char** argv)
             }
         const size_t ncorr = std::max<size_t>((size_t)nbins * nphi, 1);
         std::vector<double> c_avg(corr ? ncorr : 0), c_sqr(corr ? ncorr : 0);
+        const bool every = corr && (f.correlograph_everyimage || f.correlograph_everytxt);
+        std::vector<double> c_each(every ? xp.size() * yp.size() * ncorr : 0);
         const msf_fem_outputs fo{ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5],
-                                 corr ? c_avg.data() : nullptr, corr ? c_sqr.data() : nullptr};
+                                 corr ? c_avg.data() : nullptr, corr ? c_sqr.data() : nullptr,
+                                 every ? c_each.data() : nullptr};
         if (msf_run_stem_fem(c, spec, &o, &det, &fem, xp.data(), (uint32_t)xp.size(), yp.data(),
                              (uint32_t)yp.size(), img.data(), &fo) != MSF_OK)
             return die("adfstem", c);
@@ -611,6 +622,23 @@ int main(int argc, char** argv)
                 msf_write_variance_txt((out + files[m].suffix + ".txt").c_str(), files[m].v.data(),
                                        edges.data(), nbins))
                 std::cout << " failed to write variance data to a txt file : " << out << std::endl;
+        // every position's correlograph, <prefix>_<x index>_<y index>.tif / .txt
+        // (adfstem.cpp:2655-2685)
+        for (size_t ix = 0; every && ix < xp.size(); ++ix)
+            for (size_t iy = 0; iy < yp.size(); ++iy) {
+                const double* cp = c_each.data() + (ix * yp.size() + iy) * ncorr;
+                const std::string name = out + "_" + std::to_string(ix) + "_" + std::to_string(iy);
+                int w = MSF_OK;
+                if (f.correlograph_everyimage && nbins * nphi > 0)
+                    w |= write_correlograph_image(cp, nphi, nbins, name + ".tif", 0, nbins, nphi);
+                if (f.correlograph_everytxt)
+                    w |= msf_write_correlograph_txt((name + ".txt").c_str(), cp, k_edges.data(),
+                                                    nbins, phi_edges.data(), nphi);
+                if (w) {
+                    std::cerr << "Error writing the correlograph: " << msf_io_last_error() << std::endl;
+                    rc = EXIT_FAILURE;
+                }
+            }
         if (corr && write_correlograph(out, c_avg, c_sqr, k_edges, phi_edges, nbins, nphi,
                                        xp.size() * yp.size(), f.correlograph_variance)) {
             std::cerr << "Error writing the correlograph: " << msf_io_last_error() << std::endl;

@@ -106,11 +106,12 @@ hipError_t launch_mtf_magnitude_f64(double2* a, int64_t m, int count, const uint
 hipError_t launch_mtf_compact(const double2* full, int nx, int ny, int ncp, int ka, int kb,
                               const int2* keep, double s, int batch, float2* psi, hipStream_t st);
 hipError_t launch_div_f64(double2* a, int64_t n, double d, hipStream_t s);
-// correlograph sums of c and c^2 per (ring, phi sector) over a batch
+// correlograph sums of c and c^2 per (ring, phi sector) over a batch; with
+// `each`, also c of every position of the batch ([b][ii + jj * nbins])
 hipError_t launch_fem_correlograph(const double* partial, int groups, int batch, int nbins,
                                    int nphi, int off_abs, int off_corr, const int32_t* counts,
                                    const uint8_t* zeroed, double* csum, double* csqr,
-                                   hipStream_t s);
+                                   double* each, hipStream_t s);
 // per-pixel FP64 sums of |psi| and |psi|^2 over the positions of a batch
 hipError_t launch_pixel_moments(const float2* psi, int64_t plane, int batch, double* s1,
                                 double* s2, hipStream_t s);

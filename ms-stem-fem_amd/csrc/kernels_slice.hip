@@ -682,7 +682,7 @@ hipError_t launch_fem_vbar(const double* partial, int groups, int batch, int nbi
 __global__ void fem_correlograph_kernel(const double* partial, int groups, int batch, int nbins,
                                         int nphi, int off_abs, int off_corr,
                                         const int32_t* counts, const uint8_t* zeroed,
-                                        double* csum, double* csqr)
+                                        double* csum, double* csqr, double* each)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x; // ii * nphi + jj
     if (k >= nbins * nphi) return;
@@ -699,6 +699,7 @@ __global__ void fem_correlograph_kernel(const double* partial, int groups, int b
         if ((t >= 1) && (c != 0) && (I != 0) && (r != 0)) v = t - 1;
         a += v;
         q += v * v;
+        if (each) each[(int64_t)b * nbins * nphi + o] = v; // per position (optional)
     }
     csum[o] = a;
     csqr[o] = q;
@@ -707,12 +708,13 @@ __global__ void fem_correlograph_kernel(const double* partial, int groups, int b
 hipError_t launch_fem_correlograph(const double* partial, int groups, int batch, int nbins,
                                    int nphi, int off_abs, int off_corr, const int32_t* counts,
                                    const uint8_t* zeroed, double* csum, double* csqr,
-                                   hipStream_t s)
+                                   double* each, hipStream_t s)
 {
     if (nbins <= 0 || nphi <= 0 || batch <= 0) return hipSuccess;
     const int n = nbins * nphi;
     hipLaunchKernelGGL(fem_correlograph_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partial,
-                       groups, batch, nbins, nphi, off_abs, off_corr, counts, zeroed, csum, csqr);
+                       groups, batch, nbins, nphi, off_abs, off_corr, counts, zeroed, csum, csqr,
+                       each);
     return hipGetLastError();
 }
 

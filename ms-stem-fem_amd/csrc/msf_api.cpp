@@ -87,6 +87,7 @@ struct RunPlan {
     DevBuf psi2, part2, segp2, mtf_full2;     // the second batch chain's work buffers
     DevBuf vsum, pm1, pm2;                    // Vbar_r sums; per-pixel |psi|, |psi|^2 sums
     DevBuf csum, csqr;                        // correlograph sums of c and c^2
+    DevBuf ceach;                             // correlograph of every position
     // detector MTF: the table it was built from, the 2-D MTF [nx][ny], the
     // outside-aperture map and the full-grid FP64 work planes [B][nx][ny]
     bool mtf_on = false;
@@ -348,6 +349,7 @@ struct RunOut {
     double* v_rbar = nullptr;   // rva
     double* corr_avg = nullptr; // correlograph, [ii + jj * nbins]
     double* corr_sqr = nullptr;
+    double* corr_each = nullptr; // [p][ii + jj * nbins]
     float* exit = nullptr;
 };
 
@@ -425,7 +427,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const bool want_d1 = fem_on && out.fem;
     const bool want_d2 = fem_on && out.vbar_r;
     const bool want_pix = fem_on && (out.v_re || out.omega_vimage || out.v_gt17 || out.v_rbar);
-    const bool want_corr = fem_on && (out.corr_avg || out.corr_sqr);
+    const bool want_corr = fem_on && (out.corr_avg || out.corr_sqr || out.corr_each);
     const bool want_fem = want_d1 || want_d2 || want_pix || want_corr;
     // the detector MTF changes only what the FEM statistics see
     const bool want_mtf = want_fem && fem->mtf && fem->mtf_domain && fem->mtf_n >= 1 &&
@@ -768,6 +770,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         if (int r = twiddles(c, nx, nullptr, &twx64)) return r;
         if (int r = twiddles(c, ny, nullptr, &twy64)) return r;
     }
+    const bool want_each = want_corr && out.corr_each && nbins * nphi > 0;
+    if (want_each && pl.ceach.bytes < (size_t)P * nbins * nphi * sizeof(double))
+        HIPCHK(c, pl.ceach.alloc((size_t)P * nbins * nphi * sizeof(double)));
     if (want_corr && pl.csum.bytes < ncorr * sizeof(double)) {
         HIPCHK(c, pl.csum.alloc(ncorr * sizeof(double)));
         HIPCHK(c, pl.csqr.alloc(ncorr * sizeof(double)));
@@ -926,7 +931,11 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             HIPCHK(c, msf::launch_fem_correlograph(bpart, G, nb, nbins, nphi, pl.hg.fem_abs,
                                                    pl.hg.corr, dcnt.as<int32_t>(),
                                                    dzero.as<uint8_t>(), pl.csum.as<double>(),
-                                                   pl.csqr.as<double>(), s));
+                                                   pl.csqr.as<double>(),
+                                                   want_each ? pl.ceach.as<double>() +
+                                                                   b0 * nbins * nphi
+                                                             : nullptr,
+                                                   s));
         if (want_pix)
             HIPCHK(c, msf::launch_pixel_moments(bpsi, plane, nb, pl.pm1.as<double>(),
                                                 pl.pm2.as<double>(), s));
@@ -959,6 +968,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                 (uint64_t)want_corr << 3,
             (uint64_t)(uintptr_t)pl.csum.p, (uint64_t)(uintptr_t)pl.csqr.p, (uint64_t)nphi,
             (uint64_t)pl.hg.corr, (uint64_t)want_mtf, (uint64_t)(uintptr_t)pl.mtf2d.p,
+            (uint64_t)want_each, (uint64_t)(uintptr_t)pl.ceach.p,
             (uint64_t)chains, (uint64_t)(uintptr_t)pl.psi2.p, (uint64_t)(uintptr_t)pl.part2.p,
             (uint64_t)(uintptr_t)pl.segp2.p, (uint64_t)(uintptr_t)pl.mtf_full2.p,
             (uint64_t)(uintptr_t)pl.mtf_out.p, (uint64_t)(uintptr_t)pl.mtf_full.p,
@@ -1035,6 +1045,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             for (size_t k = 0; k < n; ++k) out.corr_avg[k] = cs[k] / (double)P;
         if (out.corr_sqr)
             HIPCHK(c, hipMemcpy(out.corr_sqr, pl.csqr.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (want_each)
+            HIPCHK(c, hipMemcpy(out.corr_each, pl.ceach.p, (size_t)P * n * sizeof(double),
+                                hipMemcpyDeviceToHost));
     }
 
     msf_stats st{};
@@ -1732,6 +1745,7 @@ int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         out.v_rbar = fo->v_rbar;
         out.corr_avg = fo->correlograph_avg;
         out.corr_sqr = fo->correlograph_sqr_sum;
+        out.corr_each = fo->correlograph_each;
     }
     return run_internal(c, sp, o, det, fem, shifts, out);
 }

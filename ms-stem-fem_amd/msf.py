@@ -88,7 +88,8 @@ FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
 
 class FemOutputs(C.Structure):
     _fields_ = [(m, C.c_void_p) for m in FEM_MODES] + [("correlograph_avg", C.c_void_p),
-                                                       ("correlograph_sqr_sum", C.c_void_p)]
+                                                       ("correlograph_sqr_sum", C.c_void_p),
+                                                       ("correlograph_each", C.c_void_p)]
 
 
 class Job(C.Structure):
@@ -465,7 +466,7 @@ class Context:
         return img, v
 
     def adfstem_fem(self, spec, optics, x_p, y_p, fem, modes=FEM_MODES, detector=None,
-                    correlograph=False):
+                    correlograph=False, correlograph_each=False):
         """msf_run_stem_fem: returns (adf image or None, {mode: values [n_bins]});
         with correlograph, also "correlograph_avg" and "correlograph_sqr_sum"
         shaped [n_phi, n_bins]."""
@@ -482,6 +483,9 @@ class Context:
             for m in ("correlograph_avg", "correlograph_sqr_sum"):
                 outs[m] = np.zeros(max(nb * nphi, 1))
                 ptrs.append(outs[m].ctypes.data)
+            if correlograph_each:
+                outs["correlograph_each"] = np.zeros(max(x_p.size * y_p.size * nb * nphi, 1))
+                ptrs.append(outs["correlograph_each"].ctypes.data)
         fo = FemOutputs(*ptrs)
         self._check(lib().msf_run_stem_fem(self._h, spec._h, C.byref(optics), C.byref(det),
                                            C.byref(fem), _p(x_p), x_p.size, _p(y_p), y_p.size,
@@ -492,6 +496,10 @@ class Context:
         if correlograph:
             for m in ("correlograph_avg", "correlograph_sqr_sum"):
                 res[m] = outs[m][:nb * nphi].reshape(nphi, nb)
+            if correlograph_each:
+                P = x_p.size * y_p.size
+                res["correlograph_each"] = outs["correlograph_each"][:P * nb * nphi].reshape(
+                    P, nphi, nb)
         return img, res
 
     def submit(self, grid, q, Z, min_slice, optics, x_p, y_p, detector=None, fem=None,

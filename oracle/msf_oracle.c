@@ -780,7 +780,8 @@ int orc_correlograph_nphi(double delta_k, int nbins)
  * non-zero, else 0; avg = sum_c / P and sqr = sum_c^2, index ii + jj*nbins. */
 int orc_correlograph(const double* mag2, long P, const double* kx, const double* ky, int Nx,
                      int Ny, double delta_k, int nbins, double lambda, double alpha_max,
-                     const double* radial1, const int32_t* counts1, double* avg, double* sqr)
+                     const double* radial1, const int32_t* counts1, double* avg, double* sqr,
+                     double* each)
 {
     const size_t M = (size_t)Nx * Ny;
     const int nphi = orc_correlograph_nphi(delta_k, nbins);
@@ -832,6 +833,7 @@ int orc_correlograph(const double* mag2, long P, const double* kx, const double*
                 const size_t ci = (size_t)ii + (size_t)jj * nbins;
                 avg[ci] += c;
                 if (sqr) sqr[ci] += c * c;
+                if (each) each[(size_t)p * NC + ci] = c; /* adfstem.cpp:2655-2685 */
             }
     }
     for (size_t c = 0; c < NC; c++) avg[c] = avg[c] / P;
@@ -997,14 +999,14 @@ int orc_adfstem(const orc_job* job, const orc_specimen* spec,
                 int n_threads, double* adf_image, double* fem_v, double* exit_psi)
 {
     return orc_adfstem_fem(job, spec, x_p, nxp, y_p, nyp, n_threads, adf_image, fem_v, NULL,
-                           NULL, NULL, NULL, NULL, NULL, NULL, exit_psi);
+                           NULL, NULL, NULL, NULL, NULL, NULL, NULL, exit_psi);
 }
 
 int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                     const double* x_p, int nxp, const double* y_p, int nyp, int n_threads,
                     double* adf_image, double* fem_v, double* vbar_r, double* v_re,
                     double* omega_vimage, double* v_gt17, double* v_rbar, double* corr_avg,
-                    double* corr_sqr, double* exit_psi)
+                    double* corr_sqr, double* corr_each, double* exit_psi)
 {
     const int Nx = job->nx, Ny = job->ny;
     const size_t M = (size_t)Nx * Ny;
@@ -1086,7 +1088,7 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                       counts, vbar_r, v_re, omega_vimage, v_gt17, v_rbar);
         if (corr_avg)
             orc_correlograph(mag2, P, kx, ky, Nx, Ny, delta_k, nbins, lambda, job->alpha_max,
-                             radial, counts, corr_avg, corr_sqr);
+                             radial, counts, corr_avg, corr_sqr, corr_each);
         free(mag2);
     }
     free(kx); free(ky); free(xx); free(yy); free(probe); free(sx); free(sy); free(mtf2d);

@@ -129,7 +129,7 @@ int orc_adfstem_fem(const orc_job* job, const orc_specimen* spec,
                     const double* x_p, int nxp, const double* y_p, int nyp, int n_threads,
                     double* adf_image, double* fem_v, double* vbar_r, double* v_re,
                     double* omega_vimage, double* v_gt17, double* v_rbar, double* corr_avg,
-                    double* corr_sqr, double* exit_psi);
+                    double* corr_sqr, double* corr_each, double* exit_psi);
 int orc_correlograph_nphi(double delta_k, int nbins);
 /* the 2-D detector MTF (adfstem.cpp:300-439, tem.cpp:276-285), complex [nx][ny] */
 int orc_mtf_2d(const double* mtf, const double* mtf_domain, int n, double rate, const double* kx,
@@ -140,7 +140,8 @@ int orc_apply_mtf(double* psi, const double* mtf2d, const double* kx, const doub
                   int ny, double lambda, double alpha_max, double bwcutoff_t);
 int orc_correlograph(const double* mag2, long n_positions, const double* kx, const double* ky,
                      int nx, int ny, double delta_k, int nbins, double lambda, double alpha_max,
-                     const double* radial1, const int32_t* counts1, double* avg, double* sqr);
+                     const double* radial1, const int32_t* counts1, double* avg, double* sqr,
+                     double* each);
 
 #ifdef __cplusplus
 }

@@ -103,14 +103,15 @@ def lib():
         L.orc_slice_step.argtypes = [_cp, _cp, _cp, _cp, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                      C.c_int]
         L.orc_adfstem_fem.argtypes = [C.POINTER(Job), C.POINTER(_Spec), _dp, C.c_int, _dp,
-                                      C.c_int, C.c_int] + [C.c_void_p] * 10
+                                      C.c_int, C.c_int] + [C.c_void_p] * 11
         L.orc_correlograph_nphi.argtypes = [C.c_double, C.c_int]
         L.orc_mtf_2d.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, _dp, C.c_int, C.c_int,
                                  C.c_double, C.c_double, C.c_double, _dp]
         L.orc_apply_mtf.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                     C.c_double, C.c_double]
         L.orc_correlograph.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, C.c_int, C.c_double,
-                                       C.c_int, C.c_double, C.c_double, _dp, _i32, _dp, _dp]
+                                       C.c_int, C.c_double, C.c_double, _dp, _i32, _dp, _dp,
+                                       C.c_void_p]
         L.orc_fem_modes.argtypes = [_dp, C.c_long, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                     C.c_int, C.c_double, C.c_double, _dp, _i32] + [C.c_void_p] * 5
         _lib = L
@@ -360,7 +361,8 @@ def adfstem(job, spec, x_p, y_p, n_threads=0, exit_waves=False):
 FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
 
 
-def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=False):
+def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=False,
+                correlograph_each=False):
     """Whole raster with the FEM modes of adfstem.cpp:2489-3605: returns
     (adf image or None, {mode: values [nbins]})."""
     x_p = np.ascontiguousarray(x_p, np.float64)
@@ -372,15 +374,17 @@ def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=
     img = np.zeros(P) if job.adf else None
     outs = {m: np.zeros(max(nb, 1)) for m in modes}
     ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
-    corr = sqr = None
+    corr = sqr = each = None
     if correlograph:
         _, dk = fem_nbins(kx, ky, job.dr, bwt)
         nphi = lib().orc_correlograph_nphi(dk, nb)
         corr = np.zeros(max(nb * nphi, 1))
         sqr = np.zeros(max(nb * nphi, 1))
+        if correlograph_each:
+            each = np.zeros(max(P * nb * nphi, 1))
     rc = lib().orc_adfstem_fem(C.byref(job), C.byref(spec._s), x_p, x_p.size, y_p, y_p.size,
                                n_threads, ptr(img), *[ptr(outs.get(m)) for m in FEM_MODES],
-                               ptr(corr), ptr(sqr), None)
+                               ptr(corr), ptr(sqr), ptr(each), None)
     assert rc == 0, rc
     if img is not None:
         img = img.reshape(x_p.size, y_p.size)
@@ -389,6 +393,8 @@ def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=
         # stored [ii + jj * nbins]: k bin fastest
         res["correlograph_avg"] = corr[:nb * nphi].reshape(nphi, nb)
         res["correlograph_sqr_sum"] = sqr[:nb * nphi].reshape(nphi, nb)
+        if correlograph_each:
+            res["correlograph_each"] = each[:P * nb * nphi].reshape(P, nphi, nb)
     return img, res
 
 

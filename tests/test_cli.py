@@ -113,7 +113,7 @@ def test_dupe_replicates_for_fem_only(tmp_path, model):
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorr",
       "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--bogus"], "unknown flag"),
     (["-m", "64", "64", "3e5", "-a", "MODEL", "--output_prefix", "o", "--adfstemuncorrfem",
-      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--correlograph_everytxt"],
+      "--cs3", "0", "--defocus", "0", "--alphamax", "0.01", "--paptif"],
      "mode outside the build"),
     (["-m", "64", "64", "3e5", "-a", "missing.xyz", "--output_prefix", "o", "--adfstemuncorr",
       "--cs3", "0", "--defocus", "0", "--alphamax", "0.01"], "unreadable model"),
@@ -170,8 +170,8 @@ def test_correlograph_flags(tmp_path, model):
     rc, plan, _ = run(["--parameter_file", str(pf)] + base, tmp_path)
     assert plan["fem_modes"][0].split() == ["correlograph", "correlograph_variance"]
     for flag in ("--correlograph_everyimage", "--correlograph_everytxt"):
-        rc, _, out = run(base + [flag], tmp_path)
-        assert rc == 1 and "not supported" in out, flag
+        rc, plan, out = run(base + [flag], tmp_path)
+        assert rc == 0 and plan["fem_modes"][0].split() == ["correlograph"], flag
 
 
 def test_mtf_flags(tmp_path, model):

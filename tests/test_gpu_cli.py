@@ -107,7 +107,7 @@ def test_cli_correlograph_run(tmp_path, O):
     args = [CLI, "-m", "128", "128", "300000", "-a", "m.xyz", "--output_prefix", "run",
             "--adfstemuncorrfem", "--cs3", "1.0e6", "--defocus", "25", "--alphamax", "0.01",
             "--rasterspacing", "2.5", "--minslice", "1.5", "--dr", "0.5",
-            "--correlograph_variance"]
+            "--correlograph_variance", "--correlograph_everytxt", "--correlograph_everyimage"]
     print("cli:", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -120,7 +120,7 @@ def test_cli_correlograph_run(tmp_path, O):
     xp = np.arange(4) * 2.5
     job = c.oracle_job(O)
     _, mo = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, xp,
-                          modes=("v_omega",), correlograph=True)
+                          modes=("v_omega",), correlograph=True, correlograph_each=True)
 
     pre = tmp_path / "run_adfstem_uncorr"
     nphi, nk = mo["correlograph_avg"].shape
@@ -153,6 +153,15 @@ def test_cli_correlograph_run(tmp_path, O):
     cl = _tiff(f"{pre}_correlograph_variance_logscale.tif").astype(float)
     tl = np.log(var - var.min() + 1) / np.log(var.max() - var.min() + 1)
     assert np.abs(cl - np.where(tl > 0, np.floor(tl * 65535.0 + 0.5), 0)).max() <= 2
+
+    # every position: <prefix>_<x index>_<y index>.txt and .tif (adfstem.cpp:2655-2685)
+    for ix, iy in ((0, 0), (1, 3), (3, 2)):
+        e = np.loadtxt(f"{pre}_{ix}_{iy}.txt")
+        np.testing.assert_array_equal(e[0], avg[0])
+        assert rel_l2(e[1:, 1:].T, mo["correlograph_each"][ix * 4 + iy]) < 1e-4
+        im = _tiff(f"{pre}_{ix}_{iy}.tif")
+        assert im.shape == (nphi, nk)
+    assert not (tmp_path / "run_adfstem_uncorr_4_0.txt").exists()
 
 
 def test_cli_mtf_run(tmp_path, O):

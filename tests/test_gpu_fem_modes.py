@@ -84,14 +84,22 @@ def test_correlograph_matches_oracle(msf, ctx, O):
     yp = np.linspace(0.8, 11.6, 5)
     job = c.oracle_job(O)
     _, o = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, yp,
-                         modes=("v_omega",), correlograph=True)
+                         modes=("v_omega",), correlograph=True, correlograph_each=True)
     spec = c.gpu_specimen(msf, ctx)
     nphi, k_edges, phi_edges = msf.correlograph_bins(spec.grid, c.dr)
     nb = o["v_omega"].size
     assert nphi == O.lib().orc_correlograph_nphi(k_edges[1], nb) and nphi > 4
     print("correlograph", nb, nphi, flush=True)
     _, g = ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf), modes=("v_omega",),
-                           correlograph=True)
+                           correlograph=True, correlograph_each=True)
+    # per position (--correlograph_everyimage / _everytxt)
+    assert g["correlograph_each"].shape == o["correlograph_each"].shape == (xp.size * yp.size,
+                                                                             nphi, nb)
+    err = rel_l2(g["correlograph_each"], o["correlograph_each"])
+    print("correlograph_each", err, flush=True)
+    assert err < 1e-4
+    np.testing.assert_allclose(g["correlograph_each"].mean(0), g["correlograph_avg"],
+                               rtol=1e-12, atol=1e-15)
     for m in ("correlograph_avg", "correlograph_sqr_sum"):
         assert g[m].shape == o[m].shape == (nphi, nb), m
         assert (o[m] != 0).sum() > nb, m          # a populated correlograph

@@ -95,8 +95,14 @@ def test_correlograph_uniform_field(O):
     for p in range(P):
         r1[p], c1[p] = O.fem_radial(mag[p].astype(np.complex128), kx, ky, dk, nb, lam, alpha)
     avg = np.zeros(nb * nphi); sqr = np.zeros(nb * nphi)
+    each = np.zeros(P * nb * nphi)
     assert O.lib().orc_correlograph((mag ** 2).reshape(-1), P, kx, ky, nx, ny, dk, nb, lam,
-                                    alpha, r1.reshape(-1), c1.reshape(-1), avg, sqr) == 0
+                                    alpha, r1.reshape(-1), c1.reshape(-1), avg, sqr,
+                                    each.ctypes.data) == 0
+    # the per-position values (every position alike here) average to avg
+    each = each.reshape(P, nphi, nb)
+    np.testing.assert_allclose(each.mean(0), avg.reshape(nphi, nb), atol=1e-12)
+    np.testing.assert_allclose((each ** 2).sum(0), sqr.reshape(nphi, nb), atol=1e-10)
     avg = avg.reshape(nphi, nb)
     sqr = sqr.reshape(nphi, nb)
     # sector populations, counted independently
