@@ -267,7 +267,13 @@ int msf_probe_shifts(const msf_grid* grid, const double* x_p, uint32_t n_xp, con
  *       the sum of c^2 over positions.
  *   correlograph_each   (--correlograph_everyimage / _everytxt, adfstem.cpp:2655-2685):
  *       n_xp * n_yp * n_bins * n_phi doubles, c of every position in raster
- *       order, [p][ii + jj * n_bins]; computed with the correlograph. */
+ *       order, [p][ii + jj * n_bins]; computed with the correlograph.
+ *   diffraction_mean_abs, diffraction_mean_sqr, variance_2d   (adfstem.cpp:2968-2990,
+ *       :3060-3095, :3468-3520): nx * ny doubles each, reference layout: the
+ *       mean over positions of |psi| and of |psi|^2 per pixel, and the 2-D
+ *       variance image <|psi|^2> / <|psi|>^2 - 1 with the direct beam
+ *       (lambda^2 k^2 <= alpha_max^2) set to 0 (0/0 elsewhere outside the band
+ *       limit, as in the reference). */
 typedef struct msf_fem_outputs {
     double* v_omega;
     double* vbar_r;
@@ -278,6 +284,9 @@ typedef struct msf_fem_outputs {
     double* correlograph_avg;
     double* correlograph_sqr_sum;
     double* correlograph_each;
+    double* diffraction_mean_abs;
+    double* diffraction_mean_sqr;
+    double* variance_2d;
 } msf_fem_outputs;
 
 /* msf_run_stem with every FEM mode above; fem->enabled must be set for any

@@ -314,16 +314,15 @@ int die(const char* what, msf_ctx* c)
     return EXIT_FAILURE;
 }
 
-// log(1 + s |psi|^2), quadrants swapped (write_tif.cpp:817-1030), then a
-// linear 16-bit min-max TIFF
-int write_diffraction(const std::string& prefix, const float* psi, long nx, long ny, double s,
-                      double xres, double yres)
+// log(1 + s |psi|^2) of a complex wave (write_tif.cpp:817-1030) or
+// log(1 + s v^2) of a real array (write_tif.cpp:1413-1548), quadrants
+// swapped, then a linear 16-bit min-max TIFF.  NaN pixels (0/0 in a 2-D
+// variance) are left out of the min and max, as by the reference's
+// comparisons, and written as 0.
+int write_log_diffraction(const std::string& prefix, std::vector<double>& mag, long nx, long ny,
+                          double s, double xres, double yres)
 {
-    std::vector<double> mag((size_t)nx * ny);
-    for (size_t k = 0; k < mag.size(); ++k) {
-        const double re = psi[2 * k], im = psi[2 * k + 1];
-        mag[k] = std::log(1.0 + (s * (re * re + im * im)));
-    }
+    for (double& v : mag) v = std::log(1.0 + (s * v));
     double mn = mag[0], mx = mag[0];
     for (double v : mag) {
         if (v > mx) mx = v;
@@ -335,6 +334,25 @@ int write_diffraction(const std::string& prefix, const float* psi, long nx, long
     }
     const std::string name = prefix + "_diffraction_scalefactor_" + to_string_g(s) + ".tif";
     return msf_write_tiff16(name.c_str(), mag.data(), (uint32_t)nx, (uint32_t)ny, mn, mx, 0, 1, xres, yres);
+}
+
+int write_diffraction(const std::string& prefix, const float* psi, long nx, long ny, double s,
+                      double xres, double yres)
+{
+    std::vector<double> i2((size_t)nx * ny);
+    for (size_t k = 0; k < i2.size(); ++k) {
+        const double re = psi[2 * k], im = psi[2 * k + 1];
+        i2[k] = re * re + im * im;
+    }
+    return write_log_diffraction(prefix, i2, nx, ny, s, xres, yres);
+}
+
+int write_diffraction_real(const std::string& prefix, const std::vector<double>& v, long nx,
+                           long ny, double s, double xres, double yres)
+{
+    std::vector<double> i2(v.size());
+    for (size_t k = 0; k < v.size(); ++k) i2[k] = v[k] * v[k];
+    return write_log_diffraction(prefix, i2, nx, ny, s, xres, yres);
 }
 
 // Correlograph outputs (adfstem.cpp:3617-3745): the average as text and a
@@ -606,9 +624,17 @@ int main(int argc, char** argv)
         std::vector<double> c_avg(corr ? ncorr : 0), c_sqr(corr ? ncorr : 0);
         const bool every = corr && (f.correlograph_everyimage || f.correlograph_everytxt);
         std::vector<double> c_each(every ? xp.size() * yp.size() * ncorr : 0);
+        // per-pixel averages over the raster and the 2-D variance, written as
+        // images (adfstem.cpp:2966-3099, :3468-3520)
+        const size_t npix = (size_t)p.Nx * p.Ny;
+        const bool img_avg = f.fem && f.image_output && (f.gt17 || f.d3 || f.d4 || f.rva);
+        const bool img_sqr = img_avg && (f.gt17 || f.d3 || f.d4);
+        const bool img_var = f.fem && f.image_output && f.d4;
+        std::vector<double> p_abs(img_avg ? npix : 0), p_sqr(img_sqr ? npix : 0), p_var(img_var ? npix : 0);
         const msf_fem_outputs fo{ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5],
                                  corr ? c_avg.data() : nullptr, corr ? c_sqr.data() : nullptr,
-                                 every ? c_each.data() : nullptr};
+                                 every ? c_each.data() : nullptr, img_avg ? p_abs.data() : nullptr,
+                                 img_sqr ? p_sqr.data() : nullptr, img_var ? p_var.data() : nullptr};
         if (msf_run_stem_fem(c, spec, &o, &det, &fem, xp.data(), (uint32_t)xp.size(), yp.data(),
                              (uint32_t)yp.size(), img.data(), &fo) != MSF_OK)
             return die("adfstem", c);
@@ -639,6 +665,19 @@ int main(int argc, char** argv)
                     rc = EXIT_FAILURE;
                 }
             }
+        {
+            const double rx = grid.xperiod * 1.0e-8, ry = grid.yperiod * 1.0e-8;
+            int w = MSF_OK;
+            if (img_avg) w |= write_diffraction_real(out + "_diffracted_wave_avg", p_abs, p.Nx, p.Ny, 1.0e13, rx, ry);
+            if (img_sqr)
+                w |= write_diffraction_real(out + "_diffracted_wave_sqr_avg", p_sqr, p.Nx, p.Ny, 1.0e13, rx, ry);
+            for (double s : {1.0e0, 1.0e-10})
+                if (img_var) w |= write_diffraction_real(out + "_variance_2-D", p_var, p.Nx, p.Ny, s, rx, ry);
+            if (w) {
+                std::cerr << "Error writing images: " << msf_io_last_error() << std::endl;
+                rc = EXIT_FAILURE;
+            }
+        }
         if (corr && write_correlograph(out, c_avg, c_sqr, k_edges, phi_edges, nbins, nphi,
                                        xp.size() * yp.size(), f.correlograph_variance)) {
             std::cerr << "Error writing the correlograph: " << msf_io_last_error() << std::endl;

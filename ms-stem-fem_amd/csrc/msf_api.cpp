@@ -350,6 +350,9 @@ struct RunOut {
     double* corr_avg = nullptr; // correlograph, [ii + jj * nbins]
     double* corr_sqr = nullptr;
     double* corr_each = nullptr; // [p][ii + jj * nbins]
+    double* mean_abs = nullptr;  // per-pixel means over positions, [i * ny + j]
+    double* mean_sqr = nullptr;
+    double* var2d = nullptr;
     float* exit = nullptr;
 };
 
@@ -392,6 +395,35 @@ void fem_pixel_modes(const msf::Domain& d, const msf::Groups& g, const std::vect
     }
 }
 
+// The per-pixel means of |psi| and |psi|^2 over positions on the full grid
+// (adfstem.cpp:2968-2990) and the 2-D variance image (adfstem.cpp:3468-3492):
+// <|psi|^2> / <|psi|>^2 - 1, 0 where lambda^2 k^2 <= alpha_max^2.  Pixels
+// outside the stored columns have zero sums.
+void fem_pixel_images(const msf::Domain& d, const msf::RowRanges& keep, int ncp,
+                      const std::vector<double>& s1, const std::vector<double>& s2, int64_t P,
+                      double alpha_max, const RunOut& out)
+{
+    const int A = keep.amax, B = keep.bmax;
+    const double np = (double)P;
+    const double l2 = d.lambda * d.lambda, a2 = alpha_max * alpha_max; // adfstem.cpp:95-96
+    for (int i = 0; i < d.nx; ++i)
+        for (int j = 0; j < d.ny; ++j) {
+            const int jc = (j < A) ? j : (j >= d.ny - B ? j - d.ny + A + B : -1);
+            const int64_t cp = jc >= 0 ? (int64_t)i * ncp + jc : -1;
+            const double mag = (cp >= 0 ? s1[cp] : 0.0) / np;
+            const double sqr = (cp >= 0 ? s2[cp] : 0.0) / np;
+            const size_t k = (size_t)i * d.ny + j;
+            if (out.mean_abs) out.mean_abs[k] = mag;
+            if (out.mean_sqr) out.mean_sqr[k] = sqr;
+            if (out.var2d) {
+                double v = (sqr / (mag * mag)) - 1.0;
+                const double ksqr = msf::ref_pow2(d.kx[i]) + msf::ref_pow2(d.ky[j]);
+                if (l2 * ksqr <= a2) v = 0.0;
+                out.var2d[k] = v;
+            }
+        }
+}
+
 // Probe shifts in raster order (x outer, y inner), each index persisting
 // from the previous position when no grid point qualifies
 // (adfstem.cpp:1476-1477, :1590-1606).  False if a position lies below the
@@ -426,7 +458,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     const bool fem_on = fem && fem->enabled;
     const bool want_d1 = fem_on && out.fem;
     const bool want_d2 = fem_on && out.vbar_r;
-    const bool want_pix = fem_on && (out.v_re || out.omega_vimage || out.v_gt17 || out.v_rbar);
+    const bool want_pix = fem_on && (out.v_re || out.omega_vimage || out.v_gt17 || out.v_rbar ||
+                                     out.mean_abs || out.mean_sqr || out.var2d);
     const bool want_corr = fem_on && (out.corr_avg || out.corr_sqr || out.corr_each);
     const bool want_fem = want_d1 || want_d2 || want_pix || want_corr;
     // the detector MTF changes only what the FEM statistics see
@@ -1036,6 +1069,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, hipMemcpy(s1.data(), pl.pm1.p, (size_t)plane * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(s2.data(), pl.pm2.p, (size_t)plane * sizeof(double), hipMemcpyDeviceToHost));
         fem_pixel_modes(d, pl.hg, s1, s2, P, o->alpha_max, out);
+        if (out.mean_abs || out.mean_sqr || out.var2d)
+            fem_pixel_images(d, keep, ncp, s1, s2, P, o->alpha_max, out);
     }
     if (want_corr && nbins * nphi > 0) { // adfstem.cpp:3620-3632: the mean over positions
         const size_t n = (size_t)nbins * nphi;
@@ -1746,6 +1781,9 @@ int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         out.corr_avg = fo->correlograph_avg;
         out.corr_sqr = fo->correlograph_sqr_sum;
         out.corr_each = fo->correlograph_each;
+        out.mean_abs = fo->diffraction_mean_abs;
+        out.mean_sqr = fo->diffraction_mean_sqr;
+        out.var2d = fo->variance_2d;
     }
     return run_internal(c, sp, o, det, fem, shifts, out);
 }

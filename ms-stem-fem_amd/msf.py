@@ -89,7 +89,10 @@ FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
 class FemOutputs(C.Structure):
     _fields_ = [(m, C.c_void_p) for m in FEM_MODES] + [("correlograph_avg", C.c_void_p),
                                                        ("correlograph_sqr_sum", C.c_void_p),
-                                                       ("correlograph_each", C.c_void_p)]
+                                                       ("correlograph_each", C.c_void_p),
+                                                       ("diffraction_mean_abs", C.c_void_p),
+                                                       ("diffraction_mean_sqr", C.c_void_p),
+                                                       ("variance_2d", C.c_void_p)]
 
 
 class Job(C.Structure):
@@ -466,10 +469,11 @@ class Context:
         return img, v
 
     def adfstem_fem(self, spec, optics, x_p, y_p, fem, modes=FEM_MODES, detector=None,
-                    correlograph=False, correlograph_each=False):
+                    correlograph=False, correlograph_each=False, pixel_images=False):
         """msf_run_stem_fem: returns (adf image or None, {mode: values [n_bins]});
         with correlograph, also "correlograph_avg" and "correlograph_sqr_sum"
-        shaped [n_phi, n_bins]."""
+        shaped [n_phi, n_bins]; with pixel_images, "diffraction_mean_abs",
+        "diffraction_mean_sqr" and "variance_2d" shaped [nx, ny]."""
         x_p = np.ascontiguousarray(x_p, np.float64)
         y_p = np.ascontiguousarray(y_p, np.float64)
         det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
@@ -487,12 +491,19 @@ class Context:
                 outs["correlograph_each"] = np.zeros(max(x_p.size * y_p.size * nb * nphi, 1))
                 ptrs.append(outs["correlograph_each"].ctypes.data)
         fo = FemOutputs(*ptrs)
+        img_names = ("diffraction_mean_abs", "diffraction_mean_sqr", "variance_2d")
+        if pixel_images:
+            for m in img_names:
+                outs[m] = np.zeros((spec.grid.nx, spec.grid.ny))
+                setattr(fo, m, outs[m].ctypes.data)
         self._check(lib().msf_run_stem_fem(self._h, spec._h, C.byref(optics), C.byref(det),
                                            C.byref(fem), _p(x_p), x_p.size, _p(y_p), y_p.size,
                                            _p(img), C.byref(fo)), "msf_run_stem_fem")
         if img is not None:
             img = img.reshape(x_p.size, y_p.size)
         res = {m: v[:nb] for m, v in outs.items() if m in FEM_MODES}
+        if pixel_images:
+            res.update({m: outs[m] for m in img_names})
         if correlograph:
             for m in ("correlograph_avg", "correlograph_sqr_sum"):
                 res[m] = outs[m][:nb * nphi].reshape(nphi, nb)

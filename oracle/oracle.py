@@ -362,9 +362,11 @@ FEM_MODES = ("v_omega", "vbar_r", "v_re", "omega_vimage", "v_gt17", "v_rbar")
 
 
 def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=False,
-                correlograph_each=False):
+                correlograph_each=False, exit_waves=False):
     """Whole raster with the FEM modes of adfstem.cpp:2489-3605: returns
-    (adf image or None, {mode: values [nbins]})."""
+    (adf image or None, {mode: values [nbins]}); with exit_waves the dict
+    also holds "exit_waves" [P, nx, ny], each position's wave after the
+    detector MTF, the one the FEM statistics see."""
     x_p = np.ascontiguousarray(x_p, np.float64)
     y_p = np.ascontiguousarray(y_p, np.float64)
     P = x_p.size * y_p.size
@@ -382,9 +384,10 @@ def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=
         sqr = np.zeros(max(nb * nphi, 1))
         if correlograph_each:
             each = np.zeros(max(P * nb * nphi, 1))
+    ex = np.zeros((P, job.nx, job.ny), np.complex128) if exit_waves else None
     rc = lib().orc_adfstem_fem(C.byref(job), C.byref(spec._s), x_p, x_p.size, y_p, y_p.size,
                                n_threads, ptr(img), *[ptr(outs.get(m)) for m in FEM_MODES],
-                               ptr(corr), ptr(sqr), ptr(each), None)
+                               ptr(corr), ptr(sqr), ptr(each), ptr(ex))
     assert rc == 0, rc
     if img is not None:
         img = img.reshape(x_p.size, y_p.size)
@@ -395,6 +398,8 @@ def adfstem_fem(job, spec, x_p, y_p, modes=FEM_MODES, n_threads=0, correlograph=
         res["correlograph_sqr_sum"] = sqr[:nb * nphi].reshape(nphi, nb)
         if correlograph_each:
             res["correlograph_each"] = each[:P * nb * nphi].reshape(P, nphi, nb)
+    if exit_waves:
+        res["exit_waves"] = ex
     return img, res
 
 

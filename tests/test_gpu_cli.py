@@ -91,6 +91,29 @@ def test_cli_adfstem_fem_run(tmp_path, O):
         bad = np.abs(d - want) > 2
         assert bad.mean() < 1e-3, (s, tag, bad.sum())
 
+    # per-pixel averages over the raster and the 2-D variance
+    # (adfstem.cpp:3063-3097, :3493-3520): log(1 + s v^2) of the real arrays,
+    # quadrants swapped; the variance's 0/0 pixels are left out of the range
+    # and written as 0
+    _, r_o = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, xp,
+                           modes=("omega_vimage",), exit_waves=True)
+    a = np.abs(r_o["exit_waves"]).mean(axis=0)
+    sq = (np.abs(r_o["exit_waves"]) ** 2).mean(axis=0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        var = sq / (a * a) - 1.0
+    kx, ky, _, _ = O.domains(128, 128, 10.0, 10.0)
+    var[O.wavelength(300e3)[0] ** 2 * (kx[:, None] ** 2 + ky[None, :] ** 2) <= 0.02 ** 2] = 0.0
+    for name, v, s in (("diffracted_wave_avg", a, "1e+13"), ("diffracted_wave_sqr_avg", sq, "1e+13"),
+                       ("variance_2-D", var, "1"), ("variance_2-D", var, "1e-10")):
+        d = _tiff(f"{pre}_{name}_diffraction_scalefactor_{s}.tif").astype(float)
+        m = np.fft.fftshift(np.log(1.0 + float(s) * v * v))
+        lo, hi = np.nanmin(m), np.nanmax(m)
+        want = np.where(np.isnan(m), 0.0, np.floor((m - lo) * (65535.0 / (hi - lo)) + 0.5))
+        np.testing.assert_array_equal(d[np.isnan(m)], 0.0)
+        off = np.abs(d - want)
+        print(name, s, "max off", off.max(), "frac>2", (off > 2).mean(), flush=True)
+        assert (off > 2).mean() < 1e-3, (name, s)
+
 
 def test_cli_correlograph_run(tmp_path, O):
     """--correlograph_variance: the average and sums-of-squares text files

@@ -116,3 +116,48 @@ def test_correlograph_matches_oracle(msf, ctx, O):
         ctx.configure()
     for m in ("correlograph_avg", "correlograph_sqr_sum"):
         np.testing.assert_array_equal(g7[m], g[m], err_msg=m)
+
+
+@pytest.mark.parametrize("mtf", [False, True])
+def test_pixel_images_match_oracle_exit_waves(msf, ctx, O, mtf):
+    """Per-pixel averages of |psi| and |psi|^2 over the raster and the 2-D
+    variance <|psi|^2>/<|psi|>^2 - 1, zeroed inside the aperture
+    (adfstem.cpp:2966-3000, :3468-3492), against the same quantities taken
+    from the oracle's exit waves; NaN (0/0) exactly where the oracle's waves
+    vanish at every position, outside the band limit."""
+    c = _case()
+    if mtf:
+        d = np.linspace(0.0, 1.0, 11)
+        c = Case(**{**c.__dict__, "mtf": 1.0 / (1.0 + (d / 0.4) ** 2), "mtf_domain": d,
+                    "mtf_rate": 12.0})
+    xp = np.linspace(0.3, 11.2, 4)
+    yp = np.linspace(0.8, 11.6, 3)
+    job = c.oracle_job(O)
+    _, r_o = O.adfstem_fem(job, O.Specimen(job, c.q, c.Z.astype(np.int32)), xp, yp,
+                           modes=("omega_vimage",), exit_waves=True)
+    ex = r_o["exit_waves"]
+    a_o = np.abs(ex).mean(axis=0)
+    s_o = (np.abs(ex) ** 2).mean(axis=0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v_o = s_o / (a_o * a_o) - 1.0
+    kx, ky, _, _ = O.domains(c.nx, c.ny, c.cell[0], c.cell[1])
+    lam = O.wavelength(c.voltage)[0]
+    v_o[lam ** 2 * (kx[:, None] ** 2 + ky[None, :] ** 2) <= c.alpha_max ** 2] = 0.0
+    spec = c.gpu_specimen(msf, ctx)
+    _, r_g = ctx.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf),
+                             detector=c.detector(msf), pixel_images=True)
+    for name, got, want in (("mean_abs", r_g["diffraction_mean_abs"], a_o),
+                            ("mean_sqr", r_g["diffraction_mean_sqr"], s_o)):
+        assert got.shape == (c.nx, c.ny)
+        err = rel_l2(got, want)
+        print(name, mtf, err, flush=True)
+        assert err < 1e-4, (name, err)
+    v_g = r_g["variance_2d"]
+    np.testing.assert_array_equal(np.isnan(v_g), np.isnan(v_o))
+    assert np.isnan(v_o).any() and np.isfinite(v_o).any()
+    fin = np.isfinite(v_o)
+    err = rel_l2(v_g[fin], v_o[fin])
+    print("variance_2d", mtf, err, flush=True)
+    assert err < 1e-4
+    # the Omega_vimage mode is the ring average of this image
+    assert rel_l2(r_g["omega_vimage"], r_o["omega_vimage"]) < 1e-4
