@@ -157,6 +157,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--fft-plan", type=int, default=0,
+                    help="radix plans: 1 primary, 2 alternate, 0 automatic per length")
     ap.add_argument("--plain-positions", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -194,7 +196,7 @@ def main():
     fem = msf.Fem(1, job["dr"])
     log(f"specimen: {S} slices, build {spec_ms:.1f} ms")
 
-    ctx.configure(batch=args.batch)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
     for _ in range(args.warmup):
         ctx.adfstem(spec, optics, xp, yp, fem=fem)
     times = []
@@ -209,10 +211,10 @@ def main():
     log(f"batched: {value:.4g} props/s, batch {st['batch']}, loop {st['slice_loop_ms']:.2f} ms")
 
     # profiled step: per-kernel durations with HIP events on the library stream
-    ctx.configure(batch=args.batch, profile_kernels=True)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, profile_kernels=True)
     ctx.adfstem(spec, optics, xp, yp, fem=fem)
     pst = ctx.stats()
-    ctx.configure(batch=args.batch)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
     kern = {"row_pass": (pst["row_pass_us"], pst["row_pass_bytes"]),
             "col_pass": (pst["col_pass_us"], pst["col_pass_bytes"])}
     dom = max(kern, key=lambda k: kern[k][0])
@@ -258,13 +260,13 @@ def main():
     plain = None
     if not args.no_plain:
         n = min(args.plain_positions, P)
-        ctx.configure(batch=1, graph=False, chains=1)   # one position per launch, stream launches
+        ctx.configure(batch=1, graph=False, chains=1, fft_plan=args.fft_plan)   # one position per launch, stream launches
         xs = np.resize(xp, n)
         ctx.adfstem(spec, optics, xs[:1], yp[:1], fem=fem)
         t0 = time.perf_counter()
         ctx.adfstem(spec, optics, xs, yp[:1], fem=fem)
         dt = time.perf_counter() - t0
-        ctx.configure(batch=args.batch)
+        ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
         plain = {"value": n * S / dt, "positions": n, "batch": 1,
                  "speedup_batched_vs_plain": round(value / (n * S / dt), 2)}
         log(f"plain: {plain['value']:.4g} props/s")
@@ -301,7 +303,8 @@ def main():
                    "grid": 512, "atoms": int(Z.size), "slices": int(S), "positions": int(P),
                    "batch": int(st["batch"]), "chains": int(st["chains"]),
                    "col_tile": int(st["col_tile"]), "graph": int(st["graph"]),
-                   "autotuned": int(st["autotuned"]), "seed": 2001},
+                   "autotuned": int(st["autotuned"]),
+                   "fft_plan": [int(st["row_fft_plan"]), int(st["col_fft_plan"])], "seed": 2001},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": plain,

@@ -109,6 +109,13 @@ typedef struct msf_config {
                                           time the candidates on the job's own specimen once per
                                           shape (nx, ny, kept columns, positions up to 256) and
                                           keep the fastest; 0 (default): the fixed rules */
+    int32_t fft_plan;                  /* radix plans of the slice passes: 1 primary (every
+                                          length), 2 the alternate plans with 16 values per lane
+                                          and one fewer stage on lengths 256, 512 and 1024 (the
+                                          primary elsewhere); 0 = automatic (per length: the
+                                          alternate at 256 and 1024, else the primary; or the
+                                          autotuner's choice among these three).  Plans differ by
+                                          FP32 rounding only. */
 } msf_config;
 
 /* Timings of the last msf_run_stem() on the context. */
@@ -127,6 +134,8 @@ typedef struct msf_stats {
     int32_t chains;                    /* batch chains in flight (msf_config.chains) */
     int32_t col_tile;                  /* columns per column-pass workgroup used */
     int32_t autotuned;                 /* 0: fixed rules; 1: tuned in this call; 2: tuned earlier */
+    int32_t row_fft_plan;              /* radix plan of the row passes (length ny) and */
+    int32_t col_fft_plan;              /* of the column passes (nx): 1 primary, 2 alternate */
 } msf_stats;
 
 /* ------------------------------------------------------------ context */

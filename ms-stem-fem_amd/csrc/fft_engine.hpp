@@ -137,6 +137,68 @@ template <bool INV, typename C> struct Dft<6, INV, C> {
     }
 };
 
+// multiplication by W16^e = exp(-/+ 2 pi i e / 16) (forward / inverse)
+__host__ __device__ constexpr double cos16(int e)
+{
+    switch (((e % 16) + 16) % 16) {
+    case 0: return 1.0;
+    case 1: case 15: return 0.92387953251128675613;
+    case 2: case 14: return 0.70710678118654752440;
+    case 3: case 13: return 0.38268343236508977173;
+    case 4: case 12: return 0.0;
+    case 5: case 11: return -0.38268343236508977173;
+    case 6: case 10: return -0.70710678118654752440;
+    case 7: case 9: return -0.92387953251128675613;
+    default: return -1.0;
+    }
+}
+template <int EXP, bool INV, typename C> __device__ __forceinline__ C mul_w16(C a)
+{
+    using R = decltype(a.x);
+    constexpr int e = EXP % 16;
+    if constexpr (e == 0) {
+        return a;
+    } else if constexpr (e == 4) {
+        return mul_mi<INV>(a);
+    } else if constexpr (e == 8) {
+        return C{-a.x, -a.y};
+    } else {
+        constexpr R c = R(cos16(e)), s = R(cos16(e - 4)); // sin(2 pi e / 16)
+        return INV ? C{a.x * c - a.y * s, a.y * c + a.x * s} : C{a.x * c + a.y * s, a.y * c - a.x * s};
+    }
+}
+
+// radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2,
+// X[k] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2]
+template <bool INV, typename C> struct Dft<16, INV, C> {
+    static __device__ __forceinline__ void run(C* v)
+    {
+        C a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = v[4 * n1 + n2];
+            Dft<4, INV, C>::run(a[n2]);
+        }
+        a[1][1] = mul_w16<1, INV>(a[1][1]);
+        a[1][2] = mul_w16<2, INV>(a[1][2]);
+        a[1][3] = mul_w16<3, INV>(a[1][3]);
+        a[2][1] = mul_w16<2, INV>(a[2][1]);
+        a[2][2] = mul_w16<4, INV>(a[2][2]);
+        a[2][3] = mul_w16<6, INV>(a[2][3]);
+        a[3][1] = mul_w16<3, INV>(a[3][1]);
+        a[3][2] = mul_w16<6, INV>(a[3][2]);
+        a[3][3] = mul_w16<9, INV>(a[3][3]);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            C b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            Dft<4, INV, C>::run(b);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+        }
+    }
+};
+
 // ------------------------------------------------------------ LDS layout
 // one padding slot every 8 elements breaks the power-of-two strides of the
 // Stockham write pattern across LDS banks
@@ -224,10 +286,15 @@ __device__ __forceinline__ void run_stages(C (&x)[E], C* lds, const C* tw, int t
 
 // ------------------------------------------------------------ plans
 // E = values per lane; radices applied first to last.  kRadix lists them for
-// the host code that builds the twiddle tables.
+// the host code that builds the twiddle tables.  Plan<N> is the primary
+// schedule of each length (also used by the FP64 transforms); AltPlan<N>, where
+// it exists, a second schedule of the FP32 slice passes with 16 values per lane
+// and one fewer radix stage, chosen per run (configuration or autotuner).
 template <int N> struct Plan;
-#define MSF_PLAN(N_, E_, ...)                                                                  \
-    template <> struct Plan<N_> {                                                              \
+template <int N> struct AltPlan { static constexpr bool exists = false; };
+#define MSF_PLAN_AS(NAME, N_, E_, ...)                                                         \
+    template <> struct NAME<N_> {                                                              \
+        static constexpr bool exists = true;                                                   \
         static constexpr int N = N_, E = E_, T = N_ / E_;                                      \
         static constexpr int kRadix[] = {__VA_ARGS__};                                         \
         static constexpr int kStages = sizeof(kRadix) / sizeof(int);                           \
@@ -237,6 +304,8 @@ template <int N> struct Plan;
             run_stages<C, N_, E_, INV, WAVE, 1, 0, __VA_ARGS__>(x, lds, tw, t);                \
         }                                                                                      \
     };
+#define MSF_PLAN(N_, E_, ...) MSF_PLAN_AS(Plan, N_, E_, __VA_ARGS__)
+#define MSF_ALT_PLAN(N_, E_, ...) MSF_PLAN_AS(AltPlan, N_, E_, __VA_ARGS__)
 MSF_PLAN(16, 8, 8, 2)
 MSF_PLAN(32, 8, 8, 4)
 MSF_PLAN(64, 8, 8, 8)
@@ -248,9 +317,20 @@ MSF_PLAN(384, 12, 6, 4, 4, 4)
 MSF_PLAN(512, 8, 8, 8, 8)
 MSF_PLAN(768, 24, 6, 8, 4, 4)
 MSF_PLAN(1024, 8, 8, 8, 4, 4)
+MSF_ALT_PLAN(256, 16, 16, 16)
+MSF_ALT_PLAN(512, 16, 16, 8, 4)
+MSF_ALT_PLAN(1024, 16, 16, 16, 4)
+#undef MSF_ALT_PLAN
 #undef MSF_PLAN
+#undef MSF_PLAN_AS
+
+// plan V of length N: 0 = primary, 1 = alternate
+template <int N, int V> struct PlanSel { using type = Plan<N>; };
+template <int N> struct PlanSel<N, 1> { using type = AltPlan<N>; };
+template <int N, int V> using PlanV = typename PlanSel<N, V>::type;
 
 // entries of the concatenated stage tables: sum (R-1) NS over stages 1.. = N - R_0
-template <int N> constexpr int stage_tw_len() { return N - Plan<N>::kRadix[0]; }
+template <typename P> constexpr int plan_tw_len() { return P::N - P::kRadix[0]; }
+template <int N> constexpr int stage_tw_len() { return plan_tw_len<Plan<N>>(); }
 
 } // namespace msf

@@ -35,6 +35,7 @@ struct RowPassArgs {
     float inv_s;            // 1/sqrtNxNy
     int nx, ncp, ka, kb, batch;
     int prune;              // 1: columns [3ny/8, 5ny/8) are outside the kept band
+    int plan;               // radix plan of length ny: 0 primary, 1 alternate (has_alt_plan)
 };
 
 struct ColPassArgs {
@@ -50,15 +51,19 @@ struct ColPassArgs {
     int ny, ncp, ka, kb, batch;
     int tile;               // columns per workgroup (8 or 16; 0 = default)
     int prune;              // 1: rows [3nx/8, 5nx/8) are masked for every column
+    int plan;               // radix plan of length nx: 0 primary, 1 alternate (has_alt_plan)
 };
 
 bool slice_lengths_ok(int n);
-// concatenated per-stage twiddle tables of the length-n plan, (re, im) FP64
-bool stage_twiddles(int n, std::vector<double>& re_im);
+// concatenated per-stage twiddle tables of plan v (0 primary, 1 alternate) of
+// length n, (re, im) FP64
+bool stage_twiddles(int n, std::vector<double>& re_im, int v = 0);
+// true when length n has an alternate radix plan for the slice passes
+bool has_alt_plan(int n);
 // the column count of the compact wave is padded to a multiple of this
 int col_tile(int nx);
-// the column-pass tile width used when none is configured
-int col_tile_default(int nx);
+// the column-pass tile width used when none is configured, for plan v
+int col_tile_default(int nx, int v = 0);
 // true when the kept set leaves [3n/8, 5n/8) empty along the row (ka, kb) and
 // every row i in [3nx/8, 5nx/8) fully masked (keep ranges)
 bool row_prunable(int ny, int ka, int kb);

@@ -76,15 +76,15 @@ __device__ __forceinline__ int xcd_band(int L, int G)
     return x * q + min(x, r) + k;
 }
 
-template <int NY, bool FIRST, int TMODE, bool PRUNE>
+template <int NY, int V, bool FIRST, int TMODE, bool PRUNE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 row_pass_kernel(RowPassArgs a)
 {
-    using P = Plan<NY>;
+    using P = PlanV<NY, V>;
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = 256 / T;
     constexpr int LEN = lds_len(NY);
-    __shared__ float2 s_tw[stage_tw_len<NY>()];
+    __shared__ float2 s_tw[plan_tw_len<P>()];
     __shared__ float2 s_t[NY];
     __shared__ float2 s_buf[RPW * LEN];
 
@@ -97,7 +97,7 @@ row_pass_kernel(RowPassArgs a)
     // twiddles and the transmission row i, shared by the RPW rows of the group
     const float2* trow = a.t + (int64_t)i * NY;
     for (int e = threadIdx.x; e < NY; e += blockDim.x) {
-        if (e < stage_tw_len<NY>()) s_tw[e] = a.tw[e];
+        if (e < plan_tw_len<P>()) s_tw[e] = a.tw[e];
         s_t[e] = trow[e];
     }
     __syncthreads();
@@ -183,8 +183,8 @@ row_pass_kernel(RowPassArgs a)
 // per row) into a column-major LDS tile, each column is then transformed by
 // T lanes of one wavefront (wave-local exchanges, no workgroup barrier
 // between radix stages), and the tile is written back the same coalesced way.
-template <int NX> struct ColTile {
-    static constexpr int T = Plan<NX>::T;
+template <int NX, int V = 0> struct ColTile {
+    static constexpr int T = PlanV<NX, V>::T;
     static constexpr int WMAX = 16;  // ncp is a multiple of this
     // default tile: keep LDS (W * lds_len * 8 B) near 40 KiB so that several
     // workgroups share a CU
@@ -205,24 +205,27 @@ template <int NX, int W> __host__ __device__ constexpr int col_len()
 // workgroups from sharing a CU's 160 KB of LDS (1024 rows: a 74 KB tile),
 // the tables are read from global memory (L2-resident, shared by every
 // workgroup) and only the tile stays in LDS.
-template <int NX, int W> __host__ __device__ constexpr bool col_lean()
+template <int NX, int V, int W> __host__ __device__ constexpr bool col_lean()
 {
-    return 2 * ((size_t)W * col_len<NX, W>() * 8 + (size_t)stage_tw_len<NX>() * 8 +
+    return 2 * ((size_t)W * col_len<NX, W>() * 8 + (size_t)plan_tw_len<PlanV<NX, V>>() * 8 +
                 (size_t)NX * 16) > 160 * 1024;
 }
 
-template <int NX, bool LAST, int W, bool PRUNE>
-__global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
+// The alternate plans hold 16 values per lane: their launch bound is the
+// tile's real size, so that a 256- or 512-lane tile may use more than the
+// 128 registers a 1024-lane bound leaves per lane.
+template <int NX, int V, bool LAST, int W, bool PRUNE>
+__global__ void __launch_bounds__((V ? W * PlanV<NX, V>::T : 1024)) col_pass_kernel(ColPassArgs a)
 {
-    using P = Plan<NX>;
+    using P = PlanV<NX, V>;
     constexpr int E = P::E, T = P::T;
     constexpr int LEN = col_len<NX, W>();
-    constexpr bool LEAN = col_lean<NX, W>();
+    constexpr bool LEAN = col_lean<NX, V, W>();
     // a column transform lies in one wavefront (wave-local exchanges) or
     // spans whole wavefronts (workgroup barriers)
     static_assert(64 % T == 0 || T % 64 == 0, "a column transform must align with wavefronts");
     constexpr bool WAVE = T <= 64;
-    __shared__ float2 s_tw[LEAN ? 1 : stage_tw_len<NX>()];
+    __shared__ float2 s_tw[LEAN ? 1 : plan_tw_len<P>()];
     __shared__ float2 s_nat[LAST ? NX : 1]; // roots W_nx^e for the exit phase ramp
     __shared__ float2 s_px[LEAN ? 1 : NX];
     __shared__ int2 s_keep[LEAN ? 1 : NX];
@@ -230,7 +233,7 @@ __global__ void __launch_bounds__(1024) col_pass_kernel(ColPassArgs a)
 
     for (int e = threadIdx.x; e < NX; e += blockDim.x) {
         if constexpr (!LEAN) {
-            if (e < stage_tw_len<NX>()) s_tw[e] = a.tw[e];
+            if (e < plan_tw_len<P>()) s_tw[e] = a.tw[e];
             s_px[e] = a.px[e];
             s_keep[e] = a.keep[e];
         }
@@ -346,12 +349,12 @@ bool slice_lengths_ok(int n)
     }
 }
 
-template <int N> static void stage_table(std::vector<double>& out)
+template <typename P> static void stage_table(std::vector<double>& out)
 {
     out.clear();
     int ns = 1;
-    for (int s = 0; s < Plan<N>::kStages; ++s) {
-        const int r = Plan<N>::kRadix[s];
+    for (int s = 0; s < P::kStages; ++s) {
+        const int r = P::kRadix[s];
         if (ns > 1)
             for (int q = 1; q < r; ++q)
                 for (int k = 0; k < ns; ++k) {
@@ -364,10 +367,35 @@ template <int N> static void stage_table(std::vector<double>& out)
     }
 }
 
-bool stage_twiddles(int n, std::vector<double>& re_im)
+template <int N> static bool stage_table_v(int v, std::vector<double>& out)
+{
+    if (v == 0) {
+        stage_table<Plan<N>>(out);
+        return (int)out.size() == 2 * plan_tw_len<Plan<N>>();
+    }
+    if constexpr (AltPlan<N>::exists) {
+        stage_table<AltPlan<N>>(out);
+        return (int)out.size() == 2 * plan_tw_len<AltPlan<N>>();
+    }
+    return false;
+}
+
+bool stage_twiddles(int n, std::vector<double>& re_im, int v)
 {
     switch (n) {
-#define MSF_CASE(N) case N: stage_table<N>(re_im); return (int)re_im.size() == 2 * stage_tw_len<N>();
+#define MSF_CASE(N) case N: return stage_table_v<N>(v, re_im);
+        MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
+        MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
+#undef MSF_CASE
+    default:
+        return false;
+    }
+}
+
+bool has_alt_plan(int n)
+{
+    switch (n) {
+#define MSF_CASE(N) case N: return AltPlan<N>::exists;
         MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
         MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
 #undef MSF_CASE
@@ -402,10 +430,10 @@ int col_tile(int nx)
     }
 }
 
-int col_tile_default(int nx)
+int col_tile_default(int nx, int v)
 {
     switch (nx) {
-#define MSF_CASE(N) case N: return ColTile<N>::W;
+#define MSF_CASE(N) case N: return v ? ColTile<N, 1>::W : ColTile<N>::W;
         MSF_CASE(16) MSF_CASE(32) MSF_CASE(64) MSF_CASE(96) MSF_CASE(128) MSF_CASE(192)
         MSF_CASE(256) MSF_CASE(384) MSF_CASE(512) MSF_CASE(768) MSF_CASE(1024)
 #undef MSF_CASE
@@ -416,19 +444,19 @@ int col_tile_default(int nx)
 
 // With ev0/ev1 the dispatch packet itself stamps the kernel's start and end
 // (hipExtLaunchKernelGGL), so profiled durations carry no marker overhead.
-template <int NY>
-static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
-                             hipEvent_t ev0, hipEvent_t ev1)
+template <int NY, int V>
+static hipError_t row_pass_nv(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
+                              hipEvent_t ev0, hipEvent_t ev1)
 {
-    constexpr int RPW = 256 / Plan<NY>::T;
+    constexpr int RPW = 256 / PlanV<NY, V>::T;
     const int64_t groups = (a.batch + RPW - 1) / RPW;
     const dim3 grid((unsigned)(groups * a.nx));
-    const dim3 block(RPW * Plan<NY>::T);
+    const dim3 block(RPW * PlanV<NY, V>::T);
 #define MSF_ROW(F, M, PR)                                                                          \
     do {                                                                                           \
-        if (ev0) hipExtLaunchKernelGGL((row_pass_kernel<NY, F, M, PR>), grid, block, 0, s, ev0,    \
-                                       ev1, 0, a);                                                 \
-        else hipLaunchKernelGGL((row_pass_kernel<NY, F, M, PR>), grid, block, 0, s, a);           \
+        if (ev0) hipExtLaunchKernelGGL((row_pass_kernel<NY, V, F, M, PR>), grid, block, 0, s,      \
+                                       ev0, ev1, 0, a);                                            \
+        else hipLaunchKernelGGL((row_pass_kernel<NY, V, F, M, PR>), grid, block, 0, s, a);        \
     } while (0)
     if (a.prune) {
         if (first) { if (tmode == 0) MSF_ROW(true, 0, true); else MSF_ROW(true, 1, true); }
@@ -439,6 +467,17 @@ static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStr
     }
 #undef MSF_ROW
     return hipGetLastError();
+}
+
+template <int NY>
+static hipError_t row_pass_n(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
+                             hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (a.plan == 0) return row_pass_nv<NY, 0>(first, tmode, a, s, ev0, ev1);
+    if constexpr (AltPlan<NY>::exists) {
+        if (a.plan == 1) return row_pass_nv<NY, 1>(first, tmode, a, s, ev0, ev1);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, hipStream_t s,
@@ -454,18 +493,18 @@ hipError_t launch_row_pass(int ny, bool first, int tmode, const RowPassArgs& a, 
     }
 }
 
-template <int NX, int W>
+template <int NX, int V, int W>
 static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s, hipEvent_t ev0,
                              hipEvent_t ev1)
 {
     if (a.ncp % W) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((int64_t)a.batch * (a.ncp / W)));
-    const dim3 block(W * Plan<NX>::T);
+    const dim3 block(W * PlanV<NX, V>::T);
 #define MSF_COL(L, PR)                                                                             \
     do {                                                                                           \
-        if (ev0) hipExtLaunchKernelGGL((col_pass_kernel<NX, L, W, PR>), grid, block, 0, s, ev0,    \
-                                       ev1, 0, a);                                                 \
-        else hipLaunchKernelGGL((col_pass_kernel<NX, L, W, PR>), grid, block, 0, s, a);           \
+        if (ev0) hipExtLaunchKernelGGL((col_pass_kernel<NX, V, L, W, PR>), grid, block, 0, s,      \
+                                       ev0, ev1, 0, a);                                            \
+        else hipLaunchKernelGGL((col_pass_kernel<NX, V, L, W, PR>), grid, block, 0, s, a);        \
     } while (0)
     if (a.prune) {
         if (last) MSF_COL(true, true);
@@ -478,23 +517,35 @@ static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s, hip
     return hipGetLastError();
 }
 
-template <int NX, int W> constexpr bool col_fits()
+template <int NX, int V, int W> constexpr bool col_fits()
 {
-    const size_t tables = col_lean<NX, W>() ? (size_t)NX * 8  // s_nat of the last slice
-                                             : (size_t)NX * 24 + (size_t)stage_tw_len<NX>() * 8;
-    return (size_t)W * col_len<NX, W>() * 8 + tables <= 160 * 1024 && W * Plan<NX>::T <= 1024;
+    using P = PlanV<NX, V>;
+    const size_t tables = col_lean<NX, V, W>() ? (size_t)NX * 8  // s_nat of the last slice
+                                                : (size_t)NX * 24 + (size_t)plan_tw_len<P>() * 8;
+    return (size_t)W * col_len<NX, W>() * 8 + tables <= 160 * 1024 && W * P::T <= 1024;
+}
+
+template <int NX, int V>
+static hipError_t col_pass_nv(bool last, const ColPassArgs& a, hipStream_t s, hipEvent_t ev0,
+                              hipEvent_t ev1)
+{
+    int w = ColTile<NX, V>::W;
+    if (a.tile == 8 || a.tile == 16) w = a.tile;
+    if constexpr (col_fits<NX, V, 16>()) {
+        if (w == 16) return col_pass_w<NX, V, 16>(last, a, s, ev0, ev1);
+    }
+    return col_pass_w<NX, V, 8>(last, a, s, ev0, ev1);
 }
 
 template <int NX>
 static hipError_t col_pass_n(bool last, const ColPassArgs& a, hipStream_t s, hipEvent_t ev0,
                              hipEvent_t ev1)
 {
-    int w = ColTile<NX>::W;
-    if (a.tile == 8 || a.tile == 16) w = a.tile;
-    if constexpr (col_fits<NX, 16>()) {
-        if (w == 16) return col_pass_w<NX, 16>(last, a, s, ev0, ev1);
+    if (a.plan == 0) return col_pass_nv<NX, 0>(last, a, s, ev0, ev1);
+    if constexpr (AltPlan<NX>::exists) {
+        if (a.plan == 1) return col_pass_nv<NX, 1>(last, a, s, ev0, ev1);
     }
-    return col_pass_w<NX, 8>(last, a, s, ev0, ev1);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t s,

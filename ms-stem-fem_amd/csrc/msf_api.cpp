@@ -167,11 +167,12 @@ struct msf_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fem[2] = {nullptr, nullptr};
     std::string err;
-    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0, 0, 0};
+    msf_config cfg{0, MSF_TRANSMIT_REFERENCE, 0, 0, 0, 0, 0, 0};
     msf_stats stats{};
-    std::map<int, DevBuf> tw32, tw64, twnat;
-    // autotuned launch shape per (nx, ny, ncp, min(P, 256)): batch, column tile, chains
-    std::map<std::array<int, 4>, std::array<int, 3>> tuned;
+    std::map<int, DevBuf> tw32, tw64, twnat, tw32alt;
+    // autotuned launch shape per (nx, ny, ncp, min(P, 256)): batch, column tile,
+    // chains, radix plan
+    std::map<std::array<int, 4>, std::array<int, 4>> tuned;
     RunPlan plan;
     std::set<msf_specimen*> specimens; // live specimens built on this context
 };
@@ -248,6 +249,34 @@ int twiddles(msf_ctx* c, int n, const float2** f32, const double2** f64,
     if (f64) *f64 = c->tw64[n].as<double2>();
     if (nat32) *nat32 = c->twnat[n].as<float2>();
     return MSF_OK;
+}
+
+// Stage twiddles of the alternate radix plan of length n (FP32), for the
+// slice passes only; null when the length has none.
+int alt_twiddles(msf_ctx* c, int n, const float2** f32)
+{
+    *f32 = nullptr;
+    if (!msf::has_alt_plan(n)) return MSF_OK;
+    if (!c->tw32alt.count(n)) {
+        std::vector<double> st;
+        if (!msf::stage_twiddles(n, st, 1)) return fail(c, MSF_ERR_UNSUPPORTED, "alternate plan table");
+        std::vector<float2> f(st.size() / 2);
+        for (size_t e = 0; e < f.size(); ++e) f[e] = float2{(float)st[2 * e], (float)st[2 * e + 1]};
+        HIPCHK(c, upload(c->tw32alt[n], f, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    *f32 = c->tw32alt[n].as<float2>();
+    return MSF_OK;
+}
+
+// Radix plan of one axis of the slice passes when none is configured or
+// tuned (0 primary, 1 alternate), by measurement on the 256^2 Si, 512^2 Cu/Zr
+// and 1024^2 Si workloads: the alternate plans (16 values per lane, one
+// fewer stage) are 6-7 % faster over a job at 256 and 1024 and 10 % slower
+// at 512.
+int fft_plan_rule(int n)
+{
+    return (msf::has_alt_plan(n) && n != 512) ? 1 : 0;
 }
 
 int check_grid(msf_ctx* c, const msf_grid* g)
@@ -601,9 +630,19 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     pl.ncp = ncp;
     pl.valid = true;
 
-    const float2 *twx, *twy, *twx_nat, *twy_nat;
+    const float2 *twx, *twy, *twx_nat, *twy_nat, *twx_alt, *twy_alt;
     if (int r = twiddles(c, nx, &twx, nullptr, &twx_nat)) return r;
     if (int r = twiddles(c, ny, &twy, nullptr, &twy_nat)) return r;
+    if (int r = alt_twiddles(c, nx, &twx_alt)) return r;
+    if (int r = alt_twiddles(c, ny, &twy_alt)) return r;
+    // radix plans as vx + 2 vy (per axis 0 primary, 1 alternate): configured
+    // (1 primary, 2 alternate where the length has one) or by the rule
+    const int all_alt = (twx_alt ? 1 : 0) + (twy_alt ? 2 : 0);
+    const int plan_rule = c->cfg.fft_plan == 1   ? 0
+                          : c->cfg.fft_plan == 2 ? all_alt
+                                                 : fft_plan_rule(nx) + 2 * fft_plan_rule(ny);
+    auto vx_of = [](int pv) { return pv & 1; };
+    auto vy_of = [](int pv) { return (pv >> 1) & 1; };
 
     // batch depth: the fewest batches whose wave buffer stays within 96 MiB,
     // spread evenly over the raster.  The buffer, the t and u0 slices and the
@@ -626,10 +665,13 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     // Autotuner (opt-in): with batch, column tile and chains all automatic, time one
     // step through every slice of this job's own specimen and shifts for each
     // candidate (batch depth around the rule's, column tile, one or two
-    // chains) on trial buffers once per shape, and keep the fastest per
+    // chains, and with fft_plan automatic the primary and alternate radix
+    // plans) on trial buffers once per shape, and keep the fastest per
     // position-slice (the faster of two timings each) if it beats the rule
-    // above by more than 3 %.  Every candidate gives bit-identical results.
+    // above by more than 3 %.  Candidates of one radix plan give bit-identical
+    // results; the two plans differ by FP32 rounding.
     int tuned_state = 0;
+    int plan_v = plan_rule;  // radix plans, vx + 2 vy
     const bool prof0 = c->cfg.profile_kernels != 0;
     if (c->cfg.autotune && c->cfg.batch == 0 && c->cfg.col_tile == 0 && c->cfg.chains == 0 &&
         !prof0 && !out.exit && P >= 2) {
@@ -661,18 +703,22 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             HIPCHK(c, hipEventCreate(&e0));
             HIPCHK(c, hipEventCreate(&e1));
             msf::RowPassArgs tr{};
-            tr.nx = nx; tr.ncp = ncp; tr.ka = ka; tr.kb = kb; tr.tw = twy;
+            tr.nx = nx; tr.ncp = ncp; tr.ka = ka; tr.kb = kb;
             tr.inv_s = (float)(1.0 / d.sqrt_nxny);
             tr.prune = msf::row_prunable(ny, ka, kb) ? 1 : 0;
             msf::ColPassArgs tc{};
             tc.ny = ny; tc.ncp = ncp; tc.ka = ka; tc.kb = kb; tc.keep = sp->dkeep.as<int2>();
-            tc.tw = twx; tc.psi0 = pl.Psi0.as<float2>(); tc.twx_nat = twx_nat; tc.twy = twy_nat;
+            tc.psi0 = pl.Psi0.as<float2>(); tc.twx_nat = twx_nat; tc.twy = twy_nat;
             tc.prune = msf::col_prunable(nx, keep.ab.data()) ? 1 : 0;
             // one trial: a steady-state step through every slice of the stack
             // (so t and u0 stream as in the job) on `ch` chains of b positions
             const int T = S;
-            auto trial = [&](int64_t b, int w, int ch, float& ms) -> int {
+            auto trial = [&](int64_t b, int w, int ch, int fp, float& ms) -> int {
                 hipStream_t st[2] = {c->stream, c->stream2};
+                tr.plan = vy_of(fp);
+                tr.tw = tr.plan ? twy_alt : twy;
+                tc.plan = vx_of(fp);
+                tc.tw = tc.plan ? twx_alt : twx;
                 auto steps = [&](int count) -> int {
                     if (ch == 2) {
                         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
@@ -712,32 +758,38 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                 }
                 return MSF_OK;
             };
-            const int tile0 = msf::col_tile_default(nx);
+            const int tile0 = msf::col_tile_default(nx, vx_of(plan_rule));
             const int def_ch = (B * plane >= (4ll << 20) && (P + B - 1) / B >= 2) ? 2 : 1;
             double best = 0.0, def_cost = 0.0;
-            std::array<int, 3> pick = {(int)B, tile0, def_ch};
+            std::array<int, 4> pick = {(int)B, tile0, def_ch, plan_rule};
+            // plans: the rule's, then all primary and all alternate
+            std::vector<int> plans = {plan_rule};
+            if (c->cfg.fft_plan == 0)
+                for (int pv : {0, all_alt})
+                    if (std::find(plans.begin(), plans.end(), pv) == plans.end()) plans.push_back(pv);
             int rc = MSF_OK;
             {
                 float ms = 0.f; // bring clocks and caches to steady state first
-                rc = trial(B, tile0, 1, ms);
+                rc = trial(B, tile0, 1, plan_rule, ms);
             }
-            for (int64_t b : bs)
-                for (int w : {8, 16})
-                    for (int ch : {1, 2}) {
-                        if (ch == 2 && 2 * b > ns) continue;
-                        float ms = 0.f;
-                        if ((rc = trial(b, w, ch, ms))) break;
-                        const double cost = ms / ((double)b * ch * T);
-                        if (b == B && w == tile0 && ch == def_ch) def_cost = cost;
-                        if (best == 0.0 || cost < best) {
-                            best = cost;
-                            pick = {(int)b, w, ch};
+            for (int fp : plans)
+                for (int64_t b : bs)
+                    for (int w : {8, 16})
+                        for (int ch : {1, 2}) {
+                            if (rc || (ch == 2 && 2 * b > ns)) continue;
+                            float ms = 0.f;
+                            if ((rc = trial(b, w, ch, fp, ms))) break;
+                            const double cost = ms / ((double)b * ch * T);
+                            if (b == B && w == tile0 && ch == def_ch && fp == plan_rule) def_cost = cost;
+                            if (best == 0.0 || cost < best) {
+                                best = cost;
+                                pick = {(int)b, w, ch, fp};
+                            }
                         }
-                    }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             if (rc) return rc;
-            if (def_cost > 0.0 && best > def_cost / 1.03) pick = {(int)B, tile0, def_ch};
+            if (def_cost > 0.0 && best > def_cost / 1.03) pick = {(int)B, tile0, def_ch, plan_rule};
             it = c->tuned.emplace(key, pick).first;
             tuned_state = 1;
         } else {
@@ -746,6 +798,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         B = std::min<int64_t>(it->second[0], P);
         col_tile = it->second[1];
         chains_cfg = it->second[2];
+        plan_v = it->second[3];
     }
 
     // work buffers, grown as needed and kept for the next run
@@ -872,7 +925,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ra.ncp = ncp;
     ra.ka = ka;
     ra.kb = kb;
-    ra.tw = twy;
+    ra.plan = vy_of(plan_v);
+    ra.tw = ra.plan ? twy_alt : twy;
     ra.inv_s = (float)(1.0 / d.sqrt_nxny);
     ra.prune = msf::row_prunable(ny, ka, kb) ? 1 : 0;
     msf::ColPassArgs ca{};
@@ -881,7 +935,8 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     ca.ka = ka;
     ca.kb = kb;
     ca.keep = dkeep.as<int2>();
-    ca.tw = twx;
+    ca.plan = vx_of(plan_v);
+    ca.tw = ca.plan ? twx_alt : twx;
     ca.psi0 = dPsi0.as<float2>();
     ca.twx_nat = twx_nat;
     ca.twy = twy_nat;
@@ -1015,9 +1070,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             (uint64_t)(uintptr_t)pl.seg_pix.p, (uint64_t)(uintptr_t)pl.tile_off.p,
             (uint64_t)(uintptr_t)pl.seg_kind.p, (uint64_t)(uintptr_t)pl.seg_of_group.p,
             (uint64_t)(uintptr_t)dcnt.p, (uint64_t)(uintptr_t)dzero.p,
-            (uint64_t)(uintptr_t)twx, (uint64_t)(uintptr_t)twy, (uint64_t)(uintptr_t)twx_nat,
+            (uint64_t)(uintptr_t)ca.tw, (uint64_t)(uintptr_t)ra.tw, (uint64_t)(uintptr_t)twx_nat,
             (uint64_t)(uintptr_t)twy_nat, (uint64_t)ncp, (uint64_t)G, (uint64_t)nbins,
-            (uint64_t)pl.n_tiles, (uint64_t)col_tile};
+            (uint64_t)pl.n_tiles, (uint64_t)col_tile, (uint64_t)plan_v};
         graph_state = 2;
         bool captured = true;
         if (!pl.graph || pl.graph_sig != sig) {
@@ -1097,8 +1152,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     st.col_pass_bytes = st.row_pass_bytes;
     st.graph = graph_state;
     st.chains = chains;
-    st.col_tile = col_tile ? col_tile : msf::col_tile_default(nx);
+    st.col_tile = col_tile ? col_tile : msf::col_tile_default(nx, ca.plan);
     st.autotuned = tuned_state;
+    st.row_fft_plan = 1 + ra.plan;
+    st.col_fft_plan = 1 + ca.plan;
     if (prof) {
         double sum[K_NUM] = {0}, cnt[K_NUM] = {0};
         for (size_t k = 0; k < ev_kind.size(); ++k) {
@@ -1173,6 +1230,7 @@ int msf_destroy(msf_ctx* c)
         c->tw32.clear();
         c->tw64.clear();
         c->twnat.clear();
+        c->tw32alt.clear();
         for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_fem[0], c->ev_fem[1]})
             if (e) (void)hipEventDestroy(e);
         if (c->stream2) {
@@ -1196,6 +1254,7 @@ int msf_set_config(msf_ctx* c, const msf_config* cfg)
     if (cfg->col_tile != 0 && cfg->col_tile != 8 && cfg->col_tile != 16)
         return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 8 or 16");
     if (cfg->chains < 0 || cfg->chains > 2) return fail(c, MSF_ERR_INVALID, "chains must be 0, 1 or 2");
+    if (cfg->fft_plan < 0 || cfg->fft_plan > 2) return fail(c, MSF_ERR_INVALID, "fft_plan must be 0, 1 or 2");
     c->cfg = *cfg;
     return MSF_OK;
 }

@@ -124,7 +124,8 @@ class ServeStats(C.Structure):
 class Config(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("transmit_mode", C.c_int32),
                 ("profile_kernels", C.c_int32), ("col_tile", C.c_int32),
-                ("no_graph", C.c_int32), ("chains", C.c_int32), ("autotune", C.c_int32)]
+                ("no_graph", C.c_int32), ("chains", C.c_int32), ("autotune", C.c_int32),
+                ("fft_plan", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -136,7 +137,8 @@ class Stats(C.Structure):
                 ("reduce_us", C.c_double), ("row_pass_count", C.c_uint64),
                 ("col_pass_count", C.c_uint64), ("row_pass_bytes", C.c_double),
                 ("col_pass_bytes", C.c_double), ("graph", C.c_int32), ("chains", C.c_int32),
-                ("col_tile", C.c_int32), ("autotuned", C.c_int32)]
+                ("col_tile", C.c_int32), ("autotuned", C.c_int32), ("row_fft_plan", C.c_int32),
+                ("col_fft_plan", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -430,9 +432,12 @@ class Context:
             raise MsfError(rc, f"{what}: {lib().msf_last_error(self._h).decode()}")
 
     def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False,
-                  col_tile=0, graph=True, chains=0, autotune=False):
+                  col_tile=0, graph=True, chains=0, autotune=False, fft_plan=0):
+        """fft_plan: 1 primary radix plans, 2 the alternate plans of lengths
+        256, 512 and 1024 (16 values per lane, one fewer stage), 0 automatic
+        (the alternate at 256 and 1024, or the autotuner's choice)."""
         cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0, col_tile,
-                     0 if graph else 1, chains, 1 if autotune else 0)
+                     0 if graph else 1, chains, 1 if autotune else 0, fft_plan)
         self._check(lib().msf_set_config(self._h, C.byref(cfg)), "msf_set_config")
 
     def stats(self):
