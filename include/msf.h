@@ -97,7 +97,7 @@ typedef struct msf_config {
     uint32_t batch;                    /* probe positions per launch; 0 = automatic */
     int32_t transmit_mode;             /* MSF_TRANSMIT_* */
     int32_t profile_kernels;           /* 1: time every hot launch with HIP events */
-    int32_t col_tile;                  /* columns per column-pass workgroup: 8, 16; 0 = automatic */
+    int32_t col_tile;                  /* columns per column-pass workgroup: 4, 8, 16; 0 = automatic */
     int32_t no_graph;                  /* 1: launch on the stream; 0: capture the job's launch
                                           sequence once per shape as a HIP graph and replay it */
     int32_t chains;                    /* batch chains in flight: 1, or 2 (alternate batches on

@@ -49,7 +49,7 @@ struct ColPassArgs {
     const float2* twx_nat;  // [nx] roots W_nx^e (last pass only)
     const float2* twy;      // [ny] roots W_ny^e (last pass only)
     int ny, ncp, ka, kb, batch;
-    int tile;               // columns per workgroup (8 or 16; 0 = default)
+    int tile;               // columns per workgroup (4, 8 or 16; 0 = default)
     int prune;              // 1: rows [3nx/8, 5nx/8) are masked for every column
     int plan;               // radix plan of length nx: 0 primary, 1 alternate (has_alt_plan)
 };

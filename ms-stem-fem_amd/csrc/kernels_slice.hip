@@ -76,16 +76,31 @@ __device__ __forceinline__ int xcd_band(int L, int G)
     return x * q + min(x, r) + k;
 }
 
+// lanes of a row-pass workgroup: RPW = row_threads / T rows share the staged
+// twiddle table and transmission row.  At 512 (primary plan) eight rows per
+// workgroup halve that staging per row (row pass 41.1 -> 40.4 us per 64
+// positions); at 1024 the larger LDS footprint costs more than it saves.
+template <int NY, int V> __host__ __device__ constexpr int row_threads()
+{
+    return (NY == 512 && V == 0) ? 512 : 256;
+}
+
+// Lean row passes read the twiddle table and the transmission row from
+// global memory (L2-resident, shared by the workgroups of a row band) instead
+// of staging them in LDS, so that more workgroups fit on a CU.
+template <int NY, int V> __host__ __device__ constexpr bool row_lean() { return NY == 1024; }
+
 template <int NY, int V, bool FIRST, int TMODE, bool PRUNE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ void __launch_bounds__((row_threads<NY, V>())) __attribute__((amdgpu_waves_per_eu(6)))
 row_pass_kernel(RowPassArgs a)
 {
     using P = PlanV<NY, V>;
     constexpr int E = P::E, T = P::T;
-    constexpr int RPW = 256 / T;
+    constexpr int RPW = row_threads<NY, V>() / T;
     constexpr int LEN = lds_len(NY);
-    __shared__ float2 s_tw[plan_tw_len<P>()];
-    __shared__ float2 s_t[NY];
+    constexpr bool LEAN = row_lean<NY, V>();
+    __shared__ float2 s_tw[LEAN ? 1 : plan_tw_len<P>()];
+    __shared__ float2 s_t[LEAN ? 1 : NY];
     __shared__ float2 s_buf[RPW * LEN];
 
     // workgroup-uniform decode (scalar unit)
@@ -96,11 +111,15 @@ row_pass_kernel(RowPassArgs a)
 
     // twiddles and the transmission row i, shared by the RPW rows of the group
     const float2* trow = a.t + (int64_t)i * NY;
-    for (int e = threadIdx.x; e < NY; e += blockDim.x) {
-        if (e < plan_tw_len<P>()) s_tw[e] = a.tw[e];
-        s_t[e] = trow[e];
+    if constexpr (!LEAN) {
+        for (int e = threadIdx.x; e < NY; e += blockDim.x) {
+            if (e < plan_tw_len<P>()) s_tw[e] = a.tw[e];
+            s_t[e] = trow[e];
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    const float2* tw = LEAN ? a.tw : s_tw;
+    const float2* tt = LEAN ? trow : s_t;
 
     // with T == 64 each row is one wavefront: keep its index in a scalar register
     const int lr = (T == 64) ? __builtin_amdgcn_readfirstlane(threadIdx.x / T) : threadIdx.x / T;
@@ -130,7 +149,7 @@ row_pass_kernel(RowPassArgs a)
                 x[p] = (lo || up) ? v : float2{0.f, 0.f};
             }
         }
-        P::template run<true, float2, (T <= 64)>(x, lds, s_tw, t);
+        P::template run<true, float2, (T <= 64)>(x, lds, tw, t);
     }
 
     const int2 sh = a.shift[b];
@@ -143,7 +162,7 @@ row_pass_kernel(RowPassArgs a)
 #pragma unroll
     for (int p = 0; p < E; ++p) {
         const int j = t + T * p;
-        const float2 tv = s_t[j];
+        const float2 tv = tt[j];
         const float2 u0 = urow[T * p];
         const float da = tv.x * u0.x - tv.y * u0.y;
         const float db = (TMODE == 0) ? tv.x * u0.y + tv.y * (u0.x * is + da) : tv.x * u0.y + tv.y * u0.x;
@@ -158,7 +177,7 @@ row_pass_kernel(RowPassArgs a)
         }
     }
 
-    P::template run<false, float2, (T <= 64)>(x, lds, s_tw, t);
+    P::template run<false, float2, (T <= 64)>(x, lds, tw, t);
 
     if (active) {
 #pragma unroll
@@ -188,7 +207,10 @@ template <int NX, int V = 0> struct ColTile {
     static constexpr int WMAX = 16;  // ncp is a multiple of this
     // default tile: keep LDS (W * lds_len * 8 B) near 40 KiB so that several
     // workgroups share a CU
-    static constexpr int W = (16 * lds_len(NX) * 8 > 48 * 1024) ? 8 : 16;
+    // The alternate 1024 plan (16 values on 64 lanes, ~190 registers) fits
+    // one 8-column tile per CU; 4-column tiles with lean tables fit two
+    // (column pass 83 -> 74.5 us per 17 positions).
+    static constexpr int W = (NX == 1024 && V == 1) ? 4 : (16 * lds_len(NX) * 8 > 48 * 1024) ? 8 : 16;
 };
 
 // column stride of the LDS tile: a 16-lane group of the coalesced load/store
@@ -207,6 +229,7 @@ template <int NX, int W> __host__ __device__ constexpr int col_len()
 // workgroup) and only the tile stays in LDS.
 template <int NX, int V, int W> __host__ __device__ constexpr bool col_lean()
 {
+    if (W == 4) return NX >= 512; // small tiles: tables would outweigh the tile
     return 2 * ((size_t)W * col_len<NX, W>() * 8 + (size_t)plan_tw_len<PlanV<NX, V>>() * 8 +
                 (size_t)NX * 16) > 160 * 1024;
 }
@@ -448,7 +471,7 @@ template <int NY, int V>
 static hipError_t row_pass_nv(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
                               hipEvent_t ev0, hipEvent_t ev1)
 {
-    constexpr int RPW = 256 / PlanV<NY, V>::T;
+    constexpr int RPW = row_threads<NY, V>() / PlanV<NY, V>::T;
     const int64_t groups = (a.batch + RPW - 1) / RPW;
     const dim3 grid((unsigned)(groups * a.nx));
     const dim3 block(RPW * PlanV<NY, V>::T);
@@ -530,10 +553,11 @@ static hipError_t col_pass_nv(bool last, const ColPassArgs& a, hipStream_t s, hi
                               hipEvent_t ev1)
 {
     int w = ColTile<NX, V>::W;
-    if (a.tile == 8 || a.tile == 16) w = a.tile;
+    if (a.tile == 4 || a.tile == 8 || a.tile == 16) w = a.tile;
     if constexpr (col_fits<NX, V, 16>()) {
         if (w == 16) return col_pass_w<NX, V, 16>(last, a, s, ev0, ev1);
     }
+    if (w == 4) return col_pass_w<NX, V, 4>(last, a, s, ev0, ev1);
     return col_pass_w<NX, V, 8>(last, a, s, ev0, ev1);
 }
 

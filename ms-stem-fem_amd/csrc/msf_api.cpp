@@ -774,7 +774,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             }
             for (int fp : plans)
                 for (int64_t b : bs)
-                    for (int w : {8, 16})
+                    for (int w : {4, 8, 16})
                         for (int ch : {1, 2}) {
                             if (rc || (ch == 2 && 2 * b > ns)) continue;
                             float ms = 0.f;
@@ -1251,8 +1251,8 @@ int msf_set_config(msf_ctx* c, const msf_config* cfg)
     MSF_LOCK(c);
     if (cfg->transmit_mode != MSF_TRANSMIT_REFERENCE && cfg->transmit_mode != MSF_TRANSMIT_EXACT)
         return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
-    if (cfg->col_tile != 0 && cfg->col_tile != 8 && cfg->col_tile != 16)
-        return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 8 or 16");
+    if (cfg->col_tile != 0 && cfg->col_tile != 4 && cfg->col_tile != 8 && cfg->col_tile != 16)
+        return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 4, 8 or 16");
     if (cfg->chains < 0 || cfg->chains > 2) return fail(c, MSF_ERR_INVALID, "chains must be 0, 1 or 2");
     if (cfg->fft_plan < 0 || cfg->fft_plan > 2) return fail(c, MSF_ERR_INVALID, "fft_plan must be 0, 1 or 2");
     c->cfg = *cfg;
