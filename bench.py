@@ -151,11 +151,159 @@ def serving_sample(msf, SP, n_jobs, rate, slots, seed=5000):
                    "Si or Cu/Zr, Poisson arrivals; 512eq = probe-slices x (N/512)^2"}
 
 
+def _base_line(value, unit, steps, warmup, ms, config, roofline, extra):
+    out = {"metric": METRIC, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": config, "roofline": roofline, "cpu_baseline": None}
+    out.update(extra)
+    return out
+
+
+def single_job(args, msf, desc, seed, grid, q, Z, min_slice, optics, xp, yp, det, fem):
+    """One job timed over `steps` runs after `warmup` runs, plus a profiled
+    run for the dominant slice pass (compulsory bytes over its HIP-event
+    duration)."""
+    ctx = msf.Context(0)
+    spec = ctx.specimen(grid, q, Z, min_slice)
+    S, P = spec.n_slices, xp.size * yp.size
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
+    for _ in range(args.warmup):
+        ctx.adfstem(spec, optics, xp, yp, detector=det, fem=fem)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        ctx.adfstem(spec, optics, xp, yp, detector=det, fem=fem)
+        times.append(time.perf_counter() - t0)
+    st = ctx.stats()
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, profile_kernels=True)
+    ctx.adfstem(spec, optics, xp, yp, detector=det, fem=fem)
+    pst = ctx.stats()
+    kern = {"row_pass": (pst["row_pass_us"], pst["row_pass_bytes"]),
+            "col_pass": (pst["col_pass_us"], pst["col_pass_bytes"])}
+    dom = max(kern, key=lambda k: kern[k][0])
+    us, nbytes = kern[dom]
+    achieved = nbytes / (us * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": nbytes, "avg_us": round(us, 2),
+                "other_kernels_us": {k: round(v[0], 2) for k, v in kern.items() if k != dom}}
+    value = P * S * len(times) / sum(times)
+    config = {"workload": desc, "grid": int(grid.nx), "atoms": int(Z.size), "slices": int(S),
+              "positions": int(P), "batch": int(st["batch"]), "chains": int(st["chains"]),
+              "col_tile": int(st["col_tile"]), "graph": int(st["graph"]),
+              "fft_plan": [int(st["row_fft_plan"]), int(st["col_fft_plan"])], "seed": seed}
+    spec.free()
+    ctx.close()
+    return _base_line(value, "probe-slice propagations/s", args.steps, args.warmup,
+                      1e3 * sum(times) / len(times), config, roofline,
+                      {"slice_loop_ms": round(st["slice_loop_ms"], 3)})
+
+
+def uneven_batch(args, msf, SP):
+    """48 uneven jobs (N 256/384/512): first run one after another on one
+    context with the autotuner and graph replay (an untimed pass tunes each
+    shape, a second is timed), then submitted together to the serving engine
+    (3 slots); each job builds its own specimen."""
+    log("generating 48 uneven jobs (seeds 3000-3047)")
+    jobs = SP.uneven_jobs(48, seed=3000)
+    prep = []
+    for j in jobs:
+        grid = msf.make_grid(j["nx"], j["ny"], j["cell"][0], j["cell"][1], j["cell"][2], j["voltage"])
+        prep.append((grid, msf.Optics(0, 0.0, 0.0, 0.0, j["alpha_max"]),
+                     msf.Detector(1, j["det_inner"], j["det_outer"], msf.ADF_REFERENCE)))
+    ctx = msf.Context(0)
+    ctx.configure(autotune=True, fft_plan=args.fft_plan)
+    slices = []
+    lat = []
+    t_all = 0.0
+    for rep in range(2):
+        for j, (grid, optics, det) in zip(jobs, prep):
+            t0 = time.perf_counter()
+            spec = ctx.specimen(grid, j["q"], j["Z"], j["min_slice"])
+            ctx.adfstem(spec, optics, j["x_p"], j["y_p"], detector=det)
+            n_sl = spec.n_slices
+            spec.free()
+            dt = time.perf_counter() - t0
+            if rep == 0:
+                slices.append(n_sl)
+            else:
+                lat.append(dt)
+                t_all += dt
+        log(f"direct pass {rep}: done")
+    ctx.close()
+    work = sum(j["x_p"].size * j["y_p"].size * s for j, s in zip(jobs, slices))
+    work512 = sum(j["x_p"].size * j["y_p"].size * s * (j["nx"] / 512) ** 2
+                  for j, s in zip(jobs, slices))
+    direct = {"props_per_s": round(work / t_all, 1), "props_512eq_per_s": round(work512 / t_all, 1),
+              "makespan_s": round(t_all, 3),
+              "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
+                             "p99": round(1e3 * float(np.percentile(lat, 99)), 2)}}
+    log(f"direct: {direct['props_per_s']:.4g} props/s")
+    # one batched submission through the serving engine
+    ctx = msf.Context(0)
+    try:
+        ctx.serve_configure(slots=3)
+        t0 = time.perf_counter()
+        tickets = [ctx.submit(g, j["q"], j["Z"], j["min_slice"], o, j["x_p"], j["y_p"], detector=d)
+                   for j, (g, o, d) in zip(jobs, prep)]
+        slat = []
+        for t in tickets:
+            _, _, info = ctx.wait(t, info=True)
+            slat.append(info["queue_ms"] + info["run_ms"])
+        span = time.perf_counter() - t0
+        sst = ctx.serve_stats()
+    finally:
+        ctx.close()
+    Ns = [j["nx"] for j in jobs]
+    config = {"workload": "uneven_stem_48: 48 Si jobs, N 256/384/512, cubic 20-50 A at 0.05 A^-3, "
+                          "minslice 2.0, rasters 16-48 per side, ADF 50-180 mrad, 100 kV, 20 mrad",
+              "jobs": 48, "grids": {str(n): Ns.count(n) for n in (256, 384, 512)},
+              "atoms": int(sum(j["Z"].size for j in jobs)), "slices": int(sum(slices)),
+              "positions": int(sum(j["x_p"].size * j["y_p"].size for j in jobs)),
+              "probe_slices": int(work), "slots": 3, "seed": 3000}
+    return _base_line(work / span, "probe-slice propagations/s", 1, 0, 1e3 * span, config, None,
+                      {"props_512eq_per_s": round(work512 / span, 1),
+                       "latency_ms": {"p50": round(float(np.percentile(slat, 50)), 1),
+                                      "p99": round(float(np.percentile(slat, 99)), 1)},
+                       "max_running": sst["max_running"],
+                       "peak_admitted_gib": round(sst["peak_admitted_bytes"] / 2 ** 30, 2),
+                       "direct_autotuned_graph": direct})
+
+
+def other_workload(args):
+    import msf
+    import specimens as SP
+    if args.workload == "uneven48":
+        out = uneven_batch(args, msf, SP)
+    elif args.workload == "si256":
+        job, q, Z, xp, yp = SP.config_si_256(seed=1001)
+        grid = msf.make_grid(256, 256, 20.0, 20.0, 20.0, job["voltage"])
+        out = single_job(args, msf, "si_stem_256: 400 Si atoms, 20 A cube, 256x256, 300 kV, Cs 0, "
+                         "10 mrad, 16x16 positions over the central 10x10 A, ADF 40-200 mrad",
+                         1001, grid, q, Z, job["min_slice"],
+                         msf.Optics(0, 0.0, 0.0, 0.0, job["alpha_max"]), xp, yp,
+                         msf.Detector(1, job["det_inner"], job["det_outer"], msf.ADF_REFERENCE), None)
+    else:
+        log("generating 32000 Si atoms (seed 4001)")
+        q, Z = SP.amorphous(32000, (80.0, 80.0, 100.0), 2.0, 4001)
+        grid = msf.make_grid(1024, 1024, 80.0, 80.0, 100.0, 300e3)
+        xp = 8.0 + 0.5 * np.arange(128)
+        out = single_job(args, msf, "si_fem_1024: 32000 Si atoms, 80x80x100 A, 1024x1024, 300 kV, "
+                         "Cs 0, 20 mrad, 128x128 positions at 0.5 A, ADF 60-200 mrad, V_omega",
+                         4001, grid, q, Z, 2.0, msf.Optics(0, 0.0, 0.0, 0.0, 0.020), xp, xp.copy(),
+                         msf.Detector(1, 0.060, 0.200, msf.ADF_REFERENCE), msf.Fem(1, 0.1414213562))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cuzr512", choices=["cuzr512", "si256", "uneven48", "si1024"],
+                    help="cuzr512: the headline 512^2 FEM job (default); si256, uneven48, si1024: the "
+                         "other benchmark configurations, one JSON line each")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--fft-plan", type=int, default=0,
                     help="radix plans: 1 primary, 2 alternate, 0 automatic per length")
@@ -173,6 +321,8 @@ def main():
               "build (positions would shard across GPUs with no exchange); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload != "cuzr512":
+        return other_workload(args)
 
     import msf
     import specimens as SP

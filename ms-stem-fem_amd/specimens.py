@@ -137,6 +137,28 @@ def config_cuzr_512(seed=2001):
     return job, q, Z, xp, xp.copy()
 
 
+def uneven_jobs(n_jobs=48, seed=3000):
+    """Batched uneven-size STEM jobs: job j draws from numpy's default_rng(seed
+    + j) N in {256, 384, 512}, a cubic cell side U[20, 50] A and a raster side
+    U{16..48} (positions at spacing L/side over the cell); amorphous Si at
+    0.05 A^-3 with a 2.0 A minimum separation (seed + j); minslice 2.0;
+    ADF 50-180 mrad.  100 kV and a 20 mrad aperture (the workload names
+    neither): at 300 kV the 50 mrad inner angle lies beyond the 2/3 band
+    limit lambda * N / (3 L) of the coarser grids (N 256 with L > 33.6 A),
+    which the reference rejects (tem.cpp:721-732)."""
+    jobs = []
+    for j in range(n_jobs):
+        rng = np.random.default_rng(seed + j)
+        N = int(rng.choice([256, 384, 512]))
+        L = float(rng.uniform(20.0, 50.0))
+        side = int(rng.integers(16, 49))
+        q, Z = amorphous(int(round(0.05 * L ** 3)), (L, L, L), 2.0, seed + j)
+        jobs.append(dict(nx=N, ny=N, cell=(L, L, L), q=q, Z=Z, x_p=raster(side, L),
+                         y_p=raster(side, L), voltage=100e3, alpha_max=0.020, det_inner=0.050,
+                         det_outer=0.180, min_slice=2.0))
+    return jobs
+
+
 def serving_mix(n_jobs, seed=5000, rate=4.0):
     """Seeded job stream in the manner of a serving workload: Poisson
     arrivals at `rate` jobs/s; N in {256, 512, 1024} with weights
