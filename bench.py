@@ -131,6 +131,8 @@ def serving_sample(msf, SP, n_jobs, rate, slots, seed=5000):
             tickets.append(ctx.submit(grid, j["q"], j["Z"], j["min_slice"], optics, j["x_p"],
                                       j["y_p"], detector=det, fem=msf.Fem(1, j["dr"])))
             work += j["x_p"].size * j["y_p"].size * j["slices"] * (j["nx"] * j["ny"]) / 512 ** 2
+            if len(tickets) % 100 == 0:
+                log(f"serving: {len(tickets)}/{n_jobs} jobs submitted at t = {time.perf_counter() - t0:.1f} s")
         lat = []
         for t in tickets:
             _, _, info = ctx.wait(t, info=True)
