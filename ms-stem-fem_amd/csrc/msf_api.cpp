@@ -1615,6 +1615,8 @@ int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lowe
     return MSF_OK;
 }
 
+} // extern "C"
+
 namespace {
 
 // Every pixel of the grid kept: the full [nx][ny] layout as its own compact plane.
@@ -1653,6 +1655,7 @@ int group_sums_host(msf_ctx* c, const msf::Groups& g, const msf::Domain& d, cons
 }
 
 } // namespace
+extern "C" {
 
 int msf_adf_reduce(msf_ctx* c, const msf_grid* grid, const msf_detector* det,
                    const double* psi_k, uint32_t count, double* adf)
@@ -1893,6 +1896,8 @@ int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, cons
 }
 
 // ---------------------------------------------------------------- serving
+} // extern "C"
+
 namespace {
 
 double ms_since(double t0) { return now_ms() - t0; }
@@ -2013,6 +2018,7 @@ int start_serving(msf_ctx* c)
 }
 
 } // namespace
+extern "C" {
 
 int msf_submit(msf_ctx* c, const msf_job* j, uint64_t* id)
 {
