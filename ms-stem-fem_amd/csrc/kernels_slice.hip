@@ -234,11 +234,18 @@ template <int NX, int V, int W> __host__ __device__ constexpr bool col_lean()
                 (size_t)NX * 16) > 160 * 1024;
 }
 
-// The alternate plans hold 16 values per lane: their launch bound is the
-// tile's real size, so that a 256- or 512-lane tile may use more than the
-// 128 registers a 1024-lane bound leaves per lane.
+// Plans with 12 or more values per lane (the alternate plans, and the
+// primary plans of 96, 192, 384 and 768) take the tile's real size as their
+// launch bound, so that a 128- to 512-lane tile may use more than the 128
+// registers a 1024-lane bound leaves per lane (384: 116 B of spills per lane
+// under that bound).
+template <int NX, int V, int W> __host__ __device__ constexpr int col_bound()
+{
+    return (V || PlanV<NX, V>::E >= 12) ? W * PlanV<NX, V>::T : 1024;
+}
+
 template <int NX, int V, bool LAST, int W, bool PRUNE>
-__global__ void __launch_bounds__((V ? W * PlanV<NX, V>::T : 1024)) col_pass_kernel(ColPassArgs a)
+__global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPassArgs a)
 {
     using P = PlanV<NX, V>;
     constexpr int E = P::E, T = P::T;
