@@ -397,6 +397,7 @@ void fem_pixel_modes(const msf::Domain& d, const msf::Groups& g, const std::vect
     const int nb = g.nbins;
     const double np = (double)P;
     const double l2 = d.lambda * d.lambda, a2 = alpha_max * alpha_max; // adfstem.cpp:95-96
+    const std::vector<double> kx2 = msf::pow2_all(d.kx), ky2 = msf::pow2_all(d.ky);
     for (int ii = 0; ii < nb; ++ii) {
         double S2 = 0.0, Q2 = 0.0, S1 = 0.0, D = 0.0;
         for (int32_t k = g.bin_off[ii]; k < g.bin_off[ii + 1]; ++k) {
@@ -409,7 +410,7 @@ void fem_pixel_modes(const msf::Domain& d, const msf::Groups& g, const std::vect
             if (out.omega_vimage) {
                 const int i = g.bin_full[k] / d.ny, j = g.bin_full[k] % d.ny;
                 double v = (sqr / (mag * mag)) - 1.0;
-                const double ksqr = msf::ref_pow2(d.kx[i]) + msf::ref_pow2(d.ky[j]);
+                const double ksqr = kx2[i] + ky2[j];
                 if (l2 * ksqr <= a2) v = 0.0;
                 D += v;
             }
@@ -435,6 +436,7 @@ void fem_pixel_images(const msf::Domain& d, const msf::RowRanges& keep, int ncp,
     const int A = keep.amax, B = keep.bmax;
     const double np = (double)P;
     const double l2 = d.lambda * d.lambda, a2 = alpha_max * alpha_max; // adfstem.cpp:95-96
+    const std::vector<double> kx2 = msf::pow2_all(d.kx), ky2 = msf::pow2_all(d.ky);
     for (int i = 0; i < d.nx; ++i)
         for (int j = 0; j < d.ny; ++j) {
             const int jc = (j < A) ? j : (j >= d.ny - B ? j - d.ny + A + B : -1);
@@ -446,7 +448,7 @@ void fem_pixel_images(const msf::Domain& d, const msf::RowRanges& keep, int ncp,
             if (out.mean_sqr) out.mean_sqr[k] = sqr;
             if (out.var2d) {
                 double v = (sqr / (mag * mag)) - 1.0;
-                const double ksqr = msf::ref_pow2(d.kx[i]) + msf::ref_pow2(d.ky[j]);
+                const double ksqr = kx2[i] + ky2[j];
                 if (l2 * ksqr <= a2) v = 0.0;
                 out.var2d[k] = v;
             }

@@ -59,12 +59,22 @@ Domain make_domain(int nx, int ny, double xperiod, double yperiod, double xmin, 
     return d;
 }
 
+// ref_pow2 of every kx and ky, once per index rather than once per pixel
+// (the same values: the same function of the same arguments)
+std::vector<double> pow2_all(const std::vector<double>& k)
+{
+    std::vector<double> out(k.size());
+    for (size_t i = 0; i < k.size(); ++i) out[i] = ref_pow2(k[i]);
+    return out;
+}
+
 RowRanges slice_keep_ranges(const Domain& d)
 {
     const double c_bw = ref_pow2(d.bw_t);   // bw_limit.cpp:36
     const double c_pr = d.bw_t * d.bw_t;     // slice.cpp:1375
+    const std::vector<double> kx2 = pow2_all(d.kx), ky2 = pow2_all(d.ky);
     return row_ranges(d.nx, d.ny, [&](int i, int j) {
-        const double k2 = ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]);
+        const double k2 = kx2[i] + ky2[j];
         return !(k2 >= c_bw) && (k2 < c_pr);
     });
 }
@@ -72,17 +82,15 @@ RowRanges slice_keep_ranges(const Domain& d)
 RowRanges bwlimit_keep_ranges(const Domain& d)
 {
     const double c_bw = ref_pow2(d.bw_t);
-    return row_ranges(d.nx, d.ny, [&](int i, int j) {
-        return !(ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]) >= c_bw);
-    });
+    const std::vector<double> kx2 = pow2_all(d.kx), ky2 = pow2_all(d.ky);
+    return row_ranges(d.nx, d.ny, [&](int i, int j) { return !(kx2[i] + ky2[j] >= c_bw); });
 }
 
 RowRanges template_keep_ranges(const Domain& d)
 {
     const double c2 = d.bw_pap * d.bw_pap; // scatterer_pap.hpp:82
-    return row_ranges(d.nx, d.ny, [&](int i, int j) {
-        return ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]) < c2;
-    });
+    const std::vector<double> kx2 = pow2_all(d.kx), ky2 = pow2_all(d.ky);
+    return row_ranges(d.nx, d.ny, [&](int i, int j) { return kx2[i] + ky2[j] < c2; });
 }
 
 RowRanges aperture_keep_ranges(const Domain& d, double alpha_max)
@@ -97,10 +105,17 @@ RowRanges aperture_keep_ranges(const Domain& d, double alpha_max)
 
 int64_t shift_index(const std::vector<double>& grid, double q, int64_t previous)
 {
+    // The reference scans for the first i with grid[i] - q >= half.  The grid
+    // is non-decreasing and rounded subtraction of a fixed q is monotone, so
+    // the test holds from that index on and a bisection finds the same i.
     const double half = 0.5 * (grid[1] - grid[0]);
-    for (size_t i = 0; i < grid.size(); ++i)
-        if (grid[i] - q >= half) return (int64_t)i - 1;
-    return previous;
+    size_t lo = 0, hi = grid.size();
+    while (lo < hi) {
+        const size_t mid = lo + (hi - lo) / 2;
+        if (grid[mid] - q >= half) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo < grid.size() ? (int64_t)lo - 1 : previous;
 }
 
 Slices autoslice(const double* q, uint32_t n, double zmin, double zperiod, double min_thickness)
@@ -208,9 +223,10 @@ std::vector<uint8_t> outside_aperture(const Domain& d, double alpha_max)
 {
     const double l2 = d.lambda * d.lambda, a2 = alpha_max * alpha_max; // adfstem.cpp:95-96
     std::vector<uint8_t> o((size_t)d.nx * d.ny);
+    const std::vector<double> kx2 = pow2_all(d.kx), ky2 = pow2_all(d.ky);
     for (int i = 0; i < d.nx; ++i)
         for (int j = 0; j < d.ny; ++j) {
-            const double ksqr = ref_pow2(d.kx[i]) + ref_pow2(d.ky[j]);
+            const double ksqr = kx2[i] + ky2[j];
             o[(size_t)i * d.ny + j] = (l2 * ksqr > a2) ? 1 : 0;
         }
     return o;

@@ -19,6 +19,8 @@ constexpr double kRefPi = 3.14159265369; // the reference's PI macro (slice.cpp:
 // libm pow(x, 2.0), as the reference's -O0 build calls it (build/Makefile:37-45);
 // compilers fold a literal pow(x, 2) into x*x, which rounds differently.
 double ref_pow2(double x);
+// ref_pow2 of every element
+std::vector<double> pow2_all(const std::vector<double>& k);
 
 struct Domain {
     int nx = 0, ny = 0;

@@ -91,6 +91,26 @@ def test_probe_shifts_match_oracle(msf, O):
     assert e.value.code == -5
 
 
+@pytest.mark.parametrize("n, L", [(512, 40.0), (384, 37.3), (1024, 80.0), (96, 7.5)])
+def test_probe_shift_search_equals_the_linear_scan(msf, O, n, L):
+    """The library bisects for the first grid index with x[i] - q >= dx/2;
+    the oracle scans for it as the reference does (adfstem.cpp:1590-1606).
+    Random positions, positions exactly on the grid points and exactly half a
+    pixel from them (where the test flips), and positions past the last
+    point (the shift persists from the previous position)."""
+    g = msf.make_grid(n, n, L, L, 5.0, 300e3)
+    xx, yy = O.domains(n, n, L, L)[2:4]
+    rng = np.random.default_rng(n)
+    dx = xx[1] - xx[0]
+    x_p = np.concatenate([rng.random(200) * L, xx[::7], xx[::5] + dx / 2, xx[::11] - dx / 2,
+                          np.nextafter(xx[::9] + dx / 2, 0), [L - 1e-9, L - dx / 4]])
+    x_p = x_p[x_p >= 0]
+    y_p = np.array([0.0, yy[-1], yy[n // 3] + (yy[1] - yy[0]) / 2])
+    got = msf.probe_shifts(g, x_p, y_p)
+    want = np.stack(O.probe_shifts(xx, yy, x_p, y_p), axis=1)
+    np.testing.assert_array_equal(got, want)
+
+
 @pytest.mark.parametrize("n, L, dr", [(128, 12.0, 0.5), (256, 20.0, 0.70710678118654746),
                                       (64, 40.0, 1.0), (512, 51.2, 0.25)])
 def test_correlograph_bins_match_oracle(msf, O, n, L, dr):
