@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <thread>
 
 namespace msf {
 
@@ -232,6 +233,35 @@ std::vector<uint8_t> outside_aperture(const Domain& d, double alpha_max)
     return o;
 }
 
+// std::stable_sort on up to 8 threads: stable-sorted chunks, then pairwise
+// std::inplace_merge, which keeps the left chunk's elements first among
+// equals, so the order is exactly std::stable_sort's.
+template <class It, class Cmp> static void stable_sort_threads(It first, It last, Cmp cmp)
+{
+    const size_t n = (size_t)(last - first);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    unsigned parts = 1;
+    while (parts * 2 <= std::min(8u, hw)) parts *= 2;
+    if (n < (1u << 16) || parts < 2) {
+        std::stable_sort(first, last, cmp);
+        return;
+    }
+    std::vector<size_t> b(parts + 1);
+    for (unsigned k = 0; k <= parts; ++k) b[k] = n * k / parts;
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < parts; ++k)
+        th.emplace_back([&, k] { std::stable_sort(first + b[k], first + b[k + 1], cmp); });
+    for (auto& t : th) t.join();
+    for (unsigned w = 1; w < parts; w *= 2) {
+        th.clear();
+        for (unsigned k = 0; k + w < parts; k += 2 * w)
+            th.emplace_back([&, k, w] {
+                std::inplace_merge(first + b[k], first + b[k + w], first + b[std::min(k + 2 * w, parts)], cmp);
+            });
+        for (auto& t : th) t.join();
+    }
+}
+
 Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, double k1,
                     double k2, bool fem, double dr, double alpha_max, bool fem_sqr,
                     bool correlograph)
@@ -266,10 +296,10 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
         for (int i = 0; i < nx; ++i)
             for (int j = 0; j < ny; ++j)
                 m.emplace_back(d.kx[i] * d.kx[i] + d.ky[j] * d.ky[j], (int64_t)i * ny + j);
-        std::stable_sort(m.begin(), m.end(),
-                         [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
-                             return a.first < b.first;
-                         });
+        stable_sort_threads(m.begin(), m.end(),
+                            [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+                                return a.first < b.first;
+                            });
         size_t it = 0;
         g.fem_abs = (int)g.kind.size();
         g.bin_off.push_back(0);
