@@ -79,6 +79,11 @@ struct RunPlan {
     DevBuf U0, Psi0;                          // vacuum waves
     DevBuf seg_pix, tile_off, seg_kind, seg_of_group, cnt, zero;  // reduction groups
     int n_tiles = 0;
+    // the pixel groups depend on the grid, the aperture and the detector/FEM
+    // settings only: they are kept across specimens on the same grid
+    bool groups_ok = false;
+    msf_grid ggrid{};
+    double galpha = 0.0;
     bool fem_sqr = false;                     // groups carry the |psi|^2 FEM rings (Vbar_r)
     bool corr = false;                        // groups carry the correlograph sectors
     msf::Groups hg;                           // host copy for the per-pixel FEM modes
@@ -545,7 +550,20 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                    pl.mtf_v.size() == fem->mtf_n;
     const bool same_optics = pl.valid && pl.spec_id == sp->id &&
                              std::memcmp(&pl.optics, o, sizeof(msf_optics)) == 0;
-    const bool same_groups = same_optics && std::memcmp(&pl.det, &dkey, sizeof dkey) == 0 &&
+    msf_grid gkey;
+    std::memset(&gkey, 0, sizeof gkey);
+    gkey.nx = sp->grid.nx;
+    gkey.ny = sp->grid.ny;
+    gkey.xperiod = sp->grid.xperiod;
+    gkey.yperiod = sp->grid.yperiod;
+    gkey.zperiod = sp->grid.zperiod;
+    gkey.voltage = sp->grid.voltage;
+    gkey.xmin = sp->grid.xmin;
+    gkey.ymin = sp->grid.ymin;
+    gkey.zmin = sp->grid.zmin;
+    const bool same_groups = pl.groups_ok && std::memcmp(&pl.ggrid, &gkey, sizeof gkey) == 0 &&
+                             pl.galpha == o->alpha_max &&
+                             std::memcmp(&pl.det, &dkey, sizeof dkey) == 0 &&
                              std::memcmp(&pl.fem, &fkey, sizeof fkey) == 0 &&
                              pl.fem_sqr == want_d2 && pl.corr == want_corr && same_mtf;
     if (!same_optics || !same_groups) pl.valid = false;
@@ -577,6 +595,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, upload(pl.zero, g.zeroed, c->stream));
         pl.cap_b = 0; // partial-sum buffers depend on G
         pl.hg = g;
+        pl.groups_ok = false; // until the tables below are complete
+        pl.ggrid = gkey;
+        pl.galpha = o->alpha_max;
         pl.mtf_on = false;
         pl.mtf_v.clear();
         pl.mtf_d.clear();
@@ -593,6 +614,7 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
             pl.mtf_rate = fem->mtf_sampling_rate;
             pl.mtf_on = true;
         }
+        pl.groups_ok = true;
     }
     const int G = pl.G;
     const int nbins = pl.nbins;

@@ -345,3 +345,38 @@ def test_autotuned_launch_shape_gives_identical_results(msf, ctx):
             np.testing.assert_array_equal(v, v0)
     finally:
         ctx.configure()
+
+
+def test_pixel_groups_kept_across_specimens_on_one_grid(msf, ctx):
+    """The detector and FEM pixel groups depend on the grid, the aperture and
+    the detector/FEM settings only, so runs on a new specimen of the same
+    grid reuse them: realizations run one after another on one context give
+    the same results bit for bit as each run alone on a fresh context, also
+    when the aperture or the detector changes in between."""
+    runs = []
+    for seed in (31, 32, 33):
+        q, Z = random_atoms(60, (12.0, 12.0, 6.0), seed=seed, species=(14, 29))
+        runs.append(Case(nx=128, ny=128, cell=(12.0, 12.0, 6.0), voltage=300e3, q=q, Z=Z,
+                         min_slice=1.5, alpha_max=0.02, inner=0.03, outer=0.2, fem=True, dr=0.7))
+    runs.append(Case(**{**runs[1].__dict__, "alpha_max": 0.015}))
+    runs.append(Case(**{**runs[0].__dict__, "inner": 0.05}))
+    xp = np.linspace(0.0, 11.0, 6)
+    yp = np.linspace(0.5, 10.5, 5)
+
+    def run(context, c):
+        spec = c.gpu_specimen(msf, context)
+        out = context.adfstem_fem(spec, c.optics(msf), xp, yp, c.femcfg(msf),
+                                  detector=c.detector(msf))
+        spec.free()
+        return out
+
+    chained = [run(ctx, c) for c in runs]
+    for c, (img, v) in zip(runs, chained):
+        fresh = msf.Context(0)
+        try:
+            img0, v0 = run(fresh, c)
+        finally:
+            fresh.close()
+        np.testing.assert_array_equal(img, img0)
+        for m in msf.FEM_MODES:
+            np.testing.assert_array_equal(v[m], v0[m])
