@@ -90,9 +90,18 @@ template <int NY, int V> __host__ __device__ constexpr int row_threads()
 // of staging them in LDS, so that more workgroups fit on a CU.
 template <int NY, int V> __host__ __device__ constexpr bool row_lean() { return NY == 1024; }
 
+// Occupancy the row pass asks of the register allocator: 6 waves per SIMD for
+// the 8-value plans; the plans with 12 to 24 values per lane need more
+// registers, and ask for what they reach (their allocation is unchanged).
+template <int NY, int V> __host__ __device__ constexpr int row_waves()
+{
+    constexpr int E = PlanV<NY, V>::E;
+    return E <= 8 ? 6 : (E == 16 ? (NY == 512 ? 3 : 4) : (E == 12 ? 3 : 2));
+}
+
 template <int NY, int V, bool FIRST, int TMODE, bool PRUNE>
-__global__ void __launch_bounds__((row_threads<NY, V>())) __attribute__((amdgpu_waves_per_eu(6)))
-row_pass_kernel(RowPassArgs a)
+__global__ void __launch_bounds__((row_threads<NY, V>()))
+__attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_pass_kernel(RowPassArgs a)
 {
     using P = PlanV<NY, V>;
     constexpr int E = P::E, T = P::T;
