@@ -14,7 +14,9 @@ roofline: the dominant slice-pass kernel, compulsory HBM bytes per launch
           (read + write of the band-limited wave) over its average duration,
           measured with HIP events on the library's stream in a profiled step.
 cpu_baseline: the FP64 CPU restatement (oracle, OpenMP over positions) on a
-          bounded sample of positions of the same job.
+          bounded sample of positions of the same job; cpu_baseline.parity
+          compares the GPU path with it on a sample of the job (relative L2 of
+          V_omega and of exit-wave intensities).
 """
 import argparse
 import json
@@ -35,8 +37,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(job, q, Z, xp, yp, threads, target_s):
-    """Oracle (FP64, own FFT) on a bounded sample of the raster."""
+def rel_l2(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    m = np.isfinite(b)
+    if not np.array_equal(np.isfinite(a), m):
+        return float("inf")
+    return float(np.linalg.norm((a - b)[m]) / max(np.linalg.norm(b[m]), 1e-300))
+
+
+def cpu_baseline(job, q, Z, xp, yp, threads, target_s, gpu=None):
+    """Oracle (FP64, own FFT) on a bounded sample of the raster.  With
+    gpu = (ctx, spec, optics, fem) it also checks the GPU path against the
+    oracle on that sample: V_omega of the positions (xp[0], yp[:]) and the
+    exit-wave intensities of four of them, relative L2."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     oj = O.make_job(nx=job["nx"], ny=job["ny"], xperiod=job["xperiod"], yperiod=job["yperiod"],
@@ -46,6 +59,20 @@ def cpu_baseline(job, q, Z, xp, yp, threads, target_s):
     t0 = time.perf_counter()
     spec = O.Specimen(oj, q, Z.astype(np.int32))
     setup_s = time.perf_counter() - t0
+    parity = None
+    if gpu is not None:
+        ctx, gspec, optics, fem = gpu
+        _, v_o, _ = O.adfstem(oj, spec, xp[:1], yp, n_threads=threads)
+        _, v_g = ctx.adfstem(gspec, optics, xp[:1], yp, fem=fem)
+        _, _, ex_o = O.adfstem(oj, spec, xp[:1], yp[:4], n_threads=threads, exit_waves=True)
+        ex_g = ctx.exit_waves(gspec, optics, xp[:1], yp[:4])
+        parity = {"sample": f"positions (x_p[0], y_p[0:{yp.size}]) of the timed job; exit waves of the first 4",
+                  "v_omega_rel_l2": rel_l2(v_g, v_o),
+                  "exit_intensity_rel_l2_max": max(rel_l2(np.abs(ex_g[k]) ** 2, np.abs(ex_o[k]) ** 2)
+                                                   for k in range(4)),
+                  "tolerance": {"v_omega": 1e-4, "exit_intensity": 1e-5}}
+        log(f"parity vs oracle: V {parity['v_omega_rel_l2']:.2e}, |psi|^2 "
+            f"{parity['exit_intensity_rel_l2_max']:.2e}")
     # one position to size the sample
     t0 = time.perf_counter()
     O.adfstem(oj, spec, xp[:1], yp[:1], n_threads=1)
@@ -58,7 +85,7 @@ def cpu_baseline(job, q, Z, xp, yp, threads, target_s):
         O.adfstem(oj, spec, xp[:1], ys[k:k + yp.size], n_threads=threads)
     dt = time.perf_counter() - t0
     return {"value": n_pos * spec.n_slices / dt, "unit": "probe-slice propagations/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "parity": parity,
             "sample": f"{n_pos} positions x {spec.n_slices} slices of the same job "
                       f"(FP64 restatement, OpenMP over positions, {dt:.1f} s; "
                       f"specimen set-up {setup_s:.1f} s not included)"}
@@ -495,7 +522,8 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(job, q, Z, xp, yp, threads, args.cpu_seconds)
+        cpu = cpu_baseline(job, q, Z, xp, yp, threads, args.cpu_seconds,
+                           gpu=(ctx, spec, optics, fem))
 
     lat = np.array(times) * 1e3
     out = {
