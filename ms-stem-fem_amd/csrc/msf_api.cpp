@@ -113,6 +113,20 @@ struct RunPlan {
         graph = nullptr;
         graph_sig.clear();
     }
+    // Frees the device memory of the last run except the pixel-group tables
+    // (a few MB, keyed by grid and settings): the next run rebuilds the rest,
+    // and reuses the groups when its grid and settings match.
+    void release_work()
+    {
+        drop_graph();
+        valid = false;
+        spec_id = 0;
+        for (DevBuf* b : {&U0, &Psi0, &psi, &part, &segp, &state, &femv, &adf, &shift, &exit, &psi2,
+                          &part2, &segp2, &mtf_full2, &vsum, &pm1, &pm2, &csum, &csqr, &ceach,
+                          &mtf_full})
+            b->release();
+        cap_b = cap_p = 0;
+    }
 };
 
 // A job of the serving queue: inputs copied at submission, outputs written
@@ -1946,12 +1960,9 @@ uint64_t job_footprint(const ServeJob& j)
     return b + b / 10 + (16ull << 20);
 }
 
-// a context's cached run plan and its buffers, released between served jobs
-void reset_plan(msf_ctx* c)
-{
-    c->plan.~RunPlan();
-    new (&c->plan) RunPlan();
-}
+// a served job's work buffers, released between jobs; the pixel groups stay
+// for the next job on the same grid and settings
+void reset_plan(msf_ctx* c) { c->plan.release_work(); }
 
 void serve_worker(msf_ctx* c, uint32_t slot)
 {
