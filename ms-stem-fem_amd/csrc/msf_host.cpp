@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <system_error>
 #include <thread>
 
 namespace msf {
@@ -249,16 +250,25 @@ template <class It, class Cmp> static void stable_sort_threads(It first, It last
     std::vector<size_t> b(parts + 1);
     for (unsigned k = 0; k <= parts; ++k) b[k] = n * k / parts;
     std::vector<std::thread> th;
-    for (unsigned k = 0; k < parts; ++k)
-        th.emplace_back([&, k] { std::stable_sort(first + b[k], first + b[k + 1], cmp); });
-    for (auto& t : th) t.join();
-    for (unsigned w = 1; w < parts; w *= 2) {
-        th.clear();
-        for (unsigned k = 0; k + w < parts; k += 2 * w)
-            th.emplace_back([&, k, w] {
-                std::inplace_merge(first + b[k], first + b[k + w], first + b[std::min(k + 2 * w, parts)], cmp);
-            });
+    th.reserve(parts);
+    try {
+        for (unsigned k = 0; k < parts; ++k)
+            th.emplace_back([&, k] { std::stable_sort(first + b[k], first + b[k + 1], cmp); });
         for (auto& t : th) t.join();
+        for (unsigned w = 1; w < parts; w *= 2) {
+            th.clear();
+            for (unsigned k = 0; k + w < parts; k += 2 * w)
+                th.emplace_back([&, k, w] {
+                    std::inplace_merge(first + b[k], first + b[k + w],
+                                       first + b[std::min(k + 2 * w, parts)], cmp);
+                });
+            for (auto& t : th) t.join();
+        }
+    } catch (const std::system_error&) {
+        // no thread could be started: finish what started, then sort here
+        for (auto& t : th)
+            if (t.joinable()) t.join();
+        std::stable_sort(first, last, cmp);
     }
 }
 
