@@ -376,39 +376,46 @@ Groups build_groups(const Domain& d, const RowRanges& keep, int ncp, bool adf, d
 
 SegmentTiles build_segments(const Groups& g, int seg)
 {
+    // A group of n pixels gets ns = ceil(n / seg) consecutive segments.  Its
+    // pixels, in memory order, are dealt round-robin: pixel i goes to segment
+    // i % ns at position i / ns.  Lanes of one tile handle consecutive
+    // segments, so at every step they read neighbouring pixels of the group
+    // (coalesced).  Each segment is summed in a fixed order, and the segments
+    // of a group are added in segment order, independent of the batch.
     SegmentTiles t;
-    std::vector<std::pair<int32_t, int32_t>> segs; // [begin, end) into g.pixels
+    std::vector<std::vector<int32_t>> segs;
     const int G = (int)g.kind.size();
+    std::vector<int32_t> sorted;
     for (int k = 0; k < G; ++k) {
         t.seg_of_group.push_back((int32_t)segs.size());
-        for (int32_t b = g.offsets[k]; b < g.offsets[k + 1]; b += seg) {
-            const int32_t e = std::min(b + seg, g.offsets[k + 1]);
-            // gather order within a segment: memory order
-            segs.emplace_back(b, e);
-            t.seg_kind.push_back(g.kind[k]);
-        }
+        const int32_t n = g.offsets[k + 1] - g.offsets[k];
+        if (n == 0) continue;
+        const int32_t ns = (n + seg - 1) / seg;
+        sorted.assign(g.pixels.begin() + g.offsets[k], g.pixels.begin() + g.offsets[k + 1]);
+        std::sort(sorted.begin(), sorted.end());
+        const size_t s0 = segs.size();
+        segs.resize(s0 + (size_t)ns);
+        for (int32_t i = 0; i < n; ++i) segs[s0 + (size_t)(i % ns)].push_back(sorted[(size_t)i]);
+        for (int32_t j = 0; j < ns; ++j) t.seg_kind.push_back(g.kind[k]);
     }
     t.seg_of_group.push_back((int32_t)segs.size());
     t.n_segments = (int)segs.size();
     t.n_tiles = (t.n_segments + 63) / 64;
     t.tile_off.push_back(0);
-    std::vector<int32_t> sorted;
     for (int w = 0; w < t.n_tiles; ++w) {
-        int len = 0;
+        size_t len = 0;
         for (int l = 0; l < 64; ++l) {
             const int s = w * 64 + l;
-            if (s < t.n_segments) len = std::max(len, segs[s].second - segs[s].first);
+            if (s < t.n_segments) len = std::max(len, segs[(size_t)s].size());
         }
         const size_t base = t.pix.size();
-        t.pix.resize(base + (size_t)len * 64, -1);
+        t.pix.resize(base + len * 64, -1);
         for (int l = 0; l < 64; ++l) {
             const int s = w * 64 + l;
             if (s >= t.n_segments) continue;
-            sorted.assign(g.pixels.begin() + segs[s].first, g.pixels.begin() + segs[s].second);
-            std::sort(sorted.begin(), sorted.end());
-            for (size_t k = 0; k < sorted.size(); ++k) t.pix[base + k * 64 + l] = sorted[k];
+            for (size_t k = 0; k < segs[(size_t)s].size(); ++k) t.pix[base + k * 64 + l] = segs[(size_t)s][k];
         }
-        t.tile_off.push_back(t.tile_off.back() + len);
+        t.tile_off.push_back(t.tile_off.back() + (int32_t)len);
     }
     t.seg_kind.resize((size_t)t.n_tiles * 64, 0);
     return t;
