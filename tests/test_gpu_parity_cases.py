@@ -28,9 +28,11 @@ def _exit_parity(msf, ctx, O, c, xp, yp, batch=0, tol=1e-5):
     assert np.count_nonzero(ex_g[0]) <= np.count_nonzero(np.abs(ex_o[0]) > 0)
 
 
-@pytest.mark.parametrize("n", [96, 192, 256, 384])
+@pytest.mark.parametrize("n", [96, 192, 256, 384, 768])
 def test_exit_waves_other_lengths(msf, ctx, O, n):
-    """Radix-6/12/24 plans (96, 192, 384) and the 256-point plan."""
+    """The plans with radix-6 first stages (96, 192, 384, 768; 12 or 24
+    values per lane, column passes bounded by their tile size) and the
+    256-point plan."""
     cell = (n / 8.0, n / 8.0, 6.0)
     q, Z = random_atoms(50, cell, seed=n, species=(14, 40))
     c = Case(nx=n, ny=n, cell=cell, voltage=300e3, q=q, Z=Z, min_slice=2.0, alpha_max=0.015)
@@ -38,7 +40,8 @@ def test_exit_waves_other_lengths(msf, ctx, O, n):
 
 
 def test_exit_waves_1024(msf, ctx, O):
-    """1024^2: the 16:8,8,4,4 plan and the 8-column tile of the column pass."""
+    """1024^2 with the automatic plan: 16 x 16 x 4 on 64 lanes, lean row
+    passes and 4-column lean tiles of the column pass."""
     cell = (40.0, 40.0, 3.0)
     q, Z = random_atoms(30, cell, seed=11, species=(29,))
     c = Case(nx=1024, ny=1024, cell=cell, voltage=200e3, q=q, Z=Z, min_slice=2.0,
