@@ -28,13 +28,13 @@ def _exit_parity(msf, ctx, O, c, xp, yp, batch=0, tol=1e-5):
     assert np.count_nonzero(ex_g[0]) <= np.count_nonzero(np.abs(ex_o[0]) > 0)
 
 
-@pytest.mark.parametrize("n", [96, 192, 256, 384, 768])
+@pytest.mark.parametrize("n", [16, 32, 96, 192, 256, 384, 768])
 def test_exit_waves_other_lengths(msf, ctx, O, n):
     """The plans with radix-6 first stages (96, 192, 384, 768; 12 or 24
-    values per lane, column passes bounded by their tile size) and the
-    256-point plan."""
-    cell = (n / 8.0, n / 8.0, 6.0)
-    q, Z = random_atoms(50, cell, seed=n, species=(14, 40))
+    values per lane, column passes bounded by their tile size), the 256-point
+    plan and the smallest lengths (16 and 32: 2 and 4 lanes per transform)."""
+    cell = (max(n / 8.0, 6.0), max(n / 8.0, 6.0), 6.0)
+    q, Z = random_atoms(50 if n > 32 else 8, cell, seed=n, species=(14, 40))
     c = Case(nx=n, ny=n, cell=cell, voltage=300e3, q=q, Z=Z, min_slice=2.0, alpha_max=0.015)
     _exit_parity(msf, ctx, O, c, np.array([0.3, cell[0] / 2 + 0.11]), np.array([1.7]), batch=2)
 
