@@ -619,14 +619,24 @@ __global__ void __launch_bounds__(256) reduce_segments_kernel(const float2* psi,
     const int k0 = tile_off[w], k1 = tile_off[w + 1];
     const bool amp = seg_kind[w * 64 + lane] != 0;
     double acc = 0.0;
-#pragma unroll 4
-    for (int k = k0; k < k1; ++k) {
-        const int o = pix[(int64_t)k * 64 + lane];
-        if (o >= 0) {
-            const float2 v = wave[o];
-            const double re = v.x, im = v.y;
-            const double i2 = re * re + im * im;
-            acc += amp ? sqrt(i2) : i2;
+    // chunks of 8 pixels: the index loads, then the wave loads (empty slots
+    // read element 0), are issued back to back so their latencies overlap;
+    // the sum runs over the pixels in the same order as one at a time
+    constexpr int C = 8;
+    for (int k = k0; k < k1; k += C) {
+        int o[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) o[j] = (k + j < k1) ? pix[(int64_t)(k + j) * 64 + lane] : -1;
+        float2 v[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) v[j] = wave[o[j] >= 0 ? o[j] : 0];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            if (o[j] >= 0) {
+                const double re = v[j].x, im = v[j].y;
+                const double i2 = re * re + im * im;
+                acc += amp ? sqrt(i2) : i2;
+            }
         }
     }
     seg_out[(int64_t)b * n_tiles * 64 + w * 64 + lane] = acc;
