@@ -63,7 +63,7 @@ def test_fem_bins_match_oracle(msf, O, n, L, dr):
 
 
 def test_supported_lengths(msf):
-    for n in (64, 128, 256, 384, 512, 1024):
+    for n in (16, 32, 64, 96, 128, 192, 256, 384, 512, 768, 1024):
         assert msf.supported_length(n)
     for n in (100, 2048, 0):
         assert not msf.supported_length(n)
@@ -127,3 +127,23 @@ def test_correlograph_bins_match_oracle(msf, O, n, L, dr):
     assert nphi == max(want, 0) == O.lib().orc_correlograph_nphi(dk, nb)
     if nphi:
         np.testing.assert_array_equal(phi, np.arange(nphi + 1) * (2 * 3.14159265369 / nphi))
+
+
+def test_headers_compile_as_c99(tmp_path):
+    """include/msf.h and include/msf_io.h are plain C: a C99 translation unit
+    that includes both and touches the configuration and statistics structs
+    compiles without warnings under -pedantic."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "h.c"
+    src.write_text('#include "msf.h"\n#include "msf_io.h"\n'
+                   "int main(void) { msf_config c = {0}; msf_stats s = {0}; msf_fem_outputs o = {0};\n"
+                   "  c.fft_plan = 2; c.col_tile = 4;\n"
+                   "  return (int)(c.fft_plan + s.row_fft_plan + s.col_fft_plan) + (o.variance_2d != 0) - 2; }\n")
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), "-c", str(src), "-o",
+                        str(tmp_path / "h.o")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
