@@ -20,7 +20,9 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <set>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -221,6 +223,22 @@ int fail(msf_ctx* c, int code, const std::string& m)
     if (c) c->err = m;
     return code;
 }
+
+// Every entry point turns a C++ exception (a failed host allocation, a length
+// error from a malformed size) into an error code: none may cross the C ABI.
+#define MSF_CATCH_ALL(c)                                                                      \
+    catch (const std::bad_alloc&)                                                             \
+    {                                                                                         \
+        return fail((c), MSF_ERR_MEMORY, "host allocation failed");                          \
+    }                                                                                         \
+    catch (const std::exception& e_)                                                          \
+    {                                                                                         \
+        return fail((c), MSF_ERR_INVALID, std::string("unexpected error: ") + e_.what());    \
+    }                                                                                         \
+    catch (...)                                                                               \
+    {                                                                                         \
+        return fail((c), MSF_ERR_INVALID, "unexpected error");                               \
+    }
 
 // device work of a context is serialised (direct calls and the serving worker)
 #define MSF_LOCK(c) std::lock_guard<std::recursive_mutex> msf_lock_((c)->mu)
@@ -583,6 +601,9 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     if (!same_optics || !same_groups) pl.valid = false;
 
     if (!same_groups) {
+        // the tables below are replaced one by one: until all of them and
+        // their keys are written, no later call may take them as valid
+        pl.groups_ok = false;
         double k1 = 0.0, k2 = 0.0;
         if (want_adf) {
             k1 = det->inner / d.lambda;
@@ -609,9 +630,10 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         HIPCHK(c, upload(pl.zero, g.zeroed, c->stream));
         pl.cap_b = 0; // partial-sum buffers depend on G
         pl.hg = g;
-        pl.groups_ok = false; // until the tables below are complete
         pl.ggrid = gkey;
         pl.galpha = o->alpha_max;
+        pl.det = dkey;
+        pl.fem = fkey;
         pl.mtf_on = false;
         pl.mtf_v.clear();
         pl.mtf_d.clear();
@@ -663,8 +685,6 @@ int run_internal(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
     }
     pl.spec_id = sp->id;
     pl.optics = *o;
-    pl.det = dkey;
-    pl.fem = fkey;
     pl.ncp = ncp;
     pl.valid = true;
 
@@ -1223,95 +1243,110 @@ extern "C" {
 
 int msf_create(msf_ctx** out, int device)
 {
-    if (!out) return MSF_ERR_INVALID;
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MSF_ERR_DEVICE;
-    if (hipSetDevice(device) != hipSuccess) return MSF_ERR_DEVICE;
-    msf_ctx* c = new msf_ctx;
-    c->device = device;
-    if (hipStreamCreate(&c->stream) != hipSuccess) {
-        delete c;
-        return MSF_ERR_DEVICE;
+    try {
+        if (!out) return MSF_ERR_INVALID;
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MSF_ERR_DEVICE;
+        if (hipSetDevice(device) != hipSuccess) return MSF_ERR_DEVICE;
+        msf_ctx* c = new msf_ctx;
+        c->device = device;
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return MSF_ERR_DEVICE;
+        }
+        *out = c;
+        return MSF_OK;
     }
-    *out = c;
-    return MSF_OK;
+    MSF_CATCH_ALL(nullptr)
 }
 
 int msf_destroy(msf_ctx* c)
 {
-    if (!c) return MSF_ERR_INVALID;
-    std::unique_lock<std::mutex> serve_lk(c->serve_mu);
-    if (c->serve) { // finish the queued jobs, then stop the workers
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        std::unique_lock<std::mutex> serve_lk(c->serve_mu);
+        if (c->serve) { // finish the queued jobs, then stop the workers
+            {
+                std::lock_guard<std::mutex> lk(c->serve->m);
+                c->serve->stop = true;
+            }
+            c->serve->work_cv.notify_all();
+            for (std::thread& w : c->serve->workers)
+                if (w.joinable()) w.join();
+            for (msf_ctx* sc : c->serve->slot_ctx) msf_destroy(sc);
+            c->serve.reset();
+        }
+        serve_lk.unlock();
         {
-            std::lock_guard<std::mutex> lk(c->serve->m);
-            c->serve->stop = true;
+            std::lock_guard<std::recursive_mutex> lk(c->mu);
+            (void)hipSetDevice(c->device);
+            (void)hipStreamSynchronize(c->stream);
+            // an instantiated graph refers to the stream it was launched on:
+            // destroy it (and every buffer its kernels use) before the stream
+            c->plan.drop_graph();
+            // specimens may outlive their context: detach them so that a later
+            // msf_specimen_free only releases their own device memory
+            for (msf_specimen* sp : c->specimens) sp->ctx = nullptr;
+            c->specimens.clear();
+            c->tw32.clear();
+            c->tw64.clear();
+            c->twnat.clear();
+            c->tw32alt.clear();
+            for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_fem[0], c->ev_fem[1]})
+                if (e) (void)hipEventDestroy(e);
+            if (c->stream2) {
+                (void)hipStreamSynchronize(c->stream2);
+                (void)hipStreamDestroy(c->stream2);
+            }
+            (void)hipStreamDestroy(c->stream);
         }
-        c->serve->work_cv.notify_all();
-        for (std::thread& w : c->serve->workers)
-            if (w.joinable()) w.join();
-        for (msf_ctx* sc : c->serve->slot_ctx) msf_destroy(sc);
-        c->serve.reset();
+        delete c;
+        return MSF_OK;
     }
-    serve_lk.unlock();
-    {
-        std::lock_guard<std::recursive_mutex> lk(c->mu);
-        (void)hipSetDevice(c->device);
-        (void)hipStreamSynchronize(c->stream);
-        // an instantiated graph refers to the stream it was launched on:
-        // destroy it (and every buffer its kernels use) before the stream
-        c->plan.drop_graph();
-        // specimens may outlive their context: detach them so that a later
-        // msf_specimen_free only releases their own device memory
-        for (msf_specimen* sp : c->specimens) sp->ctx = nullptr;
-        c->specimens.clear();
-        c->tw32.clear();
-        c->tw64.clear();
-        c->twnat.clear();
-        c->tw32alt.clear();
-        for (hipEvent_t e : {c->ev_fork, c->ev_join, c->ev_fem[0], c->ev_fem[1]})
-            if (e) (void)hipEventDestroy(e);
-        if (c->stream2) {
-            (void)hipStreamSynchronize(c->stream2);
-            (void)hipStreamDestroy(c->stream2);
-        }
-        (void)hipStreamDestroy(c->stream);
-    }
-    delete c;
-    return MSF_OK;
+    MSF_CATCH_ALL(c)
 }
 
 const char* msf_last_error(const msf_ctx* c) { return c ? c->err.c_str() : "no context"; }
 
 int msf_set_config(msf_ctx* c, const msf_config* cfg)
 {
-    if (!c || !cfg) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (cfg->transmit_mode != MSF_TRANSMIT_REFERENCE && cfg->transmit_mode != MSF_TRANSMIT_EXACT)
-        return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
-    if (cfg->col_tile != 0 && cfg->col_tile != 4 && cfg->col_tile != 8 && cfg->col_tile != 16)
-        return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 4, 8 or 16");
-    if (cfg->chains < 0 || cfg->chains > 2) return fail(c, MSF_ERR_INVALID, "chains must be 0, 1 or 2");
-    if (cfg->fft_plan < 0 || cfg->fft_plan > 2) return fail(c, MSF_ERR_INVALID, "fft_plan must be 0, 1 or 2");
-    c->cfg = *cfg;
-    return MSF_OK;
+    try {
+        if (!c || !cfg) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (cfg->transmit_mode != MSF_TRANSMIT_REFERENCE && cfg->transmit_mode != MSF_TRANSMIT_EXACT)
+            return fail(c, MSF_ERR_INVALID, "unknown transmit_mode");
+        if (cfg->col_tile != 0 && cfg->col_tile != 4 && cfg->col_tile != 8 && cfg->col_tile != 16)
+            return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 4, 8 or 16");
+        if (cfg->chains < 0 || cfg->chains > 2) return fail(c, MSF_ERR_INVALID, "chains must be 0, 1 or 2");
+        if (cfg->fft_plan < 0 || cfg->fft_plan > 2) return fail(c, MSF_ERR_INVALID, "fft_plan must be 0, 1 or 2");
+        c->cfg = *cfg;
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_get_stats(const msf_ctx* c, msf_stats* out)
 {
-    if (!c || !out) return MSF_ERR_INVALID;
-    MSF_LOCK(const_cast<msf_ctx*>(c));
-    *out = c->stats;
-    return MSF_OK;
+    try {
+        if (!c || !out) return MSF_ERR_INVALID;
+        MSF_LOCK(const_cast<msf_ctx*>(c));
+        *out = c->stats;
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(nullptr)
 }
 
 int msf_supported_length(uint32_t n) { return msf::slice_lengths_ok((int)n) ? 1 : 0; }
 
 int msf_wavelength(double voltage, double* lambda, double* gamma)
 {
-    if (!(voltage > 0.0) || !lambda || !gamma) return MSF_ERR_INVALID;
-    msf::wavelength(voltage, *lambda, *gamma);
-    return MSF_OK;
+    try {
+        if (!(voltage > 0.0) || !lambda || !gamma) return MSF_ERR_INVALID;
+        msf::wavelength(voltage, *lambda, *gamma);
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(nullptr)
 }
 
 double msf_scherzer_defocus(int32_t corrected, double lambda, double cs)
@@ -1334,323 +1369,350 @@ double msf_scherzer_cs3(double lambda, double cs5)
 int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const uint32_t* Z,
                        uint32_t n, double min_slice, msf_specimen** out)
 {
-    if (!c || !out) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    *out = nullptr;
-    if (int r = check_grid(c, grid)) return r;
-    if (n > 0 && (!q || !Z)) return fail(c, MSF_ERR_INVALID, "atom arrays are NULL");
-    if (!(min_slice >= 0.0)) return fail(c, MSF_ERR_INVALID, "minslice must be >= 0");
-    HIPCHK(c, hipSetDevice(c->device));
-    const int nx = (int)grid->nx, ny = (int)grid->ny;
-    msf_specimen* sp = new msf_specimen;
-    sp->ctx = c;
-    sp->grid = *grid;
-    sp->dom = msf::make_domain(nx, ny, grid->xperiod, grid->yperiod, grid->xmin, grid->ymin,
-                               grid->voltage);
-    const msf::Domain& d = sp->dom;
-    sp->id = g_specimen_ids.fetch_add(1);
-    sp->keep = msf::slice_keep_ranges(d);
-    if (!sp->keep.ok) {
-        delete sp;
-        return fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order");
-    }
-    if (hipError_t e = upload(sp->dkeep, to_int2(sp->keep), c->stream)) {
-        delete sp;
-        return fail(c, MSF_ERR_DEVICE, hipGetErrorString(e));
-    }
-
-    // species present, ascending Z (tem.cpp:944-972)
-    std::vector<int> species_of(104, -1), zlist;
-    for (uint32_t a = 0; a < n; ++a) {
-        if (Z[a] < 1 || Z[a] > 103) {
+    try {
+        if (!c || !out) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        *out = nullptr;
+        if (int r = check_grid(c, grid)) return r;
+        if (n > 0 && (!q || !Z)) return fail(c, MSF_ERR_INVALID, "atom arrays are NULL");
+        if (!(min_slice >= 0.0)) return fail(c, MSF_ERR_INVALID, "minslice must be >= 0");
+        HIPCHK(c, hipSetDevice(c->device));
+        const int nx = (int)grid->nx, ny = (int)grid->ny;
+        msf_specimen* sp = new msf_specimen;
+        sp->ctx = c;
+        sp->grid = *grid;
+        sp->dom = msf::make_domain(nx, ny, grid->xperiod, grid->yperiod, grid->xmin, grid->ymin,
+                                   grid->voltage);
+        const msf::Domain& d = sp->dom;
+        sp->id = g_specimen_ids.fetch_add(1);
+        sp->keep = msf::slice_keep_ranges(d);
+        if (!sp->keep.ok) {
             delete sp;
-            return fail(c, MSF_ERR_INVALID, "atomic number outside 1..103 (tem_io.cpp:4768-4781)");
+            return fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order");
         }
-        species_of[Z[a]] = 0;
-    }
-    for (int z = 1; z <= 103; ++z)
-        if (species_of[z] == 0) {
-            species_of[z] = (int)zlist.size();
-            zlist.push_back(z);
+        if (hipError_t e = upload(sp->dkeep, to_int2(sp->keep), c->stream)) {
+            delete sp;
+            return fail(c, MSF_ERR_DEVICE, hipGetErrorString(e));
         }
-    const int nsp = (int)zlist.size();
 
-    // auto-slicing and per-atom nearest-pixel shifts
-    msf::Slices sl = msf::autoslice(q, n, grid->zmin, grid->zperiod, min_slice);
-    const int S = (int)sl.lower.size();
-    sp->n_slices = S;
-    sp->lower = sl.lower;
-    sp->thick = sl.thick;
-    sp->counts.resize(S);
-    std::vector<int4> atoms; // (slice, species, sx, sy)
-    atoms.reserve(n);
-    for (int s = 0; s < S; ++s) {
-        sp->counts[s] = (uint32_t)sl.atoms[s].size();
-        for (int32_t a : sl.atoms[s]) {
-            const int64_t sx = msf::shift_index(d.xx, q[3 * a], 0);
-            const int64_t sy = msf::shift_index(d.yy, q[3 * a + 1], 0);
-            if (sx < 0 || sy < 0) {
+        // species present, ascending Z (tem.cpp:944-972)
+        std::vector<int> species_of(104, -1), zlist;
+        for (uint32_t a = 0; a < n; ++a) {
+            if (Z[a] < 1 || Z[a] > 103) {
                 delete sp;
-                return fail(c, MSF_ERR_DOMAIN, "atom position below the domain minimum");
+                return fail(c, MSF_ERR_INVALID, "atomic number outside 1..103 (tem_io.cpp:4768-4781)");
             }
-            atoms.push_back(int4{s, species_of[Z[a]], (int)sx, (int)sy});
+            species_of[Z[a]] = 0;
         }
-    }
+        for (int z = 1; z <= 103; ++z)
+            if (species_of[z] == 0) {
+                species_of[z] = (int)zlist.size();
+                zlist.push_back(z);
+            }
+        const int nsp = (int)zlist.size();
 
-    // propagators (slice.cpp:1293-1363), FP64 -> FP32; P_y carries 1/sqrtNxNy
-    std::vector<float2> pxs((size_t)S * nx), pys((size_t)S * ny);
-    std::vector<double2> pxd((size_t)S * nx), pyd((size_t)S * ny);
-    for (int s = 0; s < S; ++s) {
-        std::vector<double> p;
-        msf::propagator(d.lambda, sl.thick[s], d.kx, d.bw_t, p);
-        for (int i = 0; i < nx; ++i) {
-            pxd[(size_t)s * nx + i] = double2{p[2 * i], p[2 * i + 1]};
-            pxs[(size_t)s * nx + i] = float2{(float)p[2 * i], (float)p[2 * i + 1]};
+        // auto-slicing and per-atom nearest-pixel shifts
+        msf::Slices sl = msf::autoslice(q, n, grid->zmin, grid->zperiod, min_slice);
+        const int S = (int)sl.lower.size();
+        sp->n_slices = S;
+        sp->lower = sl.lower;
+        sp->thick = sl.thick;
+        sp->counts.resize(S);
+        std::vector<int4> atoms; // (slice, species, sx, sy)
+        atoms.reserve(n);
+        for (int s = 0; s < S; ++s) {
+            sp->counts[s] = (uint32_t)sl.atoms[s].size();
+            for (int32_t a : sl.atoms[s]) {
+                const int64_t sx = msf::shift_index(d.xx, q[3 * a], 0);
+                const int64_t sy = msf::shift_index(d.yy, q[3 * a + 1], 0);
+                if (sx < 0 || sy < 0) {
+                    delete sp;
+                    return fail(c, MSF_ERR_DOMAIN, "atom position below the domain minimum");
+                }
+                atoms.push_back(int4{s, species_of[Z[a]], (int)sx, (int)sy});
+            }
         }
-        msf::propagator(d.lambda, sl.thick[s], d.ky, d.bw_t, p);
-        for (int j = 0; j < ny; ++j) {
-            pyd[(size_t)s * ny + j] = double2{p[2 * j], p[2 * j + 1]};
-            pys[(size_t)s * ny + j] = float2{(float)(p[2 * j] / d.sqrt_nxny), (float)(p[2 * j + 1] / d.sqrt_nxny)};
-        }
-    }
-    int rc = MSF_OK;
-    auto bail = [&](int code) {
-        delete sp;
-        return code;
-    };
-    if (hipError_t e = upload(sp->px, pxs, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
-    if (hipError_t e = upload(sp->py, pys, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
-    if (hipError_t e = upload(sp->px64, pxd, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
-    if (hipError_t e = upload(sp->py64, pyd, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
 
-    // transmission functions, FP64 on the device, in chunks of slices
-    const msf::RowRanges sup = msf::template_keep_ranges(d);
-    const msf::RowRanges bw = msf::bwlimit_keep_ranges(d);
-    if (!sup.ok || !bw.ok) return bail(fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order"));
-    std::vector<double> params;
-    for (int z : zlist) params.insert(params.end(), kKirkland[z - 1], kKirkland[z - 1] + 12);
-    const int64_t M = (int64_t)nx * ny;
-    const double2 *twx, *twy;
-    if ((rc = twiddles(c, nx, nullptr, &twx))) return bail(rc);
-    if ((rc = twiddles(c, ny, nullptr, &twy))) return bail(rc);
-    DevBuf dkx, dky, dsup, dbw, dpar, dmaps, datoms;
-    hipError_t e = hipSuccess;
-    if (!e) e = upload(dkx, d.kx, c->stream);
-    if (!e) e = upload(dky, d.ky, c->stream);
-    if (!e) e = upload(dsup, to_int2(sup), c->stream);
-    if (!e) e = upload(dbw, to_int2(bw), c->stream);
-    if (!e) e = upload(dpar, params, c->stream);
-    if (!e) e = sp->t64.alloc((size_t)S * M * sizeof(double2));
-    if (!e) e = sp->t32.alloc((size_t)S * M * sizeof(float2));
-    const int64_t per_slot = (int64_t)std::max(nsp, 1) * M * (int64_t)sizeof(double2);
-    const int K = (int)std::max<int64_t>(1, std::min<int64_t>(S, (256ll << 20) / per_slot));
-    if (!e) e = dmaps.alloc((size_t)K * per_slot);
-    if (!e) e = datoms.alloc(std::max<size_t>(atoms.size(), 1) * sizeof(int4));
-    if (!e && !atoms.empty())
-        e = hipMemcpyAsync(datoms.p, atoms.data(), atoms.size() * sizeof(int4), hipMemcpyHostToDevice,
-                           c->stream);
-    if (e) return bail(fail(c, e == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE, hipGetErrorString(e)));
-    const double lgab = d.lambda * d.gamma * (1 / (grid->xperiod * grid->yperiod));
-    size_t a0 = 0;
-    for (int s0 = 0; s0 < S && !e; s0 += K) {
-        const int k = std::min(K, S - s0);
-        size_t a1 = a0;
-        while (a1 < atoms.size() && atoms[a1].x < s0 + k) ++a1;
-        // slots are relative to the chunk
-        std::vector<int4> chunk(atoms.begin() + a0, atoms.begin() + a1);
-        for (auto& at : chunk) at.x -= s0;
-        if (!chunk.empty())
-            e = hipMemcpyAsync(datoms.p, chunk.data(), chunk.size() * sizeof(int4), hipMemcpyHostToDevice,
+        // propagators (slice.cpp:1293-1363), FP64 -> FP32; P_y carries 1/sqrtNxNy
+        std::vector<float2> pxs((size_t)S * nx), pys((size_t)S * ny);
+        std::vector<double2> pxd((size_t)S * nx), pyd((size_t)S * ny);
+        for (int s = 0; s < S; ++s) {
+            std::vector<double> p;
+            msf::propagator(d.lambda, sl.thick[s], d.kx, d.bw_t, p);
+            for (int i = 0; i < nx; ++i) {
+                pxd[(size_t)s * nx + i] = double2{p[2 * i], p[2 * i + 1]};
+                pxs[(size_t)s * nx + i] = float2{(float)p[2 * i], (float)p[2 * i + 1]};
+            }
+            msf::propagator(d.lambda, sl.thick[s], d.ky, d.bw_t, p);
+            for (int j = 0; j < ny; ++j) {
+                pyd[(size_t)s * ny + j] = double2{p[2 * j], p[2 * j + 1]};
+                pys[(size_t)s * ny + j] = float2{(float)(p[2 * j] / d.sqrt_nxny), (float)(p[2 * j + 1] / d.sqrt_nxny)};
+            }
+        }
+        int rc = MSF_OK;
+        auto bail = [&](int code) {
+            delete sp;
+            return code;
+        };
+        if (hipError_t e = upload(sp->px, pxs, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+        if (hipError_t e = upload(sp->py, pys, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+        if (hipError_t e = upload(sp->px64, pxd, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+        if (hipError_t e = upload(sp->py64, pyd, c->stream)) return bail(fail(c, MSF_ERR_DEVICE, hipGetErrorString(e)));
+
+        // transmission functions, FP64 on the device, in chunks of slices
+        const msf::RowRanges sup = msf::template_keep_ranges(d);
+        const msf::RowRanges bw = msf::bwlimit_keep_ranges(d);
+        if (!sup.ok || !bw.ok) return bail(fail(c, MSF_ERR_INVALID, "band limit is not a disk in FFT order"));
+        std::vector<double> params;
+        for (int z : zlist) params.insert(params.end(), kKirkland[z - 1], kKirkland[z - 1] + 12);
+        const int64_t M = (int64_t)nx * ny;
+        const double2 *twx, *twy;
+        if ((rc = twiddles(c, nx, nullptr, &twx))) return bail(rc);
+        if ((rc = twiddles(c, ny, nullptr, &twy))) return bail(rc);
+        DevBuf dkx, dky, dsup, dbw, dpar, dmaps, datoms;
+        hipError_t e = hipSuccess;
+        if (!e) e = upload(dkx, d.kx, c->stream);
+        if (!e) e = upload(dky, d.ky, c->stream);
+        if (!e) e = upload(dsup, to_int2(sup), c->stream);
+        if (!e) e = upload(dbw, to_int2(bw), c->stream);
+        if (!e) e = upload(dpar, params, c->stream);
+        if (!e) e = sp->t64.alloc((size_t)S * M * sizeof(double2));
+        if (!e) e = sp->t32.alloc((size_t)S * M * sizeof(float2));
+        const int64_t per_slot = (int64_t)std::max(nsp, 1) * M * (int64_t)sizeof(double2);
+        const int K = (int)std::max<int64_t>(1, std::min<int64_t>(S, (256ll << 20) / per_slot));
+        if (!e) e = dmaps.alloc((size_t)K * per_slot);
+        if (!e) e = datoms.alloc(std::max<size_t>(atoms.size(), 1) * sizeof(int4));
+        if (!e && !atoms.empty())
+            e = hipMemcpyAsync(datoms.p, atoms.data(), atoms.size() * sizeof(int4), hipMemcpyHostToDevice,
                                c->stream);
-        double2* maps = dmaps.as<double2>();
-        double2* sig = sp->t64.as<double2>() + (int64_t)s0 * M;
-        if (!e) e = msf::launch_fill_f64(maps, (int64_t)k * std::max(nsp, 1) * M, c->stream);
-        if (!e) e = msf::launch_scatter_atoms(maps, datoms.as<int4>(), (int)chunk.size(), nx, ny,
-                                              std::max(nsp, 1), c->stream);
-        if (!e && nsp > 0) e = msf::fft2d_f64(maps, nx, ny, k * nsp, false, twx, twy, c->stream);
-        if (!e) e = msf::launch_combine_species(maps, sig, k, nsp, dpar.as<double>(), dkx.as<double>(),
-                                                dky.as<double>(), nx, ny, dsup.as<int2>(), lgab,
-                                                grid->xmin, grid->ymin, c->stream);
-        if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
-        if (!e) e = msf::launch_exp_transmit(sig, (int64_t)k * M, 1.0 / std::sqrt((double)M), c->stream);
-        if (!e) e = msf::fft2d_f64(sig, nx, ny, k, false, twx, twy, c->stream);
-        if (!e) e = msf::launch_apply_keep(sig, nx, ny, k, dbw.as<int2>(), c->stream);
-        if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
-        if (!e) e = msf::launch_finish_transmit(sig, (int64_t)k * M, d.nxny, d.sqrt_nxny,
-                                                sp->t32.as<float2>() + (int64_t)s0 * M, c->stream);
-        // the atom list buffer is reused by the next chunk
+        if (e) return bail(fail(c, e == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE, hipGetErrorString(e)));
+        const double lgab = d.lambda * d.gamma * (1 / (grid->xperiod * grid->yperiod));
+        size_t a0 = 0;
+        for (int s0 = 0; s0 < S && !e; s0 += K) {
+            const int k = std::min(K, S - s0);
+            size_t a1 = a0;
+            while (a1 < atoms.size() && atoms[a1].x < s0 + k) ++a1;
+            // slots are relative to the chunk
+            std::vector<int4> chunk(atoms.begin() + a0, atoms.begin() + a1);
+            for (auto& at : chunk) at.x -= s0;
+            if (!chunk.empty())
+                e = hipMemcpyAsync(datoms.p, chunk.data(), chunk.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   c->stream);
+            double2* maps = dmaps.as<double2>();
+            double2* sig = sp->t64.as<double2>() + (int64_t)s0 * M;
+            if (!e) e = msf::launch_fill_f64(maps, (int64_t)k * std::max(nsp, 1) * M, c->stream);
+            if (!e) e = msf::launch_scatter_atoms(maps, datoms.as<int4>(), (int)chunk.size(), nx, ny,
+                                                  std::max(nsp, 1), c->stream);
+            if (!e && nsp > 0) e = msf::fft2d_f64(maps, nx, ny, k * nsp, false, twx, twy, c->stream);
+            if (!e) e = msf::launch_combine_species(maps, sig, k, nsp, dpar.as<double>(), dkx.as<double>(),
+                                                    dky.as<double>(), nx, ny, dsup.as<int2>(), lgab,
+                                                    grid->xmin, grid->ymin, c->stream);
+            if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
+            if (!e) e = msf::launch_exp_transmit(sig, (int64_t)k * M, 1.0 / std::sqrt((double)M), c->stream);
+            if (!e) e = msf::fft2d_f64(sig, nx, ny, k, false, twx, twy, c->stream);
+            if (!e) e = msf::launch_apply_keep(sig, nx, ny, k, dbw.as<int2>(), c->stream);
+            if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
+            if (!e) e = msf::launch_finish_transmit(sig, (int64_t)k * M, d.nxny, d.sqrt_nxny,
+                                                    sp->t32.as<float2>() + (int64_t)s0 * M, c->stream);
+            // the atom list buffer is reused by the next chunk
+            if (!e) e = hipStreamSynchronize(c->stream);
+            a0 = a1;
+        }
         if (!e) e = hipStreamSynchronize(c->stream);
-        a0 = a1;
+        if (e) return bail(fail(c, MSF_ERR_DEVICE, std::string("transmission build: ") + hipGetErrorString(e)));
+        c->specimens.insert(sp);
+        *out = sp;
+        return MSF_OK;
     }
-    if (!e) e = hipStreamSynchronize(c->stream);
-    if (e) return bail(fail(c, MSF_ERR_DEVICE, std::string("transmission build: ") + hipGetErrorString(e)));
-    c->specimens.insert(sp);
-    *out = sp;
-    return MSF_OK;
+    MSF_CATCH_ALL(c)
 }
 
 int msf_specimen_free(msf_specimen* sp)
 {
-    if (!sp) return MSF_ERR_INVALID;
-    if (sp->ctx) {
-        MSF_LOCK(sp->ctx);
-        (void)hipSetDevice(sp->ctx->device);
-        (void)hipStreamSynchronize(sp->ctx->stream);
-        if (sp->ctx->plan.spec_id == sp->id) {
-            sp->ctx->plan.valid = false;
-            sp->ctx->plan.drop_graph();
+    try {
+        if (!sp) return MSF_ERR_INVALID;
+        if (sp->ctx) {
+            MSF_LOCK(sp->ctx);
+            (void)hipSetDevice(sp->ctx->device);
+            (void)hipStreamSynchronize(sp->ctx->stream);
+            if (sp->ctx->plan.spec_id == sp->id) {
+                sp->ctx->plan.valid = false;
+                sp->ctx->plan.drop_graph();
+            }
+            sp->ctx->specimens.erase(sp);
         }
-        sp->ctx->specimens.erase(sp);
+        delete sp;
+        return MSF_OK;
     }
-    delete sp;
-    return MSF_OK;
+    MSF_CATCH_ALL(sp ? sp->ctx : nullptr)
 }
 
 uint32_t msf_specimen_slices(const msf_specimen* sp) { return sp ? (uint32_t)sp->n_slices : 0; }
 
 int msf_specimen_slice_info(const msf_specimen* sp, double* lower, double* thick, uint32_t* atoms)
 {
-    if (!sp) return MSF_ERR_INVALID;
-    for (int s = 0; s < sp->n_slices; ++s) {
-        if (lower) lower[s] = sp->lower[s];
-        if (thick) thick[s] = sp->thick[s];
-        if (atoms) atoms[s] = sp->counts[s];
+    try {
+        if (!sp) return MSF_ERR_INVALID;
+        for (int s = 0; s < sp->n_slices; ++s) {
+            if (lower) lower[s] = sp->lower[s];
+            if (thick) thick[s] = sp->thick[s];
+            if (atoms) atoms[s] = sp->counts[s];
+        }
+        return MSF_OK;
     }
-    return MSF_OK;
+    MSF_CATCH_ALL(sp ? sp->ctx : nullptr)
 }
 
 int msf_specimen_transmission(const msf_specimen* sp, uint32_t slice, double* t)
 {
-    if (!sp || !t || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
-    if (!sp->ctx) return MSF_ERR_INVALID; // its context was destroyed
-    MSF_LOCK(sp->ctx);
-    const size_t M = (size_t)sp->dom.nx * sp->dom.ny;
-    (void)hipSetDevice(sp->ctx->device);
-    if (hipMemcpy(t, sp->t64.as<double2>() + slice * M, M * sizeof(double2), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-        return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the transmission function failed");
-    return MSF_OK;
+    try {
+        if (!sp || !t || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
+        if (!sp->ctx) return MSF_ERR_INVALID; // its context was destroyed
+        MSF_LOCK(sp->ctx);
+        const size_t M = (size_t)sp->dom.nx * sp->dom.ny;
+        (void)hipSetDevice(sp->ctx->device);
+        if (hipMemcpy(t, sp->t64.as<double2>() + slice * M, M * sizeof(double2), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the transmission function failed");
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(sp ? sp->ctx : nullptr)
 }
 
 int msf_specimen_propagator(const msf_specimen* sp, uint32_t slice, double* px, double* py)
 {
-    if (!sp || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
-    if (!sp->ctx) return MSF_ERR_INVALID;
-    MSF_LOCK(sp->ctx);
-    (void)hipSetDevice(sp->ctx->device);
-    const int nx = sp->dom.nx, ny = sp->dom.ny;
-    if (px && hipMemcpy(px, sp->px64.as<double2>() + (size_t)slice * nx, nx * sizeof(double2),
-                        hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the propagator failed");
-    if (py && hipMemcpy(py, sp->py64.as<double2>() + (size_t)slice * ny, ny * sizeof(double2),
-                        hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the propagator failed");
-    return MSF_OK;
+    try {
+        if (!sp || (int)slice >= sp->n_slices) return MSF_ERR_INVALID;
+        if (!sp->ctx) return MSF_ERR_INVALID;
+        MSF_LOCK(sp->ctx);
+        (void)hipSetDevice(sp->ctx->device);
+        const int nx = sp->dom.nx, ny = sp->dom.ny;
+        if (px && hipMemcpy(px, sp->px64.as<double2>() + (size_t)slice * nx, nx * sizeof(double2),
+                            hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the propagator failed");
+        if (py && hipMemcpy(py, sp->py64.as<double2>() + (size_t)slice * ny, ny * sizeof(double2),
+                            hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(sp->ctx, MSF_ERR_DEVICE, "copy of the propagator failed");
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(sp ? sp->ctx : nullptr)
 }
 
 int msf_probe_k(msf_ctx* c, const msf_grid* grid, const msf_optics* o, int32_t sx, int32_t sy,
                 double* psi_k)
 {
-    if (!c || !psi_k) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (int r = check_grid(c, grid)) return r;
-    if (sx < 0 || sy < 0 || sx >= (int32_t)grid->nx || sy >= (int32_t)grid->ny)
-        return fail(c, MSF_ERR_DOMAIN, "shift outside the grid");
-    const int nx = (int)grid->nx, ny = (int)grid->ny;
-    const size_t M = (size_t)nx * ny;
-    std::vector<double> probe(2 * M);
-    if (int r = msf_probe_realspace(c, grid, o, probe.data())) return r;
-    // adfstem.cpp:1590-1671: circular shift by (sx, sy) pixels
-    for (int i = 0; i < nx; ++i) {
-        const int ix = (i < sx) ? i + nx - sx : i - sx;
-        for (int j = 0; j < ny; ++j) {
-            const int iy = (j < sy) ? j + ny - sy : j - sy;
-            psi_k[2 * ((size_t)i * ny + j)] = probe[2 * ((size_t)ix * ny + iy)];
-            psi_k[2 * ((size_t)i * ny + j) + 1] = probe[2 * ((size_t)ix * ny + iy) + 1];
+    try {
+        if (!c || !psi_k) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (int r = check_grid(c, grid)) return r;
+        if (sx < 0 || sy < 0 || sx >= (int32_t)grid->nx || sy >= (int32_t)grid->ny)
+            return fail(c, MSF_ERR_DOMAIN, "shift outside the grid");
+        const int nx = (int)grid->nx, ny = (int)grid->ny;
+        const size_t M = (size_t)nx * ny;
+        std::vector<double> probe(2 * M);
+        if (int r = msf_probe_realspace(c, grid, o, probe.data())) return r;
+        // adfstem.cpp:1590-1671: circular shift by (sx, sy) pixels
+        for (int i = 0; i < nx; ++i) {
+            const int ix = (i < sx) ? i + nx - sx : i - sx;
+            for (int j = 0; j < ny; ++j) {
+                const int iy = (j < sy) ? j + ny - sy : j - sy;
+                psi_k[2 * ((size_t)i * ny + j)] = probe[2 * ((size_t)ix * ny + iy)];
+                psi_k[2 * ((size_t)i * ny + j) + 1] = probe[2 * ((size_t)ix * ny + iy) + 1];
+            }
         }
+        // :1831-1837: FFT on the device, / sqrt(NxNy)
+        const double2 *twx, *twy;
+        if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+        if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+        DevBuf d;
+        HIPCHK(c, d.alloc(M * sizeof(double2)));
+        HIPCHK(c, hipMemcpyAsync(d.p, psi_k, M * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, msf::fft2d_f64(d.as<double2>(), nx, ny, 1, false, twx, twy, c->stream));
+        HIPCHK(c, hipMemcpyAsync(psi_k, d.p, M * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double s = (nx == ny) ? (double)nx : std::sqrt((double)M);
+        for (size_t k = 0; k < 2 * M; ++k) psi_k[k] = psi_k[k] / s;
+        return MSF_OK;
     }
-    // :1831-1837: FFT on the device, / sqrt(NxNy)
-    const double2 *twx, *twy;
-    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
-    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
-    DevBuf d;
-    HIPCHK(c, d.alloc(M * sizeof(double2)));
-    HIPCHK(c, hipMemcpyAsync(d.p, psi_k, M * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, msf::fft2d_f64(d.as<double2>(), nx, ny, 1, false, twx, twy, c->stream));
-    HIPCHK(c, hipMemcpyAsync(psi_k, d.p, M * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const double s = (nx == ny) ? (double)nx : std::sqrt((double)M);
-    for (size_t k = 0; k < 2 * M; ++k) psi_k[k] = psi_k[k] / s;
-    return MSF_OK;
+    MSF_CATCH_ALL(c)
 }
 
 int msf_slice_step_batch(msf_ctx* c, const msf_specimen* sp, uint32_t slice, double* psi_k,
                          uint32_t count)
 {
-    if (!c || !sp || !psi_k || count == 0) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (sp->ctx != c) return fail(c, MSF_ERR_INVALID, "specimen was built on another context");
-    if ((int)slice >= sp->n_slices) return fail(c, MSF_ERR_INVALID, "slice index out of range");
-    HIPCHK(c, hipSetDevice(c->device));
-    const msf::Domain& d = sp->dom;
-    const int nx = d.nx, ny = d.ny;
-    const size_t M = (size_t)nx * ny;
-    const double2 *twx, *twy;
-    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
-    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
-    DevBuf w;
-    HIPCHK(c, w.alloc(M * count * sizeof(double2)));
-    double2* v = w.as<double2>();
-    HIPCHK(c, hipMemcpyAsync(v, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    // adfstem.cpp:1891 IFFT; :1922-1952 transmission / s; :1986-1992 FFT / s;
-    // bw_limit.cpp:30-47 and slice.cpp:1366-1421 mask and propagator
-    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, true, twx, twy, c->stream));
-    HIPCHK(c, msf::launch_transmit_f64(v, sp->t64.as<double2>() + (size_t)slice * M, (int64_t)M,
-                                       (int)count, c->cfg.transmit_mode, d.sqrt_nxny, c->stream));
-    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, false, twx, twy, c->stream));
-    for (uint32_t k = 0; k < count; ++k)
-        HIPCHK(c, msf::launch_vacuum_step(v + (size_t)k * M, nx, ny,
-                                          sp->px64.as<double2>() + (size_t)slice * nx,
-                                          sp->py64.as<double2>() + (size_t)slice * ny,
-                                          sp->dkeep.as<int2>(), 1.0 / d.sqrt_nxny, c->stream));
-    HIPCHK(c, hipMemcpyAsync(psi_k, v, M * count * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return MSF_OK;
+    try {
+        if (!c || !sp || !psi_k || count == 0) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (sp->ctx != c) return fail(c, MSF_ERR_INVALID, "specimen was built on another context");
+        if ((int)slice >= sp->n_slices) return fail(c, MSF_ERR_INVALID, "slice index out of range");
+        HIPCHK(c, hipSetDevice(c->device));
+        const msf::Domain& d = sp->dom;
+        const int nx = d.nx, ny = d.ny;
+        const size_t M = (size_t)nx * ny;
+        const double2 *twx, *twy;
+        if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+        if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+        DevBuf w;
+        HIPCHK(c, w.alloc(M * count * sizeof(double2)));
+        double2* v = w.as<double2>();
+        HIPCHK(c, hipMemcpyAsync(v, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+        // adfstem.cpp:1891 IFFT; :1922-1952 transmission / s; :1986-1992 FFT / s;
+        // bw_limit.cpp:30-47 and slice.cpp:1366-1421 mask and propagator
+        HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, true, twx, twy, c->stream));
+        HIPCHK(c, msf::launch_transmit_f64(v, sp->t64.as<double2>() + (size_t)slice * M, (int64_t)M,
+                                           (int)count, c->cfg.transmit_mode, d.sqrt_nxny, c->stream));
+        HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, false, twx, twy, c->stream));
+        for (uint32_t k = 0; k < count; ++k)
+            HIPCHK(c, msf::launch_vacuum_step(v + (size_t)k * M, nx, ny,
+                                              sp->px64.as<double2>() + (size_t)slice * nx,
+                                              sp->py64.as<double2>() + (size_t)slice * ny,
+                                              sp->dkeep.as<int2>(), 1.0 / d.sqrt_nxny, c->stream));
+        HIPCHK(c, hipMemcpyAsync(psi_k, v, M * count * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_probe_realspace(msf_ctx* c, const msf_grid* grid, const msf_optics* o, double* probe)
 {
-    if (!c || !probe) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (int r = check_grid(c, grid)) return r;
-    if (int r = check_optics(c, o)) return r;
-    HIPCHK(c, hipSetDevice(c->device));
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    DevBuf h, norm2;
-    if (int r = probe_device(c, d, o, h, norm2)) return r;
-    const size_t M = (size_t)d.nx * d.ny;
-    HIPCHK(c, hipMemcpy(probe, h.p, M * sizeof(double2), hipMemcpyDeviceToHost));
-    // adfstem.cpp:699-704 (/ sqrtNxNy), probe_wavefunction.cpp:300-321 (A_p), :813-817
-    for (size_t k = 0; k < 2 * M; ++k) probe[k] = probe[k] / d.sqrt_nxny;
-    double ap = 0.0;
-    for (size_t k = 0; k < M; ++k) ap += probe[2 * k] * probe[2 * k] + probe[2 * k + 1] * probe[2 * k + 1];
-    ap = 1.0 / std::sqrt(ap);
-    for (size_t k = 0; k < 2 * M; ++k) probe[k] = ap * probe[k];
-    return MSF_OK;
+    try {
+        if (!c || !probe) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (int r = check_grid(c, grid)) return r;
+        if (int r = check_optics(c, o)) return r;
+        HIPCHK(c, hipSetDevice(c->device));
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        DevBuf h, norm2;
+        if (int r = probe_device(c, d, o, h, norm2)) return r;
+        const size_t M = (size_t)d.nx * d.ny;
+        HIPCHK(c, hipMemcpy(probe, h.p, M * sizeof(double2), hipMemcpyDeviceToHost));
+        // adfstem.cpp:699-704 (/ sqrtNxNy), probe_wavefunction.cpp:300-321 (A_p), :813-817
+        for (size_t k = 0; k < 2 * M; ++k) probe[k] = probe[k] / d.sqrt_nxny;
+        double ap = 0.0;
+        for (size_t k = 0; k < M; ++k) ap += probe[2 * k] * probe[2 * k] + probe[2 * k + 1] * probe[2 * k + 1];
+        ap = 1.0 / std::sqrt(ap);
+        for (size_t k = 0; k < 2 * M; ++k) probe[k] = ap * probe[k];
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_fem_bins(const msf_grid* grid, double dr, uint32_t* n_bins, double* lower_edges)
 {
-    if (!grid || !n_bins || !(dr > 0.0) || grid->nx < 2 || grid->ny < 2) return MSF_ERR_INVALID;
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    double dk = 0.0;
-    const int nb = msf::fem_bin_count(d, dr, dk);
-    *n_bins = (uint32_t)nb;
-    if (lower_edges)
-        for (int i = 0; i < nb; ++i) lower_edges[i] = (unsigned int)i * dk;
-    return MSF_OK;
+    try {
+        if (!grid || !n_bins || !(dr > 0.0) || grid->nx < 2 || grid->ny < 2) return MSF_ERR_INVALID;
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        double dk = 0.0;
+        const int nb = msf::fem_bin_count(d, dr, dk);
+        *n_bins = (uint32_t)nb;
+        if (lower_edges)
+            for (int i = 0; i < nb; ++i) lower_edges[i] = (unsigned int)i * dk;
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(nullptr)
 }
 
 } // extern "C"
@@ -1698,161 +1760,179 @@ extern "C" {
 int msf_adf_reduce(msf_ctx* c, const msf_grid* grid, const msf_detector* det,
                    const double* psi_k, uint32_t count, double* adf)
 {
-    if (!c) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (!det || !psi_k || !adf) return fail(c, MSF_ERR_INVALID, "NULL argument");
-    if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
-    if (int r = check_grid(c, grid)) return r;
-    if (det->inner < 0.0 || det->outer < 0.0)
-        return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
-    HIPCHK(c, hipSetDevice(c->device));
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    double k1 = det->inner / d.lambda, k2 = det->outer / d.lambda; // adfstem.cpp:2251-2252
-    if (det->threshold_mode == MSF_ADF_PHYSICAL) {
-        k1 = k1 * k1;
-        k2 = k2 * k2;
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (!det || !psi_k || !adf) return fail(c, MSF_ERR_INVALID, "NULL argument");
+        if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
+        if (int r = check_grid(c, grid)) return r;
+        if (det->inner < 0.0 || det->outer < 0.0)
+            return fail(c, MSF_ERR_INVALID, "detector angles must be >= 0 (tem.cpp:710-720)");
+        HIPCHK(c, hipSetDevice(c->device));
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        double k1 = det->inner / d.lambda, k2 = det->outer / d.lambda; // adfstem.cpp:2251-2252
+        if (det->threshold_mode == MSF_ADF_PHYSICAL) {
+            k1 = k1 * k1;
+            k2 = k2 * k2;
+        }
+        const msf::Groups g = msf::build_groups(d, full_ranges(d.nx, d.ny), d.ny, true, k1, k2,
+                                                false, 1.0, 0.0);
+        std::vector<double> part;
+        if (int r = group_sums_host(c, g, d, psi_k, count, part)) return r;
+        const size_t G = g.kind.size();
+        for (uint32_t p = 0; p < count; ++p) adf[p] = part[p * G];
+        return MSF_OK;
     }
-    const msf::Groups g = msf::build_groups(d, full_ranges(d.nx, d.ny), d.ny, true, k1, k2,
-                                            false, 1.0, 0.0);
-    std::vector<double> part;
-    if (int r = group_sums_host(c, g, d, psi_k, count, part)) return r;
-    const size_t G = g.kind.size();
-    for (uint32_t p = 0; p < count; ++p) adf[p] = part[p * G];
-    return MSF_OK;
+    MSF_CATCH_ALL(c)
 }
 
 int msf_fem_accumulate(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, double alpha_max,
                        const double* psi_k, uint32_t count, double* radial, int32_t* counts,
                        double* v_omega)
 {
-    if (!c) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (!fem || !psi_k) return fail(c, MSF_ERR_INVALID, "NULL argument");
-    if (!(fem->dr > 0.0)) return fail(c, MSF_ERR_INVALID, "dr must be positive");
-    if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
-    if (int r = check_grid(c, grid)) return r;
-    HIPCHK(c, hipSetDevice(c->device));
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    const msf::Groups g = msf::build_groups(d, full_ranges(d.nx, d.ny), d.ny, false, 0.0, 0.0,
-                                            true, fem->dr, alpha_max);
-    std::vector<double> part;
-    if (int r = group_sums_host(c, g, d, psi_k, count, part)) return r;
-    const int nb = g.nbins, G = (int)g.kind.size();
-    if (radial)
-        for (uint32_t p = 0; p < count; ++p)
-            for (int k = 0; k < nb; ++k)
-                radial[(size_t)p * nb + k] = g.zeroed[k] ? 0.0 : part[(size_t)p * G + g.fem_abs + k];
-    if (counts)
-        for (int k = 0; k < nb; ++k) counts[k] = g.counts[k];
-    if (v_omega && nb > 0) { // the production kernels on these sums (adfstem.cpp:2591-2599, :2837-2856)
-        DevBuf dp, dc, dz, st, v;
-        HIPCHK(c, upload(dp, part, c->stream));
-        HIPCHK(c, upload(dc, g.counts, c->stream));
-        HIPCHK(c, upload(dz, g.zeroed, c->stream));
-        HIPCHK(c, st.alloc((size_t)nb * 3 * sizeof(double)));
-        HIPCHK(c, v.alloc((size_t)nb * sizeof(double)));
-        HIPCHK(c, hipMemsetAsync(st.p, 0, (size_t)nb * 3 * sizeof(double), c->stream));
-        HIPCHK(c, msf::launch_fem_welford(dp.as<double>(), G, (int)count, nb, dc.as<int32_t>(),
-                                          dz.as<uint8_t>(), st.as<double>(), c->stream));
-        HIPCHK(c, msf::launch_fem_finalize(st.as<double>(), nb, v.as<double>(), c->stream));
-        HIPCHK(c, hipMemcpyAsync(v_omega, v.p, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (!fem || !psi_k) return fail(c, MSF_ERR_INVALID, "NULL argument");
+        if (!(fem->dr > 0.0)) return fail(c, MSF_ERR_INVALID, "dr must be positive");
+        if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
+        if (int r = check_grid(c, grid)) return r;
+        HIPCHK(c, hipSetDevice(c->device));
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        const msf::Groups g = msf::build_groups(d, full_ranges(d.nx, d.ny), d.ny, false, 0.0, 0.0,
+                                                true, fem->dr, alpha_max);
+        std::vector<double> part;
+        if (int r = group_sums_host(c, g, d, psi_k, count, part)) return r;
+        const int nb = g.nbins, G = (int)g.kind.size();
+        if (radial)
+            for (uint32_t p = 0; p < count; ++p)
+                for (int k = 0; k < nb; ++k)
+                    radial[(size_t)p * nb + k] = g.zeroed[k] ? 0.0 : part[(size_t)p * G + g.fem_abs + k];
+        if (counts)
+            for (int k = 0; k < nb; ++k) counts[k] = g.counts[k];
+        if (v_omega && nb > 0) { // the production kernels on these sums (adfstem.cpp:2591-2599, :2837-2856)
+            DevBuf dp, dc, dz, st, v;
+            HIPCHK(c, upload(dp, part, c->stream));
+            HIPCHK(c, upload(dc, g.counts, c->stream));
+            HIPCHK(c, upload(dz, g.zeroed, c->stream));
+            HIPCHK(c, st.alloc((size_t)nb * 3 * sizeof(double)));
+            HIPCHK(c, v.alloc((size_t)nb * sizeof(double)));
+            HIPCHK(c, hipMemsetAsync(st.p, 0, (size_t)nb * 3 * sizeof(double), c->stream));
+            HIPCHK(c, msf::launch_fem_welford(dp.as<double>(), G, (int)count, nb, dc.as<int32_t>(),
+                                              dz.as<uint8_t>(), st.as<double>(), c->stream));
+            HIPCHK(c, msf::launch_fem_finalize(st.as<double>(), nb, v.as<double>(), c->stream));
+            HIPCHK(c, hipMemcpyAsync(v_omega, v.p, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        return MSF_OK;
     }
-    return MSF_OK;
+    MSF_CATCH_ALL(c)
 }
 
 int msf_mtf_2d(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, double* mtf2d)
 {
-    if (!c) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (!mtf2d) return fail(c, MSF_ERR_INVALID, "NULL argument");
-    if (int r = check_grid(c, grid)) return r;
-    if (int r = check_mtf(c, fem)) return r;
-    HIPCHK(c, hipSetDevice(c->device));
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    DevBuf m;
-    if (int r = build_mtf2d(c, d, grid->xperiod, grid->yperiod, fem, m)) return r;
-    HIPCHK(c, hipMemcpy(mtf2d, m.p, (size_t)d.nx * d.ny * sizeof(double2), hipMemcpyDeviceToHost));
-    return MSF_OK;
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (!mtf2d) return fail(c, MSF_ERR_INVALID, "NULL argument");
+        if (int r = check_grid(c, grid)) return r;
+        if (int r = check_mtf(c, fem)) return r;
+        HIPCHK(c, hipSetDevice(c->device));
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        DevBuf m;
+        if (int r = build_mtf2d(c, d, grid->xperiod, grid->yperiod, fem, m)) return r;
+        HIPCHK(c, hipMemcpy(mtf2d, m.p, (size_t)d.nx * d.ny * sizeof(double2), hipMemcpyDeviceToHost));
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_apply_mtf(msf_ctx* c, const msf_grid* grid, const msf_fem* fem, double alpha_max,
                   double* psi_k, uint32_t count)
 {
-    if (!c) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    if (!psi_k) return fail(c, MSF_ERR_INVALID, "NULL argument");
-    if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
-    if (int r = check_grid(c, grid)) return r;
-    if (int r = check_mtf(c, fem)) return r;
-    HIPCHK(c, hipSetDevice(c->device));
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    const int nx = d.nx, ny = d.ny;
-    const size_t M = (size_t)nx * ny;
-    DevBuf m, w, out, keep;
-    if (int r = build_mtf2d(c, d, grid->xperiod, grid->yperiod, fem, m)) return r;
-    const double2 *twx, *twy;
-    if (int r = twiddles(c, nx, nullptr, &twx)) return r;
-    if (int r = twiddles(c, ny, nullptr, &twy)) return r;
-    HIPCHK(c, upload(out, msf::outside_aperture(d, alpha_max), c->stream));
-    HIPCHK(c, upload(keep, to_int2(msf::bwlimit_keep_ranges(d)), c->stream));
-    HIPCHK(c, w.alloc(M * count * sizeof(double2)));
-    double2* v = w.as<double2>();
-    HIPCHK(c, hipMemcpyAsync(v, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, msf::launch_mtf_magnitude_f64(v, (int64_t)M, (int)count, out.as<uint8_t>(), c->stream));
-    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, true, twx, twy, c->stream));
-    HIPCHK(c, msf::launch_transmit_f64(v, m.as<double2>(), (int64_t)M, (int)count, 0, d.sqrt_nxny,
-                                       c->stream));
-    HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, false, twx, twy, c->stream));
-    HIPCHK(c, msf::launch_div_f64(v, (int64_t)(M * count), d.sqrt_nxny, c->stream));
-    HIPCHK(c, msf::launch_apply_keep(v, nx, ny, (int)count, keep.as<int2>(), c->stream));
-    HIPCHK(c, hipMemcpyAsync(psi_k, v, M * count * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return MSF_OK;
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        if (!psi_k) return fail(c, MSF_ERR_INVALID, "NULL argument");
+        if (count == 0 || count > 65535) return fail(c, MSF_ERR_INVALID, "count must be 1..65535");
+        if (int r = check_grid(c, grid)) return r;
+        if (int r = check_mtf(c, fem)) return r;
+        HIPCHK(c, hipSetDevice(c->device));
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        const int nx = d.nx, ny = d.ny;
+        const size_t M = (size_t)nx * ny;
+        DevBuf m, w, out, keep;
+        if (int r = build_mtf2d(c, d, grid->xperiod, grid->yperiod, fem, m)) return r;
+        const double2 *twx, *twy;
+        if (int r = twiddles(c, nx, nullptr, &twx)) return r;
+        if (int r = twiddles(c, ny, nullptr, &twy)) return r;
+        HIPCHK(c, upload(out, msf::outside_aperture(d, alpha_max), c->stream));
+        HIPCHK(c, upload(keep, to_int2(msf::bwlimit_keep_ranges(d)), c->stream));
+        HIPCHK(c, w.alloc(M * count * sizeof(double2)));
+        double2* v = w.as<double2>();
+        HIPCHK(c, hipMemcpyAsync(v, psi_k, M * count * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, msf::launch_mtf_magnitude_f64(v, (int64_t)M, (int)count, out.as<uint8_t>(), c->stream));
+        HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, true, twx, twy, c->stream));
+        HIPCHK(c, msf::launch_transmit_f64(v, m.as<double2>(), (int64_t)M, (int)count, 0, d.sqrt_nxny,
+                                           c->stream));
+        HIPCHK(c, msf::fft2d_f64(v, nx, ny, (int)count, false, twx, twy, c->stream));
+        HIPCHK(c, msf::launch_div_f64(v, (int64_t)(M * count), d.sqrt_nxny, c->stream));
+        HIPCHK(c, msf::launch_apply_keep(v, nx, ny, (int)count, keep.as<int2>(), c->stream));
+        HIPCHK(c, hipMemcpyAsync(psi_k, v, M * count * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_correlograph_bins(const msf_grid* grid, double dr, uint32_t* n_phi, double* k_edges,
                           double* phi_edges)
 {
-    if (!grid || !n_phi || !(dr > 0.0) || grid->nx < 2 || grid->ny < 2) return MSF_ERR_INVALID;
-    const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
-                                           grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
-    double dk = 0.0;
-    const int nb = msf::fem_bin_count(d, dr, dk);
-    const int np = msf::correlograph_nphi(dk, nb);
-    *n_phi = (uint32_t)np;
-    if (k_edges)
-        for (unsigned int ii = 0; ii < (unsigned int)nb + 1; ++ii) k_edges[ii] = ii * dk;
-    if (phi_edges && np > 0) {
-        const double dphi = 2 * msf::kRefPi / np;
-        for (unsigned int ii = 0; ii < (unsigned int)np + 1; ++ii) phi_edges[ii] = ii * dphi;
+    try {
+        if (!grid || !n_phi || !(dr > 0.0) || grid->nx < 2 || grid->ny < 2) return MSF_ERR_INVALID;
+        const msf::Domain d = msf::make_domain((int)grid->nx, (int)grid->ny, grid->xperiod,
+                                               grid->yperiod, grid->xmin, grid->ymin, grid->voltage);
+        double dk = 0.0;
+        const int nb = msf::fem_bin_count(d, dr, dk);
+        const int np = msf::correlograph_nphi(dk, nb);
+        *n_phi = (uint32_t)np;
+        if (k_edges)
+            for (unsigned int ii = 0; ii < (unsigned int)nb + 1; ++ii) k_edges[ii] = ii * dk;
+        if (phi_edges && np > 0) {
+            const double dphi = 2 * msf::kRefPi / np;
+            for (unsigned int ii = 0; ii < (unsigned int)np + 1; ++ii) phi_edges[ii] = ii * dphi;
+        }
+        return MSF_OK;
     }
-    return MSF_OK;
+    MSF_CATCH_ALL(nullptr)
 }
 
 int msf_run_stem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const msf_detector* det,
                  const msf_fem* fem, const double* x_p, uint32_t n_xp, const double* y_p,
                  uint32_t n_yp, double* adf_image, double* fem_v)
 {
-    if (!c) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    HIPCHK(c, hipSetDevice(c->device));
-    if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
-    if (fem && fem->enabled && !fem_v) return fail(c, MSF_ERR_INVALID, "fem_v is NULL");
-    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
-    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
-    std::vector<int2> shifts;
-    if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
-        return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
-    RunOut out;
-    out.adf = adf_image;
-    out.fem = fem_v;
-    return run_internal(c, sp, o, det, fem, shifts, out);
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        HIPCHK(c, hipSetDevice(c->device));
+        if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
+        if (fem && fem->enabled && !fem_v) return fail(c, MSF_ERR_INVALID, "fem_v is NULL");
+        if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+        if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+        std::vector<int2> shifts;
+        if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
+            return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
+        RunOut out;
+        out.adf = adf_image;
+        out.fem = fem_v;
+        return run_internal(c, sp, o, det, fem, shifts, out);
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
@@ -1860,77 +1940,89 @@ int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                      uint32_t n_xp, const double* y_p, uint32_t n_yp, double* adf_image,
                      const msf_fem_outputs* fo)
 {
-    if (!c) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    HIPCHK(c, hipSetDevice(c->device));
-    if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
-    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
-    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
-    std::vector<int2> shifts;
-    if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
-        return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
-    RunOut out;
-    out.adf = adf_image;
-    if (fo) {
-        out.fem = fo->v_omega;
-        out.vbar_r = fo->vbar_r;
-        out.v_re = fo->v_re;
-        out.omega_vimage = fo->omega_vimage;
-        out.v_gt17 = fo->v_gt17;
-        out.v_rbar = fo->v_rbar;
-        out.corr_avg = fo->correlograph_avg;
-        out.corr_sqr = fo->correlograph_sqr_sum;
-        out.corr_each = fo->correlograph_each;
-        out.mean_abs = fo->diffraction_mean_abs;
-        out.mean_sqr = fo->diffraction_mean_sqr;
-        out.var2d = fo->variance_2d;
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        HIPCHK(c, hipSetDevice(c->device));
+        if (det && det->enabled && !adf_image) return fail(c, MSF_ERR_INVALID, "adf_image is NULL");
+        if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+        if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+        std::vector<int2> shifts;
+        if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
+            return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
+        RunOut out;
+        out.adf = adf_image;
+        if (fo) {
+            out.fem = fo->v_omega;
+            out.vbar_r = fo->vbar_r;
+            out.v_re = fo->v_re;
+            out.omega_vimage = fo->omega_vimage;
+            out.v_gt17 = fo->v_gt17;
+            out.v_rbar = fo->v_rbar;
+            out.corr_avg = fo->correlograph_avg;
+            out.corr_sqr = fo->correlograph_sqr_sum;
+            out.corr_each = fo->correlograph_each;
+            out.mean_abs = fo->diffraction_mean_abs;
+            out.mean_sqr = fo->diffraction_mean_sqr;
+            out.var2d = fo->variance_2d;
+        }
+        return run_internal(c, sp, o, det, fem, shifts, out);
     }
-    return run_internal(c, sp, o, det, fem, shifts, out);
+    MSF_CATCH_ALL(c)
 }
 
 int msf_probe_shifts(const msf_grid* g, const double* x_p, uint32_t n_xp, const double* y_p,
                      uint32_t n_yp, int32_t* shifts)
 {
-    if (!g || !x_p || !y_p || !shifts || g->nx < 2 || g->ny < 2) return MSF_ERR_INVALID;
-    const msf::Domain d =
-        msf::make_domain((int)g->nx, (int)g->ny, g->xperiod, g->yperiod, g->xmin, g->ymin, g->voltage);
-    std::vector<int2> sh;
-    if (!raster_shifts(d, x_p, n_xp, y_p, n_yp, sh)) return MSF_ERR_DOMAIN;
-    for (size_t k = 0; k < sh.size(); ++k) {
-        shifts[2 * k] = sh[k].x;
-        shifts[2 * k + 1] = sh[k].y;
+    try {
+        if (!g || !x_p || !y_p || !shifts || g->nx < 2 || g->ny < 2) return MSF_ERR_INVALID;
+        const msf::Domain d =
+            msf::make_domain((int)g->nx, (int)g->ny, g->xperiod, g->yperiod, g->xmin, g->ymin, g->voltage);
+        std::vector<int2> sh;
+        if (!raster_shifts(d, x_p, n_xp, y_p, n_yp, sh)) return MSF_ERR_DOMAIN;
+        for (size_t k = 0; k < sh.size(); ++k) {
+            shifts[2 * k] = sh[k].x;
+            shifts[2 * k + 1] = sh[k].y;
+        }
+        return MSF_OK;
     }
-    return MSF_OK;
+    MSF_CATCH_ALL(nullptr)
 }
 
 int msf_exit_waves_shifted(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
                            const int32_t* shifts, uint32_t n, float* psi)
 {
-    if (!c || !psi || !shifts) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    HIPCHK(c, hipSetDevice(c->device));
-    if (n == 0) return fail(c, MSF_ERR_INVALID, "empty shift list");
-    std::vector<int2> sh(n);
-    for (uint32_t k = 0; k < n; ++k) sh[k] = int2{shifts[2 * k], shifts[2 * k + 1]};
-    RunOut out;
-    out.exit = psi;
-    return run_internal(c, sp, o, nullptr, nullptr, sh, out);
+    try {
+        if (!c || !psi || !shifts) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        HIPCHK(c, hipSetDevice(c->device));
+        if (n == 0) return fail(c, MSF_ERR_INVALID, "empty shift list");
+        std::vector<int2> sh(n);
+        for (uint32_t k = 0; k < n; ++k) sh[k] = int2{shifts[2 * k], shifts[2 * k + 1]};
+        RunOut out;
+        out.exit = psi;
+        return run_internal(c, sp, o, nullptr, nullptr, sh, out);
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_exit_waves(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const double* x_p,
                    uint32_t n_xp, const double* y_p, uint32_t n_yp, float* psi)
 {
-    if (!c || !psi) return MSF_ERR_INVALID;
-    MSF_LOCK(c);
-    HIPCHK(c, hipSetDevice(c->device));
-    if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
-    if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
-    std::vector<int2> shifts;
-    if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
-        return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
-    RunOut out;
-    out.exit = psi;
-    return run_internal(c, sp, o, nullptr, nullptr, shifts, out);
+    try {
+        if (!c || !psi) return MSF_ERR_INVALID;
+        MSF_LOCK(c);
+        HIPCHK(c, hipSetDevice(c->device));
+        if (!sp) return fail(c, MSF_ERR_INVALID, "specimen is NULL");
+        if (!x_p || !y_p || n_xp == 0 || n_yp == 0) return fail(c, MSF_ERR_INVALID, "empty raster");
+        std::vector<int2> shifts;
+        if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
+            return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
+        RunOut out;
+        out.exit = psi;
+        return run_internal(c, sp, o, nullptr, nullptr, shifts, out);
+    }
+    MSF_CATCH_ALL(c)
 }
 
 // ---------------------------------------------------------------- serving
@@ -2057,111 +2149,126 @@ extern "C" {
 
 int msf_submit(msf_ctx* c, const msf_job* j, uint64_t* id)
 {
-    if (!c || !j || !id) return MSF_ERR_INVALID;
-    if (!j->x_p || !j->y_p || j->n_xp == 0 || j->n_yp == 0 || (j->n_atoms && (!j->q_xyz || !j->Z)))
-        return MSF_ERR_INVALID;
-    if (j->detector.enabled && !j->adf_image) return MSF_ERR_INVALID;
-    ServeJob job;
-    job.grid = j->grid;
-    job.q.assign(j->q_xyz, j->q_xyz + 3 * (size_t)j->n_atoms);
-    job.Z.assign(j->Z, j->Z + j->n_atoms);
-    job.min_slice = j->min_slice;
-    job.optics = j->optics;
-    job.xp.assign(j->x_p, j->x_p + j->n_xp);
-    job.yp.assign(j->y_p, j->y_p + j->n_yp);
-    job.det = j->detector;
-    job.fem = j->fem;
-    if (j->fem.mtf && j->fem.mtf_domain && j->fem.mtf_n) {
-        job.mtf.assign(j->fem.mtf, j->fem.mtf + j->fem.mtf_n);
-        job.mtf_dom.assign(j->fem.mtf_domain, j->fem.mtf_domain + j->fem.mtf_n);
+    try {
+        if (!c || !j || !id) return MSF_ERR_INVALID;
+        if (!j->x_p || !j->y_p || j->n_xp == 0 || j->n_yp == 0 || (j->n_atoms && (!j->q_xyz || !j->Z)))
+            return MSF_ERR_INVALID;
+        if (j->detector.enabled && !j->adf_image) return MSF_ERR_INVALID;
+        ServeJob job;
+        job.grid = j->grid;
+        job.q.assign(j->q_xyz, j->q_xyz + 3 * (size_t)j->n_atoms);
+        job.Z.assign(j->Z, j->Z + j->n_atoms);
+        job.min_slice = j->min_slice;
+        job.optics = j->optics;
+        job.xp.assign(j->x_p, j->x_p + j->n_xp);
+        job.yp.assign(j->y_p, j->y_p + j->n_yp);
+        job.det = j->detector;
+        job.fem = j->fem;
+        if (j->fem.mtf && j->fem.mtf_domain && j->fem.mtf_n) {
+            job.mtf.assign(j->fem.mtf, j->fem.mtf + j->fem.mtf_n);
+            job.mtf_dom.assign(j->fem.mtf_domain, j->fem.mtf_domain + j->fem.mtf_n);
+        }
+        job.fem.mtf = nullptr;
+        job.fem.mtf_domain = nullptr;
+        job.adf = j->adf_image;
+        job.fo = j->fem_out;
+        {
+            MSF_LOCK(c);
+            job.cfg = c->cfg;
+            job.cfg.profile_kernels = 0;
+            // served jobs launch on their streams: a graph capture on one worker
+            // would be invalidated by another worker's allocations
+            job.cfg.no_graph = 1;
+        }
+        job.footprint = job_footprint(job);
+        job.t_submit = now_ms();
+        if (int r = start_serving(c)) return r;
+        ServeState& st = *c->serve;
+        {
+            std::lock_guard<std::mutex> lk(st.m);
+            job.id = st.next_id++;
+            *id = job.id;
+            st.pending.insert(job.id);
+            st.queue.push_back(std::move(job));
+        }
+        st.work_cv.notify_one();
+        return MSF_OK;
     }
-    job.fem.mtf = nullptr;
-    job.fem.mtf_domain = nullptr;
-    job.adf = j->adf_image;
-    job.fo = j->fem_out;
-    {
-        MSF_LOCK(c);
-        job.cfg = c->cfg;
-        job.cfg.profile_kernels = 0;
-        // served jobs launch on their streams: a graph capture on one worker
-        // would be invalidated by another worker's allocations
-        job.cfg.no_graph = 1;
-    }
-    job.footprint = job_footprint(job);
-    job.t_submit = now_ms();
-    if (int r = start_serving(c)) return r;
-    ServeState& st = *c->serve;
-    {
-        std::lock_guard<std::mutex> lk(st.m);
-        job.id = st.next_id++;
-        *id = job.id;
-        st.pending.insert(job.id);
-        st.queue.push_back(std::move(job));
-    }
-    st.work_cv.notify_one();
-    return MSF_OK;
+    MSF_CATCH_ALL(c)
 }
 
 int msf_poll(msf_ctx* c, uint64_t id)
 {
-    if (!c || !c->serve) return MSF_ERR_INVALID;
-    ServeState& st = *c->serve;
-    std::lock_guard<std::mutex> lk(st.m);
-    auto it = st.done.find(id);
-    if (it != st.done.end()) return it->second.rc;
-    return st.pending.count(id) ? MSF_PENDING : MSF_ERR_INVALID;
+    try {
+        if (!c || !c->serve) return MSF_ERR_INVALID;
+        ServeState& st = *c->serve;
+        std::lock_guard<std::mutex> lk(st.m);
+        auto it = st.done.find(id);
+        if (it != st.done.end()) return it->second.rc;
+        return st.pending.count(id) ? MSF_PENDING : MSF_ERR_INVALID;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_wait_info(msf_ctx* c, uint64_t id, msf_job_info* info)
 {
-    if (!c || !c->serve) return MSF_ERR_INVALID;
-    ServeState& st = *c->serve;
-    ServeDone r;
-    {
-        std::unique_lock<std::mutex> lk(st.m);
-        if (!st.done.count(id) && !st.pending.count(id)) return MSF_ERR_INVALID;
-        st.done_cv.wait(lk, [&] { return st.done.count(id) != 0; });
-        r = st.done[id];
-        st.done.erase(id);
+    try {
+        if (!c || !c->serve) return MSF_ERR_INVALID;
+        ServeState& st = *c->serve;
+        ServeDone r;
+        {
+            std::unique_lock<std::mutex> lk(st.m);
+            if (!st.done.count(id) && !st.pending.count(id)) return MSF_ERR_INVALID;
+            st.done_cv.wait(lk, [&] { return st.done.count(id) != 0; });
+            r = st.done[id];
+            st.done.erase(id);
+        }
+        if (info) *info = r.info;
+        if (r.rc != MSF_OK) {
+            MSF_LOCK(c);
+            c->err = "job " + std::to_string(id) + ": " + r.msg;
+        }
+        return r.rc;
     }
-    if (info) *info = r.info;
-    if (r.rc != MSF_OK) {
-        MSF_LOCK(c);
-        c->err = "job " + std::to_string(id) + ": " + r.msg;
-    }
-    return r.rc;
+    MSF_CATCH_ALL(c)
 }
 
 int msf_wait(msf_ctx* c, uint64_t id) { return msf_wait_info(c, id, nullptr); }
 
 int msf_serve_configure(msf_ctx* c, uint32_t slots, uint64_t mem_limit)
 {
-    if (!c) return MSF_ERR_INVALID;
-    std::lock_guard<std::mutex> lk(c->serve_mu);
-    if (c->serve) return fail(c, MSF_ERR_INVALID, "the serving engine has already started");
-    if (slots > 16) return fail(c, MSF_ERR_INVALID, "slots must be 0..16");
-    c->serve_slots = slots;
-    c->serve_limit = mem_limit;
-    return MSF_OK;
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        std::lock_guard<std::mutex> lk(c->serve_mu);
+        if (c->serve) return fail(c, MSF_ERR_INVALID, "the serving engine has already started");
+        if (slots > 16) return fail(c, MSF_ERR_INVALID, "slots must be 0..16");
+        c->serve_slots = slots;
+        c->serve_limit = mem_limit;
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 int msf_serve_get_stats(msf_ctx* c, msf_serve_stats* out)
 {
-    if (!c || !out) return MSF_ERR_INVALID;
-    *out = msf_serve_stats{};
-    std::lock_guard<std::mutex> lk0(c->serve_mu);
-    if (!c->serve) return MSF_OK;
-    ServeState& st = *c->serve;
-    std::lock_guard<std::mutex> lk(st.m);
-    out->slots = st.slots;
-    out->running = st.running;
-    out->max_running = st.max_running;
-    out->queued = st.queue.size();
-    out->jobs_done = st.jobs_done;
-    out->mem_limit = st.limit;
-    out->admitted_bytes = st.admitted;
-    out->peak_admitted_bytes = st.peak_admitted;
-    return MSF_OK;
+    try {
+        if (!c || !out) return MSF_ERR_INVALID;
+        *out = msf_serve_stats{};
+        std::lock_guard<std::mutex> lk0(c->serve_mu);
+        if (!c->serve) return MSF_OK;
+        ServeState& st = *c->serve;
+        std::lock_guard<std::mutex> lk(st.m);
+        out->slots = st.slots;
+        out->running = st.running;
+        out->max_running = st.max_running;
+        out->queued = st.queue.size();
+        out->jobs_done = st.jobs_done;
+        out->mem_limit = st.limit;
+        out->admitted_bytes = st.admitted;
+        out->peak_admitted_bytes = st.peak_admitted;
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
 }
 
 uint64_t msf_job_footprint(const msf_job* j)

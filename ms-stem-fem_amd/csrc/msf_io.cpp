@@ -12,7 +12,9 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <new>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -25,6 +27,28 @@ int io_fail(int code, const std::string& msg)
     g_err = msg;
     return code;
 }
+
+// Every entry point turns a C++ exception into an error code: none may cross
+// the C ABI.
+#define MSF_IO_CATCH_ALL                                                                      \
+    catch (const std::bad_alloc&)                                                             \
+    {                                                                                         \
+        return io_fail(MSF_ERR_MEMORY, "out of host memory");                                \
+    }                                                                                         \
+    catch (const std::exception& e_)                                                          \
+    {                                                                                         \
+        return io_fail(MSF_ERR_INVALID, std::string("unexpected error: ") + e_.what());      \
+    }                                                                                         \
+    catch (...)                                                                               \
+    {                                                                                         \
+        return io_fail(MSF_ERR_INVALID, "unexpected error");                                 \
+    }
+
+// Room reserved up front for a declared atom count: never more than this, so
+// a malformed count (a negative number read as size_t wraps to 2^64 - 1)
+// fails by running out of atom lines, as the reference's reader does, instead
+// of in the allocator.
+constexpr size_t kReserveAtoms = size_t(1) << 20;
 
 std::string lower(std::string s)
 {
@@ -182,17 +206,20 @@ uint32_t msf_element_z(const char* name)
 
 int msf_guess_z(double mass, uint32_t* Z)
 {
-    if (!Z) return io_fail(MSF_ERR_INVALID, "Z is NULL");
-    for (const MassWindow& w : kMasses) {
-        if ((w.exact != kNone && mass == w.exact) || (mass >= w.lo && mass <= w.hi)) {
-            *Z = w.Z;
-            return MSF_OK;
+    try {
+        if (!Z) return io_fail(MSF_ERR_INVALID, "Z is NULL");
+        for (const MassWindow& w : kMasses) {
+            if ((w.exact != kNone && mass == w.exact) || (mass >= w.lo && mass <= w.hi)) {
+                *Z = w.Z;
+                return MSF_OK;
+            }
         }
+        *Z = 0;
+        std::ostringstream m;
+        m << "could not guess element having mass " << mass;
+        return io_fail(MSF_ERR_INVALID, m.str());
     }
-    *Z = 0;
-    std::ostringstream m;
-    m << "could not guess element having mass " << mass;
-    return io_fail(MSF_ERR_INVALID, m.str());
+    MSF_IO_CATCH_ALL
 }
 
 void msf_model_free(msf_model* m)
@@ -217,358 +244,382 @@ void msf_mtf_table_free(msf_mtf_table* t)
 
 int msf_read_mtf(const char* path, msf_mtf_table* out)
 {
-    if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
-    *out = msf_mtf_table{};
-    std::ifstream in(path);
-    if (!in.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening MTF file: ") + path);
-    std::vector<double> rows[2]; // domain, mtf
-    for (std::vector<double>& v : rows) {
-        std::string line;
-        if (!(std::getline(in, line) && in.good())) break;
-        std::replace(line.begin(), line.end(), ',', ' ');
-        std::istringstream ls(line);
-        // one value per line, as declared by the reference before its loop: an
-        // extraction that finds only trailing blanks leaves it unchanged, so the
-        // last value repeats (for a blank line the reference reads an
-        // uninitialised double; 0 here)
-        double value = 0.0;
-        while (!ls.eof()) {
-            ls >> value;
-            if (ls.fail() && !ls.eof()) // the reference's loop never ends here
-                return io_fail(MSF_ERR_INVALID, std::string("non-numeric entry in MTF file: ") +
-                                                    path);
-            v.push_back(value);
+    try {
+        if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
+        *out = msf_mtf_table{};
+        std::ifstream in(path);
+        if (!in.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening MTF file: ") + path);
+        std::vector<double> rows[2]; // domain, mtf
+        for (std::vector<double>& v : rows) {
+            std::string line;
+            if (!(std::getline(in, line) && in.good())) break;
+            std::replace(line.begin(), line.end(), ',', ' ');
+            std::istringstream ls(line);
+            // one value per line, as declared by the reference before its loop: an
+            // extraction that finds only trailing blanks leaves it unchanged, so the
+            // last value repeats (for a blank line the reference reads an
+            // uninitialised double; 0 here)
+            double value = 0.0;
+            while (!ls.eof()) {
+                ls >> value;
+                if (ls.fail() && !ls.eof()) // the reference's loop never ends here
+                    return io_fail(MSF_ERR_INVALID, std::string("non-numeric entry in MTF file: ") +
+                                                        path);
+                v.push_back(value);
+            }
         }
+        if (rows[0].size() != rows[1].size())
+            return io_fail(MSF_ERR_INVALID, std::string("Error reading MTF file: ") + path +
+                                                "\n MTF (1st line) and MTF domain (2nd line) do not "
+                                                "have the same number of elements.");
+        if (rows[0].empty()) return io_fail(MSF_ERR_INVALID, std::string("empty MTF table: ") + path);
+        const size_t n = rows[0].size();
+        out->domain = (double*)std::malloc(n * sizeof(double));
+        out->mtf = (double*)std::malloc(n * sizeof(double));
+        if (!out->domain || !out->mtf) {
+            msf_mtf_table_free(out);
+            return io_fail(MSF_ERR_INVALID, "out of memory");
+        }
+        std::copy(rows[0].begin(), rows[0].end(), out->domain);
+        std::copy(rows[1].begin(), rows[1].end(), out->mtf);
+        out->n = (uint32_t)n;
+        return MSF_OK;
     }
-    if (rows[0].size() != rows[1].size())
-        return io_fail(MSF_ERR_INVALID, std::string("Error reading MTF file: ") + path +
-                                            "\n MTF (1st line) and MTF domain (2nd line) do not "
-                                            "have the same number of elements.");
-    if (rows[0].empty()) return io_fail(MSF_ERR_INVALID, std::string("empty MTF table: ") + path);
-    const size_t n = rows[0].size();
-    out->domain = (double*)std::malloc(n * sizeof(double));
-    out->mtf = (double*)std::malloc(n * sizeof(double));
-    if (!out->domain || !out->mtf) {
-        msf_mtf_table_free(out);
-        return io_fail(MSF_ERR_INVALID, "out of memory");
-    }
-    std::copy(rows[0].begin(), rows[0].end(), out->domain);
-    std::copy(rows[1].begin(), rows[1].end(), out->mtf);
-    out->n = (uint32_t)n;
-    return MSF_OK;
+    MSF_IO_CATCH_ALL
 }
 
 int msf_read_xyz(const char* path, msf_model* out)
 {
-    if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
-    *out = msf_model{};
-    std::ifstream f(path);
-    if (!f.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening position file: ") + path);
-    std::string line;
-    // line 1: declared population (tem_io.cpp:4659-4688)
-    size_t declared = 0;
-    if (!next_line(f, line)) return io_fail(MSF_ERR_INVALID, "Error reading position data file; first line");
-    {
-        std::istringstream s(line);
-        s >> declared;
-        if (!declared)
-            return io_fail(MSF_ERR_INVALID, "Error reading position data file; declared_population : 0");
-    }
-    // line 2: periods, all > 0 (tem_io.cpp:4690-4731)
-    double xp = 0, yp = 0, zp = 0;
-    if (!next_line(f, line)) return io_fail(MSF_ERR_INVALID, "Error reading position data file; second line");
-    {
-        std::istringstream s(line);
-        s >> xp >> yp >> zp;
-        if (!(xp > 0.0 && yp > 0.0 && zp > 0.0))
-            return io_fail(MSF_ERR_INVALID,
-                           "Error reading xyz position data file; second line should begin with "
-                           "<xperiod> <yperiod> <zperiod>");
-    }
-    // atoms (tem_io.cpp:4747-4815); extrema start at 0
-    std::vector<double> q;
-    std::vector<uint32_t> Z;
-    q.reserve(3 * declared);
-    Z.reserve(declared);
-    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
-    while (Z.size() < declared) {
-        if (!next_line(f, line)) {
-            std::ostringstream m;
-            m << "Error reading position data file; atoms read : " << Z.size()
-              << ", declared population : " << declared;
-            return io_fail(MSF_ERR_INVALID, m.str());
+    try {
+        if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
+        *out = msf_model{};
+        std::ifstream f(path);
+        if (!f.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening position file: ") + path);
+        std::string line;
+        // line 1: declared population (tem_io.cpp:4659-4688)
+        size_t declared = 0;
+        if (!next_line(f, line)) return io_fail(MSF_ERR_INVALID, "Error reading position data file; first line");
+        {
+            std::istringstream s(line);
+            s >> declared;
+            if (!declared)
+                return io_fail(MSF_ERR_INVALID, "Error reading position data file; declared_population : 0");
         }
-        std::istringstream s(line);
-        std::string element;
-        s >> element;
-        const uint32_t z = msf_element_z(lower(element).c_str());
-        if (z < 1 || z > 103) {
-            std::ostringstream m;
-            m << "Error reading position data file; atom " << Z.size() << ", element " << element
-              << " invalid";
-            return io_fail(MSF_ERR_INVALID, m.str());
+        // line 2: periods, all > 0 (tem_io.cpp:4690-4731)
+        double xp = 0, yp = 0, zp = 0;
+        if (!next_line(f, line)) return io_fail(MSF_ERR_INVALID, "Error reading position data file; second line");
+        {
+            std::istringstream s(line);
+            s >> xp >> yp >> zp;
+            if (!(xp > 0.0 && yp > 0.0 && zp > 0.0))
+                return io_fail(MSF_ERR_INVALID,
+                               "Error reading xyz position data file; second line should begin with "
+                               "<xperiod> <yperiod> <zperiod>");
         }
-        double r[3] = {0, 0, 0};
-        s >> r[0] >> r[1] >> r[2];
-        for (int k = 0; k < 3; ++k) {
-            if (r[k] < mn[k]) mn[k] = r[k];
-            if (r[k] > mx[k]) mx[k] = r[k];
-            q.push_back(r[k]);
+        // atoms (tem_io.cpp:4747-4815); extrema start at 0
+        std::vector<double> q;
+        std::vector<uint32_t> Z;
+        q.reserve(3 * std::min(declared, kReserveAtoms));
+        Z.reserve(std::min(declared, kReserveAtoms));
+        double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+        while (Z.size() < declared) {
+            if (!next_line(f, line)) {
+                std::ostringstream m;
+                m << "Error reading position data file; atoms read : " << Z.size()
+                  << ", declared population : " << declared;
+                return io_fail(MSF_ERR_INVALID, m.str());
+            }
+            std::istringstream s(line);
+            std::string element;
+            s >> element;
+            const uint32_t z = msf_element_z(lower(element).c_str());
+            if (z < 1 || z > 103) {
+                std::ostringstream m;
+                m << "Error reading position data file; atom " << Z.size() << ", element " << element
+                  << " invalid";
+                return io_fail(MSF_ERR_INVALID, m.str());
+            }
+            double r[3] = {0, 0, 0};
+            s >> r[0] >> r[1] >> r[2];
+            for (int k = 0; k < 3; ++k) {
+                if (r[k] < mn[k]) mn[k] = r[k];
+                if (r[k] > mx[k]) mx[k] = r[k];
+                q.push_back(r[k]);
+            }
+            Z.push_back(z);
         }
-        Z.push_back(z);
+        if (mx[0] - mn[0] > xp || mx[1] - mn[1] > yp || mx[2] - mn[2] > zp)
+            return io_fail(MSF_ERR_INVALID, "Error reading position data file; positions exceed period");
+        // shift by the (non-positive) minima (tem_io.cpp:4843-4858)
+        for (size_t a = 0; a < Z.size(); ++a)
+            for (int k = 0; k < 3; ++k) q[3 * a + k] = q[3 * a + k] - mn[k];
+        return fill_model(out, q, Z, xp, yp, zp);
     }
-    if (mx[0] - mn[0] > xp || mx[1] - mn[1] > yp || mx[2] - mn[2] > zp)
-        return io_fail(MSF_ERR_INVALID, "Error reading position data file; positions exceed period");
-    // shift by the (non-positive) minima (tem_io.cpp:4843-4858)
-    for (size_t a = 0; a < Z.size(); ++a)
-        for (int k = 0; k < 3; ++k) q[3 * a + k] = q[3 * a + k] - mn[k];
-    return fill_model(out, q, Z, xp, yp, zp);
+    MSF_IO_CATCH_ALL
 }
 
 int msf_read_lammps_data(const char* path, msf_model* out)
 {
-    if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
-    *out = msf_model{};
-    std::ifstream f(path);
-    if (!f.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening position file: ") + path);
-    const std::string where = "Error reading position data file; ";
-    std::string line;
-    std::getline(f, line); // title line (tem_io.cpp:62)
-    // "<N> atoms" (tem_io.cpp:66-118)
-    size_t declared = 0;
-    if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "atoms line");
-    {
-        std::istringstream s(line);
-        std::string d;
-        s >> declared >> d;
-        if (!declared || lower(d) != "atoms")
-            return io_fail(MSF_ERR_INVALID, where + "line should be : <number of atoms> atoms");
-    }
-    // "<n> atom types" (tem_io.cpp:120-176)
-    size_t n_species = 0;
-    if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "atom types line");
-    {
-        std::istringstream s(line);
-        std::string d1, d2;
-        s >> n_species >> d1 >> d2;
-        if (!n_species || lower(d1) != "atom" || lower(d2) != "types")
-            return io_fail(MSF_ERR_INVALID, where + "line should be : <number> atom types");
-    }
-    // box bounds (tem_io.cpp:178-364)
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    const char* names[3][2] = {{"xlo", "xhi"}, {"ylo", "yhi"}, {"zlo", "zhi"}};
-    for (int k = 0; k < 3; ++k) {
-        if (!next_line(f, line) || !skip_blank(f, line))
-            return io_fail(MSF_ERR_INVALID, where + "box bounds line");
-        std::istringstream s(line);
-        std::string d1, d2;
-        s >> lo[k] >> hi[k] >> d1 >> d2;
-        if (lower(d1) != names[k][0] || lower(d2) != names[k][1])
-            return io_fail(MSF_ERR_INVALID, where + "line should be : <lo> <hi> " + names[k][0] + " " +
-                                                names[k][1]);
-    }
-    for (int k = 0; k < 3; ++k)
-        if (hi[k] <= lo[k]) return io_fail(MSF_ERR_INVALID, where + "upper bound <= lower bound");
-    // "Masses" section and one "<id> <mass>" line per species (tem_io.cpp:398-690)
-    if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "Masses");
-    {
-        std::istringstream s(line);
-        std::string sec;
-        s >> sec;
-        if (lower(sec) != "masses") return io_fail(MSF_ERR_INVALID, where + "expected section Masses");
-    }
-    std::vector<size_t> species_id(n_species, 0);
-    std::vector<double> mass(n_species, 0.0);
-    std::vector<uint32_t> species_z(n_species, 0);
-    size_t n_read = 0;
-    while (f.good() && n_read < n_species) {
-        if (next_line(f, line)) {
-            if (!skip_blank(f, line)) break;
+    try {
+        if (!path || !out) return io_fail(MSF_ERR_INVALID, "NULL argument");
+        *out = msf_model{};
+        std::ifstream f(path);
+        if (!f.is_open()) return io_fail(MSF_ERR_INVALID, std::string("Error opening position file: ") + path);
+        const std::string where = "Error reading position data file; ";
+        std::string line;
+        std::getline(f, line); // title line (tem_io.cpp:62)
+        // "<N> atoms" (tem_io.cpp:66-118)
+        size_t declared = 0;
+        if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "atoms line");
+        {
             std::istringstream s(line);
-            s >> species_id[n_read] >> mass[n_read];
+            std::string d;
+            s >> declared >> d;
+            if (!declared || lower(d) != "atoms")
+                return io_fail(MSF_ERR_INVALID, where + "line should be : <number of atoms> atoms");
         }
-        ++n_read;
+        // "<n> atom types" (tem_io.cpp:120-176)
+        size_t n_species = 0;
+        if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "atom types line");
+        {
+            std::istringstream s(line);
+            std::string d1, d2;
+            s >> n_species >> d1 >> d2;
+            if (!n_species || lower(d1) != "atom" || lower(d2) != "types")
+                return io_fail(MSF_ERR_INVALID, where + "line should be : <number> atom types");
+        }
+        // box bounds (tem_io.cpp:178-364)
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        const char* names[3][2] = {{"xlo", "xhi"}, {"ylo", "yhi"}, {"zlo", "zhi"}};
+        for (int k = 0; k < 3; ++k) {
+            if (!next_line(f, line) || !skip_blank(f, line))
+                return io_fail(MSF_ERR_INVALID, where + "box bounds line");
+            std::istringstream s(line);
+            std::string d1, d2;
+            s >> lo[k] >> hi[k] >> d1 >> d2;
+            if (lower(d1) != names[k][0] || lower(d2) != names[k][1])
+                return io_fail(MSF_ERR_INVALID, where + "line should be : <lo> <hi> " + names[k][0] + " " +
+                                                    names[k][1]);
+        }
+        for (int k = 0; k < 3; ++k)
+            if (hi[k] <= lo[k]) return io_fail(MSF_ERR_INVALID, where + "upper bound <= lower bound");
+        // "Masses" section and one "<id> <mass>" line per species (tem_io.cpp:398-690)
+        if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "Masses");
+        {
+            std::istringstream s(line);
+            std::string sec;
+            s >> sec;
+            if (lower(sec) != "masses") return io_fail(MSF_ERR_INVALID, where + "expected section Masses");
+        }
+        // grown line by line: a malformed type count fails when the lines run out
+        std::vector<size_t> species_id;
+        std::vector<double> mass;
+        size_t n_read = 0;
+        while (f.good() && n_read < n_species) {
+            species_id.push_back(0);
+            mass.push_back(0.0);
+            if (next_line(f, line)) {
+                if (!skip_blank(f, line)) break;
+                std::istringstream s(line);
+                s >> species_id[n_read] >> mass[n_read];
+            }
+            ++n_read;
+        }
+        if (n_read != n_species) return io_fail(MSF_ERR_INVALID, where + "fewer masses than atom types");
+        std::vector<uint32_t> species_z(n_species, 0);
+        for (size_t k = 0; k < n_species; ++k)
+            if (int r = msf_guess_z(mass[k], &species_z[k])) return r;
+        // "Atoms" section (tem_io.cpp:700-760)
+        if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "Atoms");
+        {
+            std::istringstream s(line);
+            std::string sec;
+            s >> sec;
+            if (sec.empty() || lower(sec) != "atoms")
+                return io_fail(MSF_ERR_INVALID, where + "expected section Atoms");
+        }
+        // atom lines "<number> <type> <x> <y> <z>" (tem_io.cpp:762-900)
+        std::vector<double> q;
+        std::vector<uint32_t> Z;
+        q.reserve(3 * std::min(declared, kReserveAtoms));
+        Z.reserve(std::min(declared, kReserveAtoms));
+        while (Z.size() < declared) {
+            if (!next_line(f, line) || !skip_blank(f, line))
+                return io_fail(MSF_ERR_INVALID, where + "fewer atoms than declared");
+            std::istringstream s(line);
+            size_t number = 0, type = 0;
+            double r[3] = {0, 0, 0};
+            s >> number >> type >> r[0] >> r[1] >> r[2];
+            size_t k = 0;
+            for (; k < n_species; ++k)
+                if (type == species_id[k]) break;
+            if (k >= n_species) return io_fail(MSF_ERR_INVALID, where + "atom type without a mass");
+            const uint32_t z = species_z[k];
+            if (number > declared || z == 0 || r[0] > hi[0] || r[0] < lo[0] || r[1] > hi[1] ||
+                r[1] < lo[1] || r[2] > hi[2] || r[2] < lo[2])
+                return io_fail(MSF_ERR_INVALID, where + "atom outside the box or numbered beyond the count");
+            for (int c = 0; c < 3; ++c) q.push_back(r[c]);
+            Z.push_back(z);
+        }
+        // distinct atomic numbers must equal the number of atom types (tem_io.cpp:908-960)
+        std::vector<uint32_t> uz(Z);
+        std::sort(uz.begin(), uz.end());
+        uz.erase(std::unique(uz.begin(), uz.end()), uz.end());
+        if (uz.size() != n_species)
+            return io_fail(MSF_ERR_INVALID, where + "distinct elements differ from the number of atom types");
+        // shift by the box's lower bounds (tem_io.cpp:918-923)
+        for (size_t a = 0; a < Z.size(); ++a)
+            for (int c = 0; c < 3; ++c) q[3 * a + c] = q[3 * a + c] - lo[c];
+        return fill_model(out, q, Z, hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
     }
-    if (n_read != n_species) return io_fail(MSF_ERR_INVALID, where + "fewer masses than atom types");
-    for (size_t k = 0; k < n_species; ++k)
-        if (int r = msf_guess_z(mass[k], &species_z[k])) return r;
-    // "Atoms" section (tem_io.cpp:700-760)
-    if (!next_line(f, line) || !skip_blank(f, line)) return io_fail(MSF_ERR_INVALID, where + "Atoms");
-    {
-        std::istringstream s(line);
-        std::string sec;
-        s >> sec;
-        if (sec.empty() || lower(sec) != "atoms")
-            return io_fail(MSF_ERR_INVALID, where + "expected section Atoms");
-    }
-    // atom lines "<number> <type> <x> <y> <z>" (tem_io.cpp:762-900)
-    std::vector<double> q;
-    std::vector<uint32_t> Z;
-    q.reserve(3 * declared);
-    Z.reserve(declared);
-    while (Z.size() < declared) {
-        if (!next_line(f, line) || !skip_blank(f, line))
-            return io_fail(MSF_ERR_INVALID, where + "fewer atoms than declared");
-        std::istringstream s(line);
-        size_t number = 0, type = 0;
-        double r[3] = {0, 0, 0};
-        s >> number >> type >> r[0] >> r[1] >> r[2];
-        size_t k = 0;
-        for (; k < n_species; ++k)
-            if (type == species_id[k]) break;
-        if (k >= n_species) return io_fail(MSF_ERR_INVALID, where + "atom type without a mass");
-        const uint32_t z = species_z[k];
-        if (number > declared || z == 0 || r[0] > hi[0] || r[0] < lo[0] || r[1] > hi[1] ||
-            r[1] < lo[1] || r[2] > hi[2] || r[2] < lo[2])
-            return io_fail(MSF_ERR_INVALID, where + "atom outside the box or numbered beyond the count");
-        for (int c = 0; c < 3; ++c) q.push_back(r[c]);
-        Z.push_back(z);
-    }
-    // distinct atomic numbers must equal the number of atom types (tem_io.cpp:908-960)
-    std::vector<uint32_t> uz(Z);
-    std::sort(uz.begin(), uz.end());
-    uz.erase(std::unique(uz.begin(), uz.end()), uz.end());
-    if (uz.size() != n_species)
-        return io_fail(MSF_ERR_INVALID, where + "distinct elements differ from the number of atom types");
-    // shift by the box's lower bounds (tem_io.cpp:918-923)
-    for (size_t a = 0; a < Z.size(); ++a)
-        for (int c = 0; c < 3; ++c) q[3 * a + c] = q[3 * a + c] - lo[c];
-    return fill_model(out, q, Z, hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
+    MSF_IO_CATCH_ALL
 }
 
 int msf_write_tiff16(const char* path, const double* data, uint32_t height, uint32_t width,
                      double min, double max, int logscale, uint32_t resolution_unit,
                      double x_resolution, double y_resolution)
 {
-    if (!path || !data || height == 0 || width == 0) return io_fail(MSF_ERR_INVALID, "bad image");
-    // scaling (write_tif.hpp:230-264)
-    const double digits = 65535.0; // 2^16 - 1
-    std::vector<uint16_t> img((size_t)height * width);
-    if (logscale) {
-        const double len_log = std::log(max - min + 1);
-        for (size_t i = 0; i < img.size(); ++i) {
-            const double t = std::log(data[i] - min + 1) / len_log;
-            img[i] = (t > 0.0) ? to_u16(t * digits + 0.5) : (uint16_t)0;
+    try {
+        if (!path || !data || height == 0 || width == 0) return io_fail(MSF_ERR_INVALID, "bad image");
+        // scaling (write_tif.hpp:230-264)
+        const double digits = 65535.0; // 2^16 - 1
+        std::vector<uint16_t> img((size_t)height * width);
+        if (logscale) {
+            const double len_log = std::log(max - min + 1);
+            for (size_t i = 0; i < img.size(); ++i) {
+                const double t = std::log(data[i] - min + 1) / len_log;
+                img[i] = (t > 0.0) ? to_u16(t * digits + 0.5) : (uint16_t)0;
+            }
+        } else {
+            const double scale = digits / (max - min);
+            for (size_t i = 0; i < img.size(); ++i) img[i] = to_u16((data[i] - min) * scale + 0.5);
         }
-    } else {
-        const double scale = digits / (max - min);
-        for (size_t i = 0; i < img.size(); ++i) img[i] = to_u16((data[i] - min) * scale + 0.5);
+        // baseline little-endian TIFF: header, pixel strips, IFD, rationals.
+        // RowsPerStrip is the image width, the value the reference requests
+        // from TIFFDefaultStripSize (write_tif.hpp:298-299).
+        const uint32_t rps = width;
+        const uint32_t n_strips = (height + rps - 1) / rps;
+        const size_t row_bytes = (size_t)width * 2;
+        const size_t pix_off = 8;
+        const size_t pix_bytes = row_bytes * height;
+        size_t ifd_off = pix_off + pix_bytes;
+        ifd_off += ifd_off & 1;
+        const int n_tags = 14;
+        const size_t ifd_bytes = 2 + 12 * (size_t)n_tags + 4;
+        const size_t extra_off = ifd_off + ifd_bytes;  // rationals, then strip arrays
+        const size_t xres_off = extra_off, yres_off = extra_off + 8;
+        const size_t offs_off = extra_off + 16;
+        const size_t cnts_off = offs_off + 4 * (size_t)n_strips;
+        std::vector<uint8_t> b(cnts_off + 4 * (size_t)n_strips, 0);
+        b[0] = 'I';
+        b[1] = 'I';
+        put16(b, 2, 42);
+        put32(b, 4, (uint32_t)ifd_off);
+        for (uint32_t r = 0; r < height; ++r) { // bottom-up rows (write_tif.hpp:317-321)
+            const uint16_t* src = &img[(size_t)(height - r - 1) * width];
+            for (uint32_t j = 0; j < width; ++j) put16(b, pix_off + r * row_bytes + 2 * (size_t)j, src[j]);
+        }
+        for (uint32_t s = 0; s < n_strips; ++s) {
+            const uint32_t rows = std::min(rps, height - s * rps);
+            put32(b, offs_off + 4 * (size_t)s, (uint32_t)(pix_off + (size_t)s * rps * row_bytes));
+            put32(b, cnts_off + 4 * (size_t)s, (uint32_t)(rows * row_bytes));
+        }
+        uint32_t xn, xd, yn, yd;
+        rational(x_resolution, xn, xd);
+        rational(y_resolution, yn, yd);
+        put32(b, xres_off, xn);
+        put32(b, xres_off + 4, xd);
+        put32(b, yres_off, yn);
+        put32(b, yres_off + 4, yd);
+        size_t p = ifd_off;
+        put16(b, p, (uint16_t)n_tags);
+        p += 2;
+        auto tag = [&](uint16_t id, uint16_t type, uint32_t count, uint32_t value) {
+            put16(b, p, id);
+            put16(b, p + 2, type);
+            put32(b, p + 4, count);
+            if (type == 3 && count == 1)
+                put16(b, p + 8, (uint16_t)value);
+            else
+                put32(b, p + 8, value);
+            p += 12;
+        };
+        const uint16_t SHORT = 3, LONG = 4, RATIONAL = 5;
+        tag(256, LONG, 1, width);
+        tag(257, LONG, 1, height);
+        tag(258, SHORT, 1, 16);
+        tag(259, SHORT, 1, 1);        // no compression
+        tag(262, SHORT, 1, 1);        // black is zero
+        tag(273, LONG, n_strips, n_strips == 1 ? (uint32_t)pix_off : (uint32_t)offs_off);
+        tag(274, SHORT, 1, 1);        // top-left origin
+        tag(277, SHORT, 1, 1);
+        tag(278, LONG, 1, rps);
+        tag(279, LONG, n_strips, n_strips == 1 ? (uint32_t)pix_bytes : (uint32_t)cnts_off);
+        tag(282, RATIONAL, 1, (uint32_t)xres_off);
+        tag(283, RATIONAL, 1, (uint32_t)yres_off);
+        tag(284, SHORT, 1, 1);        // contiguous planes
+        tag(296, SHORT, 1, resolution_unit);
+        put32(b, p, 0); // no further IFD
+        FILE* fo = std::fopen(path, "wb");
+        if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Output file could not be opened: ") + path);
+        const size_t w = std::fwrite(b.data(), 1, b.size(), fo);
+        const int c = std::fclose(fo);
+        if (w != b.size() || c != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
+        return MSF_OK;
     }
-    // baseline little-endian TIFF: header, pixel strips, IFD, rationals.
-    // RowsPerStrip is the image width, the value the reference requests
-    // from TIFFDefaultStripSize (write_tif.hpp:298-299).
-    const uint32_t rps = width;
-    const uint32_t n_strips = (height + rps - 1) / rps;
-    const size_t row_bytes = (size_t)width * 2;
-    const size_t pix_off = 8;
-    const size_t pix_bytes = row_bytes * height;
-    size_t ifd_off = pix_off + pix_bytes;
-    ifd_off += ifd_off & 1;
-    const int n_tags = 14;
-    const size_t ifd_bytes = 2 + 12 * (size_t)n_tags + 4;
-    const size_t extra_off = ifd_off + ifd_bytes;  // rationals, then strip arrays
-    const size_t xres_off = extra_off, yres_off = extra_off + 8;
-    const size_t offs_off = extra_off + 16;
-    const size_t cnts_off = offs_off + 4 * (size_t)n_strips;
-    std::vector<uint8_t> b(cnts_off + 4 * (size_t)n_strips, 0);
-    b[0] = 'I';
-    b[1] = 'I';
-    put16(b, 2, 42);
-    put32(b, 4, (uint32_t)ifd_off);
-    for (uint32_t r = 0; r < height; ++r) { // bottom-up rows (write_tif.hpp:317-321)
-        const uint16_t* src = &img[(size_t)(height - r - 1) * width];
-        for (uint32_t j = 0; j < width; ++j) put16(b, pix_off + r * row_bytes + 2 * (size_t)j, src[j]);
-    }
-    for (uint32_t s = 0; s < n_strips; ++s) {
-        const uint32_t rows = std::min(rps, height - s * rps);
-        put32(b, offs_off + 4 * (size_t)s, (uint32_t)(pix_off + (size_t)s * rps * row_bytes));
-        put32(b, cnts_off + 4 * (size_t)s, (uint32_t)(rows * row_bytes));
-    }
-    uint32_t xn, xd, yn, yd;
-    rational(x_resolution, xn, xd);
-    rational(y_resolution, yn, yd);
-    put32(b, xres_off, xn);
-    put32(b, xres_off + 4, xd);
-    put32(b, yres_off, yn);
-    put32(b, yres_off + 4, yd);
-    size_t p = ifd_off;
-    put16(b, p, (uint16_t)n_tags);
-    p += 2;
-    auto tag = [&](uint16_t id, uint16_t type, uint32_t count, uint32_t value) {
-        put16(b, p, id);
-        put16(b, p + 2, type);
-        put32(b, p + 4, count);
-        if (type == 3 && count == 1)
-            put16(b, p + 8, (uint16_t)value);
-        else
-            put32(b, p + 8, value);
-        p += 12;
-    };
-    const uint16_t SHORT = 3, LONG = 4, RATIONAL = 5;
-    tag(256, LONG, 1, width);
-    tag(257, LONG, 1, height);
-    tag(258, SHORT, 1, 16);
-    tag(259, SHORT, 1, 1);        // no compression
-    tag(262, SHORT, 1, 1);        // black is zero
-    tag(273, LONG, n_strips, n_strips == 1 ? (uint32_t)pix_off : (uint32_t)offs_off);
-    tag(274, SHORT, 1, 1);        // top-left origin
-    tag(277, SHORT, 1, 1);
-    tag(278, LONG, 1, rps);
-    tag(279, LONG, n_strips, n_strips == 1 ? (uint32_t)pix_bytes : (uint32_t)cnts_off);
-    tag(282, RATIONAL, 1, (uint32_t)xres_off);
-    tag(283, RATIONAL, 1, (uint32_t)yres_off);
-    tag(284, SHORT, 1, 1);        // contiguous planes
-    tag(296, SHORT, 1, resolution_unit);
-    put32(b, p, 0); // no further IFD
-    FILE* fo = std::fopen(path, "wb");
-    if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Output file could not be opened: ") + path);
-    const size_t w = std::fwrite(b.data(), 1, b.size(), fo);
-    const int c = std::fclose(fo);
-    if (w != b.size() || c != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
-    return MSF_OK;
+    MSF_IO_CATCH_ALL
 }
 
 int msf_write_variance_txt(const char* path, const double* values, const double* lower_edges,
                            uint32_t n_bins)
 {
-    if (!path || (n_bins && (!values || !lower_edges))) return io_fail(MSF_ERR_INVALID, "NULL argument");
-    FILE* fo = std::fopen(path, "w");
-    if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Unable to write variance to file: ") + path);
-    for (uint32_t i = 0; i < n_bins; ++i) std::fprintf(fo, "%.10e %.10e\n", lower_edges[i], values[i]);
-    if (std::fclose(fo) != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
-    return MSF_OK;
+    try {
+        if (!path || (n_bins && (!values || !lower_edges))) return io_fail(MSF_ERR_INVALID, "NULL argument");
+        FILE* fo = std::fopen(path, "w");
+        if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Unable to write variance to file: ") + path);
+        for (uint32_t i = 0; i < n_bins; ++i) std::fprintf(fo, "%.10e %.10e\n", lower_edges[i], values[i]);
+        if (std::fclose(fo) != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
+        return MSF_OK;
+    }
+    MSF_IO_CATCH_ALL
 }
 
 int msf_write_correlograph_txt(const char* path, const double* data, const double* k_edges,
                                uint32_t n_k, const double* phi_edges, uint32_t n_phi)
 {
-    if (!path || (n_k && !k_edges) || (n_phi && !phi_edges) || (n_k && n_phi && !data))
-        return io_fail(MSF_ERR_INVALID, "NULL argument");
-    FILE* fo = std::fopen(path, "w");
-    if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Unable to write variance to file: ") + path);
-    std::fputs("0 ", fo); // the (0,0) corner, not data
-    for (uint32_t jj = 0; jj < n_phi; ++jj) std::fprintf(fo, "%.10e ", phi_edges[jj]);
-    std::fputc('\n', fo);
-    for (uint32_t ii = 0; ii < n_k; ++ii) {
-        std::fprintf(fo, "%.10e ", k_edges[ii]);
-        for (uint32_t jj = 0; jj < n_phi; ++jj)
-            std::fprintf(fo, "%.10e ", data[(size_t)ii + (size_t)jj * n_k]);
+    try {
+        if (!path || (n_k && !k_edges) || (n_phi && !phi_edges) || (n_k && n_phi && !data))
+            return io_fail(MSF_ERR_INVALID, "NULL argument");
+        FILE* fo = std::fopen(path, "w");
+        if (!fo) return io_fail(MSF_ERR_INVALID, std::string("Unable to write variance to file: ") + path);
+        std::fputs("0 ", fo); // the (0,0) corner, not data
+        for (uint32_t jj = 0; jj < n_phi; ++jj) std::fprintf(fo, "%.10e ", phi_edges[jj]);
         std::fputc('\n', fo);
+        for (uint32_t ii = 0; ii < n_k; ++ii) {
+            std::fprintf(fo, "%.10e ", k_edges[ii]);
+            for (uint32_t jj = 0; jj < n_phi; ++jj)
+                std::fprintf(fo, "%.10e ", data[(size_t)ii + (size_t)jj * n_k]);
+            std::fputc('\n', fo);
+        }
+        if (std::fclose(fo) != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
+        return MSF_OK;
     }
-    if (std::fclose(fo) != 0) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
-    return MSF_OK;
+    MSF_IO_CATCH_ALL
 }
 
 int msf_write_f64(const char* path, const double* data, uint32_t rows, uint32_t cols)
 {
-    if (!path || (rows && cols && !data)) return io_fail(MSF_ERR_INVALID, "NULL argument");
-    FILE* fo = std::fopen(path, "wb");
-    if (!fo) return io_fail(MSF_ERR_INVALID, std::string("cannot open ") + path);
-    std::fprintf(fo, "msf-f64 %u %u\n", rows, cols);
-    const size_t n = (size_t)rows * cols;
-    const size_t w = n ? std::fwrite(data, sizeof(double), n, fo) : 0;
-    if (std::fclose(fo) != 0 || w != n) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
-    return MSF_OK;
+    try {
+        if (!path || (rows && cols && !data)) return io_fail(MSF_ERR_INVALID, "NULL argument");
+        FILE* fo = std::fopen(path, "wb");
+        if (!fo) return io_fail(MSF_ERR_INVALID, std::string("cannot open ") + path);
+        std::fprintf(fo, "msf-f64 %u %u\n", rows, cols);
+        const size_t n = (size_t)rows * cols;
+        const size_t w = n ? std::fwrite(data, sizeof(double), n, fo) : 0;
+        if (std::fclose(fo) != 0 || w != n) return io_fail(MSF_ERR_INVALID, std::string("short write: ") + path);
+        return MSF_OK;
+    }
+    MSF_IO_CATCH_ALL
 }
 
 } // extern "C"

@@ -147,6 +147,29 @@ def test_lammps_reader_rejects(tmp_path, old, new):
         msf.read_model(_write(tmp_path, "bad.data", LAMMPS.replace(old, new, 1)))
 
 
+@pytest.mark.parametrize("text", [
+    "-1\n10 10 10\nSi 1 2 3\n",              # wraps to 2^64 - 1 as a size_t
+    "18446744073709551615\n10 10 10\nSi 1 2 3\n",
+    "99999999999999999999\n10 10 10\nSi 1 2 3\n",  # beyond size_t: the extraction fails
+])
+def test_xyz_reader_malformed_count_is_an_error_not_an_abort(tmp_path, text):
+    # the reference reads atom lines until they run out (tem_io.cpp:4747-4815);
+    # no count may reach the allocator
+    with pytest.raises(msf.MsfError):
+        msf.read_model(_write(tmp_path, "huge.xyz", text))
+
+
+@pytest.mark.parametrize("old, new", [
+    ("4 atoms", "-1 atoms"),
+    ("2 atom types", "-1 atom types"),
+    ("2 atom types", "18446744073709551615 atom types"),
+])
+def test_lammps_reader_malformed_counts_are_errors(tmp_path, old, new):
+    assert old in LAMMPS
+    with pytest.raises(msf.MsfError):
+        msf.read_model(_write(tmp_path, "huge.data", LAMMPS.replace(old, new, 1)))
+
+
 # ------------------------------------------------------------ TIFF
 def _read_tiff16(path):
     b = open(path, "rb").read()
