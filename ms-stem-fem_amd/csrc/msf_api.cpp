@@ -243,12 +243,16 @@ int fail(msf_ctx* c, int code, const std::string& m)
 // device work of a context is serialised (direct calls and the serving worker)
 #define MSF_LOCK(c) std::lock_guard<std::recursive_mutex> msf_lock_((c)->mu)
 
+// A failed call is reported once: the runtime's last-error state is cleared,
+// so the next launch's hipGetLastError() check reports only its own status.
 #define HIPCHK(ctx, expr)                                                                     \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
+        if (e_ != hipSuccess) {                                                               \
+            (void)hipGetLastError();                                                          \
             return fail((ctx), e_ == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE,    \
                         std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+        }                                                                                     \
     } while (0)
 
 template <typename T> hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s)
@@ -1451,6 +1455,7 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
         }
         int rc = MSF_OK;
         auto bail = [&](int code) {
+            (void)hipGetLastError(); // the failure is reported here, not by a later launch
             delete sp;
             return code;
         };
@@ -1485,7 +1490,10 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
         if (!e && !atoms.empty())
             e = hipMemcpyAsync(datoms.p, atoms.data(), atoms.size() * sizeof(int4), hipMemcpyHostToDevice,
                                c->stream);
-        if (e) return bail(fail(c, e == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE, hipGetErrorString(e)));
+        if (e) {
+            (void)hipGetLastError();
+            return bail(fail(c, e == hipErrorOutOfMemory ? MSF_ERR_MEMORY : MSF_ERR_DEVICE, hipGetErrorString(e)));
+        }
         const double lgab = d.lambda * d.gamma * (1 / (grid->xperiod * grid->yperiod));
         size_t a0 = 0;
         for (int s0 = 0; s0 < S && !e; s0 += K) {
@@ -1519,7 +1527,10 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
             a0 = a1;
         }
         if (!e) e = hipStreamSynchronize(c->stream);
-        if (e) return bail(fail(c, MSF_ERR_DEVICE, std::string("transmission build: ") + hipGetErrorString(e)));
+        if (e) {
+            (void)hipGetLastError();
+            return bail(fail(c, MSF_ERR_DEVICE, std::string("transmission build: ") + hipGetErrorString(e)));
+        }
         c->specimens.insert(sp);
         *out = sp;
         return MSF_OK;

@@ -383,3 +383,43 @@ def test_pixel_groups_kept_across_specimens_on_one_grid(msf, ctx):
         np.testing.assert_array_equal(img, img0)
         for m in msf.FEM_MODES:
             np.testing.assert_array_equal(v[m], v0[m])
+
+
+def test_failed_run_after_a_group_rebuild_leaves_the_cache_consistent(msf, ctx):
+    """A run that rebuilds the pixel groups for detector B and then fails
+    (here: its batch buffer cannot be allocated) must not let a later run
+    with detector A reuse B's tables: that run equals a fresh context's bit
+    for bit."""
+    q, Z = random_atoms(60, (20.0, 20.0, 6.0), seed=41, species=(14, 29))
+    a = Case(nx=256, ny=256, cell=(20.0, 20.0, 6.0), voltage=300e3, q=q, Z=Z, min_slice=1.5,
+             alpha_max=0.02, inner=0.03, outer=0.2, fem=True, dr=0.7)
+    b = Case(**{**a.__dict__, "inner": 0.05, "dr": 0.9})
+    xp = np.linspace(0.0, 19.0, 4)
+    yp = np.linspace(0.5, 18.5, 3)
+    spec = a.gpu_specimen(msf, ctx)
+    try:
+        ctx.adfstem(spec, a.optics(msf), xp, yp, detector=a.detector(msf), fem=a.femcfg(msf))
+        # 1000 x 1000 positions in one batch: ~360 GB of wave buffer at 256^2
+        big = np.linspace(0.0, 19.0, 1000)
+        ctx.configure(batch=1000 * 1000)
+        print("group rebuild for detector B, then an allocation failure", flush=True)
+        with pytest.raises(msf.MsfError) as e:
+            ctx.adfstem(spec, b.optics(msf), big, big, detector=b.detector(msf),
+                        fem=b.femcfg(msf))
+        assert e.value.code == -4                      # MSF_ERR_MEMORY
+        ctx.configure()
+        img, v = ctx.adfstem(spec, a.optics(msf), xp, yp, detector=a.detector(msf),
+                             fem=a.femcfg(msf))
+    finally:
+        ctx.configure()
+        spec.free()
+    fresh = msf.Context(0)
+    try:
+        s0 = a.gpu_specimen(msf, fresh)
+        img0, v0 = fresh.adfstem(s0, a.optics(msf), xp, yp, detector=a.detector(msf),
+                                 fem=a.femcfg(msf))
+        s0.free()
+    finally:
+        fresh.close()
+    np.testing.assert_array_equal(img, img0)
+    np.testing.assert_array_equal(v, v0)
