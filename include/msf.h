@@ -105,10 +105,18 @@ typedef struct msf_config {
                                           results bit-identical); 0 = automatic (two chains on
                                           the streams for launches of >= 4M pixels, else one
                                           chain as a graph) */
-    int32_t autotune;                  /* 1: when batch, col_tile and chains are all automatic,
-                                          time the candidates on the job's own specimen once per
-                                          shape (nx, ny, kept columns, positions up to 256) and
-                                          keep the fastest; 0 (default): the fixed rules */
+    int32_t autotune;                  /* with batch, col_tile and chains all automatic, the
+                                          launch shape (positions per batch, column tile, one or
+                                          two chains) comes from the process-wide tuning table
+                                          (msf_tuning_*), measured on first use by timing whole
+                                          batch steps of the job itself per device type and
+                                          shape (nx, ny, kept columns, positions up to 256,
+                                          outputs): 0 (default) for runs of at least 16384
+                                          probe-slices, results bit-identical to the fixed
+                                          rules; 1 for every run of two or more positions, and
+                                          with fft_plan 0 the radix plans too (results then
+                                          differ from the rules' by FP32 rounding); -1 the
+                                          fixed rules */
     int32_t fft_plan;                  /* radix plans of the slice passes: 1 primary (every
                                           length), 2 the alternate plans with 16 values per lane
                                           and one fewer stage on lengths 256, 512 and 1024 (the
@@ -146,6 +154,23 @@ int msf_set_config(msf_ctx* ctx, const msf_config* cfg);
 int msf_get_stats(const msf_ctx* ctx, msf_stats* out);
 /* 1 if transforms of length n are compiled in, else 0 */
 int msf_supported_length(uint32_t n);
+
+/* No reference counterpart (measurement aid): the rate of a streaming
+ * device copy of `bytes` (rounded down to 16 KiB) with the library's own
+ * copy kernel, read + write bytes per second in GB/s, the best of `reps`
+ * event-timed launches after one warm-up.  The achievable HBM rate the
+ * slice passes are compared with. */
+int msf_copy_rate(msf_ctx* ctx, uint64_t bytes, uint32_t reps, double* gbs);
+
+/* The tuning table (no direct reference counterpart; the reference keeps
+ * FFTW wisdom in a file per grid, tem.cpp:588-616 and :1749-1765): the
+ * autotuner's choices, process-wide, written to / read from a text file.
+ * msf_tuning_load merges a file's entries (MSF_ERR_INVALID if it cannot be
+ * read or a line is malformed; entries may be NULL); msf_tuning_clear forgets
+ * every choice. */
+int msf_tuning_save(const char* path);
+int msf_tuning_load(const char* path, uint32_t* entries);
+int msf_tuning_clear(void);
 
 /* ------------------------------------------------------------ scalars */
 /* tem.cpp:424-438 */
@@ -351,6 +376,31 @@ typedef struct msf_job {
  * depend on what runs beside it.  Direct calls on the context do not wait
  * for served jobs.  msf_destroy finishes the queue before it returns. */
 int msf_submit(msf_ctx* ctx, const msf_job* job, uint64_t* id);
+/* A run on a specimen that stays resident on the device: the transmission
+ * functions of msf_specimen_build (replacing tem.cpp:1212-1276, done once per
+ * specimen) serve every job that names it, as the reference runs its whole
+ * raster on slices built once (adfstem.cpp:1556-2793).  Optics, raster,
+ * detector, FEM settings and outputs as in msf_job. */
+typedef struct msf_run_job {
+    const msf_specimen* specimen;      /* built on the submitting context */
+    msf_optics optics;
+    const double* x_p;
+    uint32_t n_xp;
+    const double* y_p;
+    uint32_t n_yp;
+    msf_detector detector;
+    msf_fem fem;
+    double* adf_image;                 /* n_xp*n_yp doubles when detector.enabled */
+    msf_fem_outputs fem_out;           /* optional FEM outputs */
+} msf_run_job;
+
+/* No reference counterpart.  Queues a run on a resident specimen on the
+ * same serving engine as msf_submit (same admission, slots and results as
+ * msf_run_stem_fem on that specimen).  The specimen's stacks are counted
+ * against the memory limit once, while any queued or running job refers to
+ * them, instead of per job; msf_specimen_free waits for those jobs. */
+int msf_submit_run(msf_ctx* ctx, const msf_run_job* job, uint64_t* id);
+
 /* MSF_OK or the job's error code once finished, MSF_PENDING while queued or
  * running, MSF_ERR_INVALID for an unknown id */
 int msf_poll(msf_ctx* ctx, uint64_t id);
@@ -384,7 +434,10 @@ typedef struct msf_serve_stats {
     uint64_t jobs_done;
     uint64_t mem_limit;
     uint64_t admitted_bytes;           /* footprints of the running jobs */
-    uint64_t peak_admitted_bytes;
+    uint64_t peak_admitted_bytes;      /* most footprint + resident bytes claimed at once */
+    uint64_t specimen_builds;          /* specimens built for msf_submit jobs so far */
+    uint64_t resident_bytes;           /* stacks of the resident specimens queued or running
+                                          msf_submit_run jobs refer to (counted once each) */
 } msf_serve_stats;
 int msf_serve_get_stats(msf_ctx* ctx, msf_serve_stats* out);
 

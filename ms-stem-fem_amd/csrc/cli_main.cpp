@@ -562,6 +562,12 @@ int main(int argc, char** argv)
         std::cerr << "Error, no usable GPU" << std::endl;
         return EXIT_FAILURE;
     }
+    // the tuning file of this grid, read before the run and written after it,
+    // as the reference reads and writes its FFTW wisdom file per grid
+    // (tem.cpp:588-616, :1749-1765); a missing file is not an error
+    const std::string tuning_file =
+        "msf_tuning_" + std::to_string(p.Nx) + "x" + std::to_string(p.Ny) + ".txt";
+    (void)msf_tuning_load(tuning_file.c_str(), nullptr);
     msf_specimen* spec = nullptr;
     if (msf_specimen_build(c, &grid, q.data(), Z.data(), (uint32_t)nd, p.minslice, &spec) != MSF_OK)
         return die("specimen", c);
@@ -723,5 +729,7 @@ int main(int argc, char** argv)
     }
     msf_specimen_free(spec);
     msf_destroy(c);
+    if (msf_tuning_save(tuning_file.c_str()) != MSF_OK)
+        std::cerr << "Could not write tuning file: " << tuning_file << std::endl;
     return rc;
 }

@@ -42,7 +42,7 @@ struct ColPassArgs {
     float2* psi;            // [B][nx][ncp]
     const float2* px;       // [nx] propagator along x of this slice
     const float2* py;       // [ny] propagator along y of this slice, scaled by 1/s
-    const int2* keep;       // [nx] (a_i, b_i) kept-column ranges of row i
+    const int2* colkeep;    // [ncp] (c, d): kept rows [0, c) U [nx - d, nx) of compact column jc
     const float2* tw;       // stage twiddle table of length nx (stage_twiddles)
     const float2* psi0;     // [nx][ny] vacuum exit wave in k-space (last pass only)
     const int2* shift;      // [B] probe shifts (last pass only)
@@ -64,6 +64,9 @@ bool has_alt_plan(int n);
 int col_tile(int nx);
 // the column-pass tile width used when none is configured, for plan v
 int col_tile_default(int nx, int v = 0);
+// the tile used when none is configured: the default, or one column per
+// workgroup when the launch would not give every CU a workgroup
+int col_tile_auto(int nx, int v, int batch, int ncp);
 // true when the kept set leaves [3n/8, 5n/8) empty along the row (ka, kb) and
 // every row i in [3nx/8, 5nx/8) fully masked (keep ranges)
 bool row_prunable(int ny, int ka, int kb);
@@ -141,6 +144,8 @@ hipError_t launch_norm2(const double2* a, int64_t n, double* out, hipStream_t s)
 hipError_t launch_scale_to_f32(const double2* a, int64_t n, const double* norm2,
                                double numerator, float2* out, hipStream_t s);
 hipError_t launch_fill_f64(double2* a, int64_t n, hipStream_t s);
+// out[0, n16) = in[0, n16), 16-byte vectors; n16 a multiple of 1024
+hipError_t launch_stream_copy(const float4* in, float4* out, int64_t n16, hipStream_t s);
 // atom count maps: maps[(slot*nsp + sp)][sx][sy] += 1
 hipError_t launch_scatter_atoms(double2* maps, const int4* atoms, int n_atoms, int nx, int ny,
                                 int nsp, hipStream_t s);

@@ -211,6 +211,29 @@ hipError_t launch_fill_f64(double2* a, int64_t n, hipStream_t s)
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ measurement
+// A streaming copy at the rate HBM sustains for a plain read + write: each
+// lane moves four 16-byte vectors, the four loads issued before the stores.
+// n16 is a multiple of 4 * 256 (the caller sizes the buffers so).
+__global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restrict__ in,
+                                                           float4* __restrict__ out)
+{
+    const int64_t base = (int64_t)blockIdx.x * (4 * 256) + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = in[base + k * 256];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[base + k * 256] = v[k];
+}
+
+hipError_t launch_stream_copy(const float4* in, float4* out, int64_t n16, hipStream_t s)
+{
+    if (n16 <= 0 || n16 % (4 * 256)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)(n16 / (4 * 256))), dim3(256), 0, s, in,
+                       out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ transmission
 // atoms[k] = (slot, species, sx, sy); counts are small integers, so the FP64
 // atomic sums are exact and order-independent

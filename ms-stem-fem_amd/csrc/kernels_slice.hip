@@ -10,6 +10,7 @@
 //                 (slice.cpp:1366-1421) + the next slice's BACKWARD FFT (x half).
 //   reductions  = sum_detected_intensity (func.cpp:2593-2641) and
 //                 integrate_out_phi_fftw (variance.cpp:64-300), FP64.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -99,15 +100,19 @@ template <int NY, int V> __host__ __device__ constexpr int row_waves()
     return E <= 8 ? 6 : (E == 16 ? (NY == 512 ? 3 : 4) : (E == 12 ? 3 : 2));
 }
 
-template <int NY, int V, bool FIRST, int TMODE, bool PRUNE>
-__global__ void __launch_bounds__((row_threads<NY, V>()))
+// RT lanes per workgroup: row_threads() for batches that fill it, or one
+// row's T lanes when the batch has fewer positions than a workgroup has rows
+// (no lane then computes a row it does not store).
+template <int NY, int V, bool FIRST, int TMODE, bool PRUNE, int RT>
+__global__ void __launch_bounds__(RT)
 __attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_pass_kernel(RowPassArgs a)
 {
     using P = PlanV<NY, V>;
     constexpr int E = P::E, T = P::T;
-    constexpr int RPW = row_threads<NY, V>() / T;
+    constexpr int RPW = RT / T;
     constexpr int LEN = lds_len(NY);
-    constexpr bool LEAN = row_lean<NY, V>();
+    // one row per workgroup: nothing to share, tables are read from L2
+    constexpr bool LEAN = row_lean<NY, V>() || RPW == 1;
     __shared__ float2 s_tw[LEAN ? 1 : plan_tw_len<P>()];
     __shared__ float2 s_t[LEAN ? 1 : NY];
     __shared__ float2 s_buf[RPW * LEN];
@@ -232,15 +237,16 @@ template <int NX, int W> __host__ __device__ constexpr int col_len()
     return base + ((want - base % 16) + 16) % 16;
 }
 
-// Lean tiles: when the twiddle, propagator and keep tables would keep two
+// Lean tiles: when the twiddle and propagator tables would keep two
 // workgroups from sharing a CU's 160 KB of LDS (1024 rows: a 74 KB tile),
 // the tables are read from global memory (L2-resident, shared by every
 // workgroup) and only the tile stays in LDS.
 template <int NX, int V, int W> __host__ __device__ constexpr bool col_lean()
 {
+    if (W == 1) return true;      // one column: nothing to share, tables are read from L2
     if (W == 4) return NX >= 512; // small tiles: tables would outweigh the tile
     return 2 * ((size_t)W * col_len<NX, W>() * 8 + (size_t)plan_tw_len<PlanV<NX, V>>() * 8 +
-                (size_t)NX * 16) > 160 * 1024;
+                (size_t)NX * 8) > 160 * 1024;
 }
 
 // Plans with 12 or more values per lane (the alternate plans, and the
@@ -267,20 +273,17 @@ __global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPa
     __shared__ float2 s_tw[LEAN ? 1 : plan_tw_len<P>()];
     __shared__ float2 s_nat[LAST ? NX : 1]; // roots W_nx^e for the exit phase ramp
     __shared__ float2 s_px[LEAN ? 1 : NX];
-    __shared__ int2 s_keep[LEAN ? 1 : NX];
     __shared__ float2 s_buf[W * LEN];
 
     for (int e = threadIdx.x; e < NX; e += blockDim.x) {
         if constexpr (!LEAN) {
             if (e < plan_tw_len<P>()) s_tw[e] = a.tw[e];
             s_px[e] = a.px[e];
-            s_keep[e] = a.keep[e];
         }
         if constexpr (LAST) s_nat[e] = a.twx_nat[e];
     }
     const float2* tw = LEAN ? a.tw : s_tw;
     const float2* tpx = LEAN ? a.px : s_px;
-    const int2* tkeep = LEAN ? a.keep : s_keep;
 
     // neighbouring tiles share 128-byte lines (W = 8 reads 64 B per row): keep
     // them on one XCD, next to each other in dispatch order
@@ -323,6 +326,9 @@ __global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPa
     // band limit (bw_limit + propagate's k^2 test, decided on the host) and
     // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
     const float2 pyj = active ? a.py[j] : float2{0.f, 0.f};
+    // kept rows of this column, decided on the host (one pair per column
+    // instead of a range table per row)
+    const int2 ck = active ? a.colkeep[jc] : int2{0, 0};
 #pragma unroll
     for (int p = 0; p < E; ++p) {
         if (PRUNE && in_zero_band<NX, E>(p)) {
@@ -330,8 +336,7 @@ __global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPa
             continue;
         }
         const int i = t + T * p;
-        const int2 kr = tkeep[i];
-        const bool keep = (j < kr.x) || (j >= a.ny - kr.y);
+        const bool keep = (i < ck.x) || (i >= NX - ck.y);
         const float2 pxi = tpx[i];
         const float2 pr = float2{pxi.x * pyj.x - pxi.y * pyj.y, pxi.x * pyj.y + pxi.y * pyj.x};
         const float2 f = x[p];
@@ -469,6 +474,16 @@ int col_tile(int nx)
     }
 }
 
+// Launches of so few positions that the default tiles would leave CUs idle
+// (one position at 512^2: 44 workgroups of 8 columns for 256 CUs) take one
+// column per workgroup instead.
+int col_tile_auto(int nx, int v, int batch, int ncp)
+{
+    const int w = col_tile_default(nx, v);
+    constexpr int64_t kCUs = 256;
+    return (int64_t)batch * (ncp / std::max(w, 1)) < kCUs ? 1 : w;
+}
+
 int col_tile_default(int nx, int v)
 {
     switch (nx) {
@@ -483,19 +498,19 @@ int col_tile_default(int nx, int v)
 
 // With ev0/ev1 the dispatch packet itself stamps the kernel's start and end
 // (hipExtLaunchKernelGGL), so profiled durations carry no marker overhead.
-template <int NY, int V>
-static hipError_t row_pass_nv(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
+template <int NY, int V, int RT>
+static hipError_t row_pass_rt(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
                               hipEvent_t ev0, hipEvent_t ev1)
 {
-    constexpr int RPW = row_threads<NY, V>() / PlanV<NY, V>::T;
+    constexpr int RPW = RT / PlanV<NY, V>::T;
     const int64_t groups = (a.batch + RPW - 1) / RPW;
     const dim3 grid((unsigned)(groups * a.nx));
-    const dim3 block(RPW * PlanV<NY, V>::T);
+    const dim3 block(RT);
 #define MSF_ROW(F, M, PR)                                                                          \
     do {                                                                                           \
-        if (ev0) hipExtLaunchKernelGGL((row_pass_kernel<NY, V, F, M, PR>), grid, block, 0, s,      \
+        if (ev0) hipExtLaunchKernelGGL((row_pass_kernel<NY, V, F, M, PR, RT>), grid, block, 0, s,  \
                                        ev0, ev1, 0, a);                                            \
-        else hipLaunchKernelGGL((row_pass_kernel<NY, V, F, M, PR>), grid, block, 0, s, a);        \
+        else hipLaunchKernelGGL((row_pass_kernel<NY, V, F, M, PR, RT>), grid, block, 0, s, a);    \
     } while (0)
     if (a.prune) {
         if (first) { if (tmode == 0) MSF_ROW(true, 0, true); else MSF_ROW(true, 1, true); }
@@ -506,6 +521,19 @@ static hipError_t row_pass_nv(bool first, int tmode, const RowPassArgs& a, hipSt
     }
 #undef MSF_ROW
     return hipGetLastError();
+}
+
+// rows per workgroup follow the batch: a batch smaller than a full
+// workgroup's rows launches one row per workgroup
+template <int NY, int V>
+static hipError_t row_pass_nv(bool first, int tmode, const RowPassArgs& a, hipStream_t s,
+                              hipEvent_t ev0, hipEvent_t ev1)
+{
+    constexpr int RT = row_threads<NY, V>(), T = PlanV<NY, V>::T;
+    if constexpr (RT > T) {
+        if (a.batch < RT / T) return row_pass_rt<NY, V, T>(first, tmode, a, s, ev0, ev1);
+    }
+    return row_pass_rt<NY, V, RT>(first, tmode, a, s, ev0, ev1);
 }
 
 template <int NY>
@@ -560,7 +588,7 @@ template <int NX, int V, int W> constexpr bool col_fits()
 {
     using P = PlanV<NX, V>;
     const size_t tables = col_lean<NX, V, W>() ? (size_t)NX * 8  // s_nat of the last slice
-                                                : (size_t)NX * 24 + (size_t)plan_tw_len<P>() * 8;
+                                                : (size_t)NX * 16 + (size_t)plan_tw_len<P>() * 8;
     return (size_t)W * col_len<NX, W>() * 8 + tables <= 160 * 1024 && W * P::T <= 1024;
 }
 
@@ -568,8 +596,9 @@ template <int NX, int V>
 static hipError_t col_pass_nv(bool last, const ColPassArgs& a, hipStream_t s, hipEvent_t ev0,
                               hipEvent_t ev1)
 {
-    int w = ColTile<NX, V>::W;
+    int w = col_tile_auto(NX, V, a.batch, a.ncp);
     if (a.tile == 4 || a.tile == 8 || a.tile == 16) w = a.tile;
+    if (w == 1) return col_pass_w<NX, V, 1>(last, a, s, ev0, ev1);
     if constexpr (col_fits<NX, V, 16>()) {
         if (w == 16) return col_pass_w<NX, V, 16>(last, a, s, ev0, ev1);
     }

@@ -81,6 +81,29 @@ RowRanges slice_keep_ranges(const Domain& d)
     });
 }
 
+RowRanges column_keep_ranges(const RowRanges& rows, int nx, int ny, int ncp)
+{
+    const int ka = rows.amax, kb = rows.bmax, nc = ka + kb;
+    auto kept = [&](int i, int j) { return j < rows.ab[2 * i] || j >= ny - rows.ab[2 * i + 1]; };
+    RowRanges r;
+    r.ab.assign(2 * (size_t)ncp, 0);
+    for (int jc = 0; jc < ncp && jc < nc; ++jc) {
+        const int j = jc < ka ? jc : jc - nc + ny;
+        int c = 0;
+        while (c < nx && kept(c, j)) ++c;
+        int d = 0;
+        if (c < nx)
+            while (d < nx - c && kept(nx - 1 - d, j)) ++d;
+        for (int i = c; i < nx - d; ++i)
+            if (kept(i, j)) r.ok = false;
+        r.ab[2 * jc] = c;
+        r.ab[2 * jc + 1] = d;
+        r.amax = std::max(r.amax, c);
+        r.bmax = std::max(r.bmax, d);
+    }
+    return r;
+}
+
 RowRanges bwlimit_keep_ranges(const Domain& d)
 {
     const double c_bw = ref_pow2(d.bw_t);

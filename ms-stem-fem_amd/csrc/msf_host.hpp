@@ -68,6 +68,13 @@ template <typename Pred> RowRanges row_ranges(int nx, int ny, Pred keep)
 // bw_limit.cpp:30-47 together with slice::propagate's test (slice.cpp:1388-1389):
 // the pixels a slice step keeps.
 RowRanges slice_keep_ranges(const Domain& d);
+// The same kept set by column of the compact plane [nx][ncp] (compact
+// column jc holds full column j = jc < ka ? jc : jc - (ka + kb) + ny): the
+// kept rows of column jc are [0, c) U [nx - d, nx), returned as (c, d) pairs
+// (2*ncp entries; padding columns keep nothing).  ok == false if a column is
+// not of that form; the pairs then are not used.
+RowRanges column_keep_ranges(const RowRanges& rows, int nx, int ny, int ncp);
+
 // bw_limit.cpp:30-47 alone (transmission band limit, slice.cpp:1731-1737)
 RowRanges bwlimit_keep_ranges(const Domain& d);
 // scatterer_pap.hpp:108-109: template support |k|^2 < bwcutoff_pap^2

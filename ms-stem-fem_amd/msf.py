@@ -103,6 +103,13 @@ class Job(C.Structure):
                 ("fem_out", FemOutputs)]
 
 
+class RunJob(C.Structure):
+    _fields_ = [("specimen", C.c_void_p), ("optics", Optics), ("x_p", C.c_void_p),
+                ("n_xp", C.c_uint32), ("y_p", C.c_void_p), ("n_yp", C.c_uint32),
+                ("detector", Detector), ("fem", Fem), ("adf_image", C.c_void_p),
+                ("fem_out", FemOutputs)]
+
+
 class Ticket:
     """A submitted job: its id and the output arrays the worker fills."""
 
@@ -118,7 +125,8 @@ class JobInfo(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("running", C.c_uint32), ("max_running", C.c_uint32),
                 ("queued", C.c_uint64), ("jobs_done", C.c_uint64), ("mem_limit", C.c_uint64),
-                ("admitted_bytes", C.c_uint64), ("peak_admitted_bytes", C.c_uint64)]
+                ("admitted_bytes", C.c_uint64), ("peak_admitted_bytes", C.c_uint64),
+                ("specimen_builds", C.c_uint64), ("resident_bytes", C.c_uint64)]
 
 
 class Config(C.Structure):
@@ -158,7 +166,9 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_submit", "msf_poll", "msf_wait", "msf_specimen_propagator", "msf_probe_k",
            "msf_slice_step_batch", "msf_correlograph_bins", "msf_adf_reduce",
            "msf_fem_accumulate", "msf_mtf_2d", "msf_apply_mtf", "msf_wait_info",
-           "msf_serve_configure", "msf_serve_get_stats", "msf_job_footprint"]
+           "msf_serve_configure", "msf_serve_get_stats", "msf_job_footprint",
+           "msf_copy_rate", "msf_submit_run", "msf_tuning_save",
+           "msf_tuning_load", "msf_tuning_clear"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -198,6 +208,10 @@ def lib():
                                             _dp, _dp]
         L.msf_run_stem.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(Detector),
                                    C.POINTER(Fem), _dp, C.c_uint32, _dp, C.c_uint32, _dp, _dp]
+        L.msf_copy_rate.argtypes = [vp, C.c_uint64, C.c_uint32, _dp]
+        L.msf_tuning_save.argtypes = [C.c_char_p]
+        L.msf_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
+        L.msf_tuning_clear.argtypes = []
         L.msf_exit_waves.argtypes = [vp, vp, C.POINTER(Optics), _dp, C.c_uint32, _dp,
                                      C.c_uint32, C.POINTER(C.c_float)]
         L.msf_probe_shifts.argtypes = [C.POINTER(Grid), _dp, C.c_uint32, _dp, C.c_uint32,
@@ -221,6 +235,7 @@ def lib():
         L.msf_read_mtf.argtypes = [C.c_char_p, C.POINTER(MtfTable)]
         L.msf_mtf_table_free.argtypes = [C.POINTER(MtfTable)]
         L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
+        L.msf_submit_run.argtypes = [vp, C.POINTER(RunJob), C.POINTER(C.c_uint64)]
         L.msf_poll.argtypes = [vp, C.c_uint64]
         L.msf_wait.argtypes = [vp, C.c_uint64]
         L.msf_wait_info.argtypes = [vp, C.c_uint64, C.POINTER(JobInfo)]
@@ -352,6 +367,24 @@ def write_f64(path, data):
               "msf_write_f64")
 
 
+def tuning_save(path):
+    """msf_tuning_save: the process-wide tuning table as a text file."""
+    if lib().msf_tuning_save(str(path).encode()):
+        raise MsfError(-1, f"msf_tuning_save({path})")
+
+
+def tuning_load(path):
+    """msf_tuning_load: merges a tuning file; returns its entry count."""
+    n = C.c_uint32()
+    if lib().msf_tuning_load(str(path).encode(), C.byref(n)):
+        raise MsfError(-1, f"msf_tuning_load({path})")
+    return n.value
+
+
+def tuning_clear():
+    lib().msf_tuning_clear()
+
+
 def supported_length(n):
     return bool(lib().msf_supported_length(n))
 
@@ -432,13 +465,23 @@ class Context:
             raise MsfError(rc, f"{what}: {lib().msf_last_error(self._h).decode()}")
 
     def configure(self, batch=0, transmit_mode=TRANSMIT_REFERENCE, profile_kernels=False,
-                  col_tile=0, graph=True, chains=0, autotune=False, fft_plan=0):
+                  col_tile=0, graph=True, chains=0, autotune=0, fft_plan=0):
         """fft_plan: 1 primary radix plans, 2 the alternate plans of lengths
         256, 512 and 1024 (16 values per lane, one fewer stage), 0 automatic
-        (the alternate at 256 and 1024, or the autotuner's choice)."""
+        (the alternate at 256 and 1024, or the autotuner's choice).
+        autotune: 0 (or False) automatic, for runs of >= 16384 probe-slices,
+        launch shapes only; 1 (or True) every run, radix plans too with
+        fft_plan 0; -1 the fixed rules."""
         cfg = Config(batch, transmit_mode, 1 if profile_kernels else 0, col_tile,
-                     0 if graph else 1, chains, 1 if autotune else 0, fft_plan)
+                     0 if graph else 1, chains, int(autotune), fft_plan)
         self._check(lib().msf_set_config(self._h, C.byref(cfg)), "msf_set_config")
+
+    def copy_rate(self, nbytes=1 << 30, reps=5):
+        """msf_copy_rate: GB/s (read + write) of the library's streaming copy
+        kernel over nbytes, best of reps."""
+        g = C.c_double()
+        self._check(lib().msf_copy_rate(self._h, nbytes, reps, C.byref(g)), "msf_copy_rate")
+        return g.value
 
     def stats(self):
         s = Stats()
@@ -542,6 +585,25 @@ class Context:
         t = Ticket(jid.value, img, outs, (q, Z, x_p, y_p, nb))
         t.footprint = lib().msf_job_footprint(C.byref(job))
         return t
+
+    def submit_run(self, spec, optics, x_p, y_p, detector=None, fem=None, modes=("v_omega",)):
+        """msf_submit_run: queue a run on a resident specimen of this context
+        on the serving engine; returns a Ticket for poll() / wait()."""
+        x_p = np.ascontiguousarray(x_p, np.float64)
+        y_p = np.ascontiguousarray(y_p, np.float64)
+        det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
+        fm = fem if fem is not None else Fem(0, 0.0)
+        img = np.zeros(x_p.size * y_p.size) if det.enabled else None
+        outs, nb = {}, 0
+        if fm.enabled:
+            nb = fem_bins(spec.grid, fm.dr).size
+            outs = {m: np.zeros(max(nb, 1)) for m in modes}
+        fo = FemOutputs(*[outs[m].ctypes.data if m in outs else None for m in FEM_MODES])
+        job = RunJob(spec._h, optics, x_p.ctypes.data, x_p.size, y_p.ctypes.data, y_p.size, det,
+                     fm, img.ctypes.data if img is not None else None, fo)
+        jid = C.c_uint64()
+        self._check(lib().msf_submit_run(self._h, C.byref(job), C.byref(jid)), "msf_submit_run")
+        return Ticket(jid.value, img, outs, (spec, None, x_p, y_p, nb))
 
     def poll(self, ticket):
         return lib().msf_poll(self._h, ticket.id)

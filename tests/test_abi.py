@@ -147,3 +147,28 @@ def test_headers_compile_as_c99(tmp_path):
                         "-I", os.path.join(ROOT, "include"), "-c", str(src), "-o",
                         str(tmp_path / "h.o")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
+
+
+def test_tuning_file_round_trip(msf, tmp_path):
+    """msf_tuning_load merges a file's entries, msf_tuning_save writes them
+    back, malformed files are rejected; host only."""
+    msf.tuning_clear()
+    p = tmp_path / "tune.txt"
+    p.write_text("# comment\n"
+                 "gfx950-256cu 512 512 352 256 av- plan0 = 64 8 2 0\n"
+                 "gfx950-256cu 1024 1024 688 256 av- plan3 = 17 4 2 3\n")
+    assert msf.tuning_load(p) == 2
+    out = tmp_path / "saved.txt"
+    msf.tuning_save(out)
+    lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert sorted(lines) == ["gfx950-256cu 1024 1024 688 256 av- plan3 = 17 4 2 3",
+                             "gfx950-256cu 512 512 352 256 av- plan0 = 64 8 2 0"]
+    bad = tmp_path / "bad.txt"
+    bad.write_text("gfx950-256cu 512 512 352 256 av- plan0 = 64 5 2 0\n")   # no 5-column tile
+    with pytest.raises(msf.MsfError):
+        msf.tuning_load(bad)
+    with pytest.raises(msf.MsfError):
+        msf.tuning_load(tmp_path / "missing.txt")
+    msf.tuning_clear()
+    msf.tuning_save(out)
+    assert [l for l in out.read_text().splitlines() if not l.startswith("#")] == []

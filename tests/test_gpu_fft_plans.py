@@ -116,6 +116,7 @@ def test_autotuned_plan_reproduces_when_set(msf, ctx):
     try:
         ctx.configure(fft_plan=1)
         img1, v1 = run()
+        msf.tuning_clear()
         ctx.configure(autotune=True)
         img_t, v_t = run()
         st = ctx.stats()

@@ -327,9 +327,10 @@ def test_autotuned_launch_shape_gives_identical_results(msf, ctx):
     run = lambda: ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
                               fem=c.femcfg(msf))
     try:
+        msf.tuning_clear()                                # the table is process-wide
         ctx.configure()
         img0, v0 = run()
-        assert ctx.stats()["autotuned"] == 0              # opt-in
+        assert ctx.stats()["autotuned"] == 0              # below the automatic size
         ctx.configure(autotune=True)
         img1, v1 = run()
         st = ctx.stats()

@@ -149,3 +149,45 @@ def test_admission_by_footprint(msf):
     finally:
         ctx2.close()
         ctx1.close()
+
+
+def test_runs_on_a_resident_specimen_share_one_build(msf):
+    """Four served runs on one resident specimen (other optics, rasters and
+    detectors each): the engine builds no specimen for them, and each result
+    equals the same job submitted with its atoms (a build per job) and a
+    direct run, bit for bit.  Freeing the specimen while runs on it are
+    queued waits for them."""
+    (c, xp), = _jobs(1, seed=3300)
+    eng = msf.Context(0)
+    try:
+        eng.serve_configure(slots=3)
+        spec = c.gpu_specimen(msf, eng)
+        fem = c.femcfg(msf)
+        variants = [(c.optics(msf), xp, xp, c.detector(msf)),
+                    (msf.Optics(0, 0.0, 0.0, 30.0, 0.012), xp[:2], xp, c.detector(msf)),
+                    (c.optics(msf), xp, xp[::-1].copy(), msf.Detector(1, 0.04, 0.15, 0)),
+                    (msf.Optics(0, 1.0e5, 0.0, -20.0, 0.018), xp, xp[1:], c.detector(msf))]
+        runs = [eng.submit_run(spec, o, x, y, detector=d, fem=fem) for o, x, y, d in variants]
+        got = [eng.wait(t) for t in runs]
+        st = eng.serve_stats()
+        print("resident:", st, flush=True)
+        assert st["specimen_builds"] == 0 and st["jobs_done"] == 4 and st["resident_bytes"] == 0
+        own = [eng.submit(c.grid(msf), c.q, c.Z, c.min_slice, o, x, y, detector=d, fem=fem)
+               for o, x, y, d in variants]
+        got_own = [eng.wait(t) for t in own]
+        assert eng.serve_stats()["specimen_builds"] == 4
+        for (o, x, y, d), (img, v), (img2, v2) in zip(variants, got, got_own):
+            np.testing.assert_array_equal(img, img2)
+            np.testing.assert_array_equal(v["v_omega"], v2["v_omega"])
+            img3, v3 = eng.adfstem(spec, o, x, y, detector=d, fem=fem)
+            np.testing.assert_array_equal(img, img3)
+            np.testing.assert_array_equal(v["v_omega"], v3)
+        # runs still queued when the specimen is freed: the free waits for them
+        late = [eng.submit_run(spec, o, x, y, detector=d, fem=fem) for o, x, y, d in variants[:2]]
+        spec.free()
+        for t, (img, v) in zip(late, got[:2]):
+            img4, v4 = eng.wait(t)
+            np.testing.assert_array_equal(img4, img)
+            np.testing.assert_array_equal(v4["v_omega"], v["v_omega"])
+    finally:
+        eng.close()
