@@ -40,6 +40,12 @@ def algorithmic_bytes(nx, ny, batch):
     return 16.0 * nx * ny * batch
 
 
+def _tune(args, default=0):
+    """msf_config.autotune of a workload: --autotune if given, else the
+    workload's default (0: the library's automatic tuning)."""
+    return default if args.autotune is None else args.autotune
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -218,7 +224,7 @@ def single_job(args, msf, desc, seed, grid, q, Z, min_slice, optics, xp, yp, det
     ctx = msf.Context(0)
     spec = ctx.specimen(grid, q, Z, min_slice)
     S, P = spec.n_slices, xp.size * yp.size
-    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args))
     for _ in range(args.warmup):
         ctx.adfstem(spec, optics, xp, yp, detector=det, fem=fem)
     times = []
@@ -227,7 +233,7 @@ def single_job(args, msf, desc, seed, grid, q, Z, min_slice, optics, xp, yp, det
         ctx.adfstem(spec, optics, xp, yp, detector=det, fem=fem)
         times.append(time.perf_counter() - t0)
     st = ctx.stats()
-    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, profile_kernels=True)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args), profile_kernels=True)
     ctx.adfstem(spec, optics, xp, yp, detector=det, fem=fem)
     pst = ctx.stats()
     kern = {"row_pass": (pst["row_pass_us"], pst["row_pass_bytes"]),
@@ -259,15 +265,16 @@ def uneven_batch(args, msf, SP):
     context with the autotuner and graph replay (an untimed pass tunes each
     shape, a second is timed), then submitted together to the serving engine
     (3 slots); each job builds its own specimen."""
-    log("generating 48 uneven jobs (seeds 3000-3047)")
-    jobs = SP.uneven_jobs(48, seed=3000)
+    grids = [args.grid] if args.grid else None
+    log("generating 48 uneven jobs (seeds 3000-3047)" + (f", grid {args.grid} only" if grids else ""))
+    jobs = SP.uneven_jobs(48, seed=3000, grids=grids)
     prep = []
     for j in jobs:
         grid = msf.make_grid(j["nx"], j["ny"], j["cell"][0], j["cell"][1], j["cell"][2], j["voltage"])
         prep.append((grid, msf.Optics(0, 0.0, 0.0, 0.0, j["alpha_max"]),
                      msf.Detector(1, j["det_inner"], j["det_outer"], msf.ADF_REFERENCE)))
     ctx = msf.Context(0)
-    ctx.configure(autotune=True, fft_plan=args.fft_plan)
+    ctx.configure(autotune=_tune(args, 1), fft_plan=args.fft_plan)
     slices = []
     lat = []
     t_all = 0.0
@@ -312,7 +319,7 @@ def uneven_batch(args, msf, SP):
     Ns = [j["nx"] for j in jobs]
     config = {"workload": "uneven_stem_48: 48 Si jobs, N 256/384/512, cubic 20-50 A at 0.05 A^-3, "
                           "minslice 2.0, rasters 16-48 per side, ADF 50-180 mrad, 100 kV, 20 mrad",
-              "jobs": 48, "grids": {str(n): Ns.count(n) for n in (256, 384, 512)},
+              "jobs": len(jobs), "grids": {str(n): Ns.count(n) for n in (256, 384, 512)},
               "atoms": int(sum(j["Z"].size for j in jobs)), "slices": int(sum(slices)),
               "positions": int(sum(j["x_p"].size * j["y_p"].size for j in jobs)),
               "probe_slices": int(work), "slots": 3, "seed": 3000}
@@ -332,7 +339,7 @@ def cuzr_realizations(args, msf, SP):
     computation (empty rings are NaN in every realization alike)."""
     seeds = (2001, 2002, 2003, 2004)
     ctx = msf.Context(0)
-    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args))
     fem = None
     runs = []
     for seed in seeds:
@@ -383,6 +390,8 @@ def cuzr_realizations(args, msf, SP):
 def other_workload(args):
     import msf
     import specimens as SP
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib << 20)
     if args.workload == "uneven48":
         out = uneven_batch(args, msf, SP)
     elif args.workload == "cuzr512x4":
@@ -420,6 +429,10 @@ def main():
     ap.add_argument("--v-out", default="",
                     help="cuzr512x4: write the averaged V_omega as the reference's text file")
     ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=None, choices=[-1, 0, 1],
+                    help="msf_config.autotune (-1: the fixed rules, as for counter profiles)")
+    ap.add_argument("--grid", type=int, default=0, choices=[0, 256, 384, 512],
+                    help="uneven48: only the jobs of this grid (0: all 48)")
     ap.add_argument("--fft-plan", type=int, default=0,
                     help="radix plans: 1 primary, 2 alternate, 0 automatic per length")
     ap.add_argument("--plain-positions", type=int, default=64)
@@ -461,7 +474,7 @@ def main():
     fem = msf.Fem(1, job["dr"])
     log(f"specimen: {S} slices, build {spec_ms:.1f} ms")
 
-    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args))
     for _ in range(args.warmup):
         ctx.adfstem(spec, optics, xp, yp, fem=fem)
     times = []
@@ -476,10 +489,10 @@ def main():
     log(f"batched: {value:.4g} props/s, batch {st['batch']}, loop {st['slice_loop_ms']:.2f} ms")
 
     # profiled step: per-kernel durations with HIP events on the library stream
-    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, profile_kernels=True)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args), profile_kernels=True)
     ctx.adfstem(spec, optics, xp, yp, fem=fem)
     pst = ctx.stats()
-    ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
+    ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args))
     kern = {"row_pass": (pst["row_pass_us"], pst["row_pass_bytes"]),
             "col_pass": (pst["col_pass_us"], pst["col_pass_bytes"])}
     dom = max(kern, key=lambda k: kern[k][0])
@@ -542,7 +555,7 @@ def main():
         ctx.adfstem(spec, optics, xs, yp[:1], fem=fem)
         dt = time.perf_counter() - t0
         pls = ctx.stats()
-        ctx.configure(batch=args.batch, fft_plan=args.fft_plan)
+        ctx.configure(batch=args.batch, fft_plan=args.fft_plan, autotune=_tune(args))
         plain = {"value": n * S / dt, "positions": n, "batch": 1,
                  "shape": f"stream launches, one position per launch: row pass one row per "
                           f"workgroup, column pass {pls['col_tile']} column(s) per workgroup",

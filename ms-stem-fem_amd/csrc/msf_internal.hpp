@@ -222,6 +222,8 @@ struct ServeState {
 // chains in flight and the radix plans (vx + 2 vy, per axis 0 primary,
 // 1 alternate).
 using TunedShape = std::array<int, 4>;       // batch, col_tile, chains, plan
+// msf_config.autotune of served jobs: the tuning table's choices, no trials
+constexpr int32_t kAutotuneLookup = 2;
 
 } // namespace msf_impl
 
@@ -386,8 +388,9 @@ std::string device_arch(int device);
 using TrialFn = std::function<int(const TunedShape&, double&)>;
 // The tuned shape of `key` (process-wide), timing the candidates (and the
 // rule's shape) on first use.  state: 1 tuned in this call, 2 tuned earlier.
+// With measure false, a shape not in the table takes the rule's (state 0).
 int autotune_shape(msf_ctx* c, const std::string& key, const TunedShape& rule,
-                   const std::vector<TunedShape>& cands, const TrialFn& trial, TunedShape& pick,
-                   int& state);
+                   const std::vector<TunedShape>& cands, const TrialFn& trial, bool measure,
+                   TunedShape& pick, int& state);
 
 } // namespace msf_impl

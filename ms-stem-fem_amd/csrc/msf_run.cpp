@@ -605,6 +605,7 @@ int launch_shape(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp,
     // probe-slices, shapes only, results bit-identical to the rules;
     // 1: every run of two or more positions, and with fft_plan 0 the radix
     // plans too; -1: the rules).
+    // (kAutotuneLookup, served jobs: the table's choice, no trials)
     const int amode = c->cfg.autotune;
     if (amode >= 0 && c->cfg.batch == 0 && c->cfg.col_tile == 0 && c->cfg.chains == 0 && !prof &&
         !out.exit && P >= 2 && !m.scratch() &&
@@ -908,6 +909,7 @@ int tune_shape(msf_ctx* c, RunPlan& pl, const msf_specimen* sp, const std::vecto
     const int64_t P = f.P;
     if (c->arch.empty()) c->arch = device_arch(c->device);
     const bool plans = c->cfg.autotune == 1 && c->cfg.fft_plan == 0;
+    const bool measure = c->cfg.autotune != kAutotuneLookup;
     std::ostringstream key;
     key << c->arch << " " << f.nx << " " << f.ny << " " << f.ncp << " " << std::min<int64_t>(P, 256)
         << " " << (f.want_adf ? "a" : "") << (f.want_d1 ? "v" : "") << (f.want_d2 ? "b" : "")
@@ -972,7 +974,7 @@ int tune_shape(msf_ctx* c, RunPlan& pl, const msf_specimen* sp, const std::vecto
         cost = 1e-3 * ms / ((double)ft.P * f.S);
         return MSF_OK;
     };
-    const int r = autotune_shape(c, key.str(), rule, cands, trial, pick, state);
+    const int r = autotune_shape(c, key.str(), rule, cands, trial, measure, pick, state);
     (void)hipStreamSynchronize(c->stream);
     return r;
 }

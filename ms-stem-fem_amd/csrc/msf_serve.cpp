@@ -189,6 +189,9 @@ int queue_job(msf_ctx* c, ServeJob& job, uint64_t* id)
         // served jobs launch on their streams: a graph capture on one worker
         // would be invalidated by another worker's allocations
         job.cfg.no_graph = 1;
+        // and measure no launch shapes (trials would add seconds to a job's
+        // latency): they take the tuning table's choices or the rules
+        if (job.cfg.autotune == 0) job.cfg.autotune = kAutotuneLookup;
     }
     job.footprint = job_footprint(job);
     job.t_submit = now_ms();

@@ -39,8 +39,8 @@ std::string device_arch(int device)
 // statistics, with the candidate's chains) keep the fastest, unless it
 // beats the rule's shape by less than 3 % (trial noise).
 int autotune_shape(msf_ctx* c, const std::string& key, const TunedShape& rule,
-                   const std::vector<TunedShape>& cands, const TrialFn& trial, TunedShape& pick,
-                   int& state)
+                   const std::vector<TunedShape>& cands, const TrialFn& trial, bool measure,
+                   TunedShape& pick, int& state)
 {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -50,6 +50,11 @@ int autotune_shape(msf_ctx* c, const std::string& key, const TunedShape& rule,
             state = 2;
             return MSF_OK;
         }
+    }
+    if (!measure) {
+        pick = rule;
+        state = 0;
+        return MSF_OK;
     }
     double warm = 0.0; // clocks and caches to steady state first
     if (int r = trial(rule, warm)) return r;

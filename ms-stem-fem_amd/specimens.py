@@ -137,7 +137,7 @@ def config_cuzr_512(seed=2001):
     return job, q, Z, xp, xp.copy()
 
 
-def uneven_jobs(n_jobs=48, seed=3000):
+def uneven_jobs(n_jobs=48, seed=3000, grids=None):
     """Batched uneven-size STEM jobs: job j draws from numpy's default_rng(seed
     + j) N in {256, 384, 512}, a cubic cell side U[20, 50] A and a raster side
     U{16..48} (positions at spacing L/side over the cell); amorphous Si at
@@ -145,13 +145,16 @@ def uneven_jobs(n_jobs=48, seed=3000):
     ADF 50-180 mrad.  100 kV and a 20 mrad aperture (the workload names
     neither): at 300 kV the 50 mrad inner angle lies beyond the 2/3 band
     limit lambda * N / (3 L) of the coarser grids (N 256 with L > 33.6 A),
-    which the reference rejects (tem.cpp:721-732)."""
+    which the reference rejects (tem.cpp:721-732).  With `grids`, only the
+    jobs whose N is listed (the others are drawn but not built)."""
     jobs = []
     for j in range(n_jobs):
         rng = np.random.default_rng(seed + j)
         N = int(rng.choice([256, 384, 512]))
         L = float(rng.uniform(20.0, 50.0))
         side = int(rng.integers(16, 49))
+        if grids is not None and N not in grids:
+            continue
         q, Z = amorphous(int(round(0.05 * L ** 3)), (L, L, L), 2.0, seed + j)
         jobs.append(dict(nx=N, ny=N, cell=(L, L, L), q=q, Z=Z, x_p=raster(side, L),
                          y_p=raster(side, L), voltage=100e3, alpha_max=0.020, det_inner=0.050,
