@@ -276,6 +276,27 @@ int msf_run_stem(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optic
                  const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
                  double* adf_image, double* fem_v);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, allocation-free serving loops):
+ * msf_run_stem on a resident specimen with every device buffer of the run
+ * (twiddle tables, pixel-group tables, vacuum waves, the batch wave buffer,
+ * partial sums and accumulators) carved out of one static device array of
+ * MSF_NOALLOC_SCRATCH_BYTES that each device the library runs on holds until
+ * the process ends.  It calls no hipMalloc, hipFree or other allocating
+ * routine, and takes no state from an earlier call: the array's contents
+ * are rebuilt at every call (set-up is repeated: probe, vacuum waves, table
+ * uploads), and the batch depth is the ordinary rule's or less, one chain,
+ * stream launches.  Results equal msf_run_stem's bit for bit.  A run that
+ * does not fit fails with MSF_ERR_INVALID before it launches its slice loop.
+ * All calls on one device share its array: the library serialises them, so
+ * calls that overlap on one device (from several contexts or threads) wait
+ * for each other instead of racing. */
+#define MSF_NOALLOC_SCRATCH_BYTES (256u << 20)
+int msf_run_stem_noalloc(msf_ctx* ctx, const msf_specimen* spec, const msf_optics* optics,
+                         const msf_detector* det, const msf_fem* fem,
+                         const double* x_p, uint32_t n_xp, const double* y_p, uint32_t n_yp,
+                         double* adf_image, double* fem_v);
+
 /* Probe shift indices of every raster position, (sx, sy) pairs in raster
  * order (x outer, y inner): the nearest-pixel rule of adfstem.cpp:1590-1606
  * with indices persisting from the previous position when no grid point

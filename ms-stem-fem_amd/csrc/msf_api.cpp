@@ -41,6 +41,11 @@ int msf_create(msf_ctx** out, int device)
             delete c;
             return MSF_ERR_DEVICE;
         }
+        if (scratch_attach(device, &c->scratch) != MSF_OK) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return MSF_ERR_DEVICE;
+        }
         *out = c;
         return MSF_OK;
     }

@@ -246,6 +246,7 @@ struct msf_ctx {
     std::string arch;           // device key of the tuning table
     msf_impl::RunPlan plan;     // ordinary runs
     msf_impl::RunPlan plan_na;  // the allocation-free run (device views rebuilt per call)
+    char* scratch = nullptr;    // this device's copy of the allocation-free run's array
     msf_ctx* owner = nullptr;   // serving slot: the context whose specimens it may run on
     std::set<msf_specimen*> specimens; // live specimens built on this context
 };
@@ -380,6 +381,11 @@ int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o
 // ---- serving (msf_serve.cpp)
 // Blocks until no queued or running served job refers to the specimen.
 void wait_unserved(msf_specimen* sp);
+
+// ---- the allocation-free run (msf_noalloc.cpp)
+// With `device` current: the address of its copy of the static scratch
+// array (looked up once per device).
+int scratch_attach(int device, char** out);
 
 // ---- autotuner (msf_tune.cpp)
 // "<gcnArchName>-<CUs>cu" of a device: the tuning table's device key

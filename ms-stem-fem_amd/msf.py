@@ -152,6 +152,10 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+# device bytes of the static array the allocation-free run works in
+# (MSF_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 256 << 20
+
 IO_EXPORTS = ["msf_io_last_error", "msf_element_z", "msf_guess_z", "msf_read_xyz",
               "msf_read_lammps_data", "msf_model_free", "msf_write_tiff16",
               "msf_write_variance_txt", "msf_write_f64", "msf_write_correlograph_txt",
@@ -167,7 +171,7 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_slice_step_batch", "msf_correlograph_bins", "msf_adf_reduce",
            "msf_fem_accumulate", "msf_mtf_2d", "msf_apply_mtf", "msf_wait_info",
            "msf_serve_configure", "msf_serve_get_stats", "msf_job_footprint",
-           "msf_copy_rate", "msf_submit_run", "msf_tuning_save",
+           "msf_run_stem_noalloc", "msf_copy_rate", "msf_submit_run", "msf_tuning_save",
            "msf_tuning_load", "msf_tuning_clear"]
 
 _lib = None
@@ -212,6 +216,9 @@ def lib():
         L.msf_tuning_save.argtypes = [C.c_char_p]
         L.msf_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
         L.msf_tuning_clear.argtypes = []
+        L.msf_run_stem_noalloc.argtypes = [vp, vp, C.POINTER(Optics), C.POINTER(Detector),
+                                           C.POINTER(Fem), _dp, C.c_uint32, _dp, C.c_uint32, _dp,
+                                           _dp]
         L.msf_exit_waves.argtypes = [vp, vp, C.POINTER(Optics), _dp, C.c_uint32, _dp,
                                      C.c_uint32, C.POINTER(C.c_float)]
         L.msf_probe_shifts.argtypes = [C.POINTER(Grid), _dp, C.c_uint32, _dp, C.c_uint32,
@@ -515,6 +522,22 @@ class Context:
         if v is not None:
             v = v[:fem_bins(spec.grid, fm.dr).size]
         return img, v
+
+    def adfstem_noalloc(self, spec, optics, x_p, y_p, detector=None, fem=None):
+        """msf_run_stem_noalloc: adfstem() with every device buffer in the
+        library's static scratch array (no device allocation)."""
+        x_p = np.ascontiguousarray(x_p, np.float64)
+        y_p = np.ascontiguousarray(y_p, np.float64)
+        det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
+        fm = fem if fem is not None else Fem(0, 0.0)
+        img = np.zeros(x_p.size * y_p.size) if det.enabled else None
+        nb = fem_bins(spec.grid, fm.dr).size if fm.enabled else 0
+        v = np.zeros(max(nb, 1)) if fm.enabled else None
+        self._check(lib().msf_run_stem_noalloc(self._h, spec._h, C.byref(optics), C.byref(det),
+                                               C.byref(fm), _p(x_p), x_p.size, _p(y_p),
+                                               y_p.size, _p(img), _p(v)), "msf_run_stem_noalloc")
+        return (img.reshape(x_p.size, y_p.size) if img is not None else None,
+                v[:nb] if v is not None else None)
 
     def adfstem_fem(self, spec, optics, x_p, y_p, fem, modes=FEM_MODES, detector=None,
                     correlograph=False, correlograph_each=False, pixel_images=False):
