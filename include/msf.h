@@ -100,14 +100,18 @@ typedef struct msf_config {
     int32_t col_tile;                  /* columns per column-pass workgroup: 4, 8, 16; 0 = automatic */
     int32_t no_graph;                  /* 1: launch on the stream; 0: capture the job's launch
                                           sequence once per shape as a HIP graph and replay it */
-    int32_t chains;                    /* batch chains in flight: 1, or 2 (alternate batches on
-                                          two streams, FEM statistics still in raster order,
-                                          results bit-identical); 0 = automatic (two chains on
-                                          the streams for launches of >= 4M pixels, else one
-                                          chain as a graph) */
+    int32_t chains;                    /* batch chains in flight: 1, 2 (alternate batches on
+                                          two streams), or 3 (batches in pairs on one stream,
+                                          the row pass of one and the column pass of the other
+                                          fused into one launch, on square 256/384/512 grids with
+                                          their default radix plans and tiles; elsewhere the
+                                          passes launch one after the other); FEM statistics
+                                          still in raster order, results bit-identical;
+                                          0 = automatic (two chains on the streams for
+                                          launches of >= 4M pixels, else one chain as a graph) */
     int32_t autotune;                  /* with batch, col_tile and chains all automatic, the
-                                          launch shape (positions per batch, column tile, one or
-                                          two chains) comes from the process-wide tuning table
+                                          launch shape (positions per batch, column tile,
+                                          chains 1, 2 or 3) comes from the process-wide tuning table
                                           (msf_tuning_*), measured on first use by timing whole
                                           batch steps of the job itself per device type and
                                           shape (nx, ny, kept columns, positions up to 256,
@@ -144,6 +148,8 @@ typedef struct msf_stats {
     int32_t autotuned;                 /* 0: fixed rules; 1: tuned in this call; 2: tuned earlier */
     int32_t row_fft_plan;              /* radix plan of the row passes (length ny) and */
     int32_t col_fft_plan;              /* of the column passes (nx): 1 primary, 2 alternate */
+    uint64_t fused_launches;           /* launches that ran a row pass and a column pass
+                                          together (chains 3) */
 } msf_stats;
 
 /* ------------------------------------------------------------ context */

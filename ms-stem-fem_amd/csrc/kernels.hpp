@@ -54,6 +54,17 @@ struct ColPassArgs {
     int plan;               // radix plan of length nx: 0 primary, 1 alternate (has_alt_plan)
 };
 
+// One launch with the row pass of one batch (ra) and the column pass of
+// another (ca), same transform length n = nx = ny: the rows' FIRST and the
+// columns' LAST flags as for the separate passes.  hipErrorNotSupported when
+// fused_supported() is false for these arguments.
+hipError_t launch_fused_pass(int n, bool first, bool last, int tmode, const RowPassArgs& ra,
+                             const ColPassArgs& ca, hipStream_t s);
+bool fused_supported(int n, const RowPassArgs& ra, const ColPassArgs& ca);
+// true when length n has fused pass pairs for row plan vr, column plan vc and
+// column tile `tile` (0: the plan's default)
+bool fused_shape_ok(int n, int vr, int vc, int tile);
+
 bool slice_lengths_ok(int n);
 // concatenated per-stage twiddle tables of plan v (0 primary, 1 alternate) of
 // length n, (re, im) FP64

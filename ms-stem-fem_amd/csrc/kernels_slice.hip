@@ -83,7 +83,10 @@ __device__ __forceinline__ int xcd_band(int L, int G)
 // positions); at 1024 the larger LDS footprint costs more than it saves.
 template <int NY, int V> __host__ __device__ constexpr int row_threads()
 {
-    return (NY == 512 && V == 0) ? 512 : 256;
+    // the alternate 1024 plan (16 values on 64 lanes, ~120 registers): two
+    // rows per 128-lane workgroup, 71 us against 76 us on 256 lanes per 17
+    // positions
+    return (NY == 512 && V == 0) ? 512 : (NY == 1024 && V == 1) ? 128 : 256;
 }
 
 // Lean row passes read the twiddle table and the transmission row from
@@ -103,23 +106,36 @@ template <int NY, int V> __host__ __device__ constexpr int row_waves()
 // RT lanes per workgroup: row_threads() for batches that fill it, or one
 // row's T lanes when the batch has fewer positions than a workgroup has rows
 // (no lane then computes a row it does not store).
+// LDS of a row-pass workgroup in float2 units: the staged twiddle table and
+// transmission row (unless lean), and one exchange buffer per row.
+template <int NY, int V, int RT> __host__ __device__ constexpr bool row_is_lean()
+{
+    // one row per workgroup: nothing to share, tables are read from L2
+    return row_lean<NY, V>() || RT / PlanV<NY, V>::T == 1;
+}
+template <int NY, int V, int RT> __host__ __device__ constexpr int row_smem_len()
+{
+    return (row_is_lean<NY, V, RT>() ? 0 : plan_tw_len<PlanV<NY, V>>() + NY) +
+           (RT / PlanV<NY, V>::T) * lds_len(NY);
+}
+
+// One row-pass workgroup: logical block `blk` of the pass (XCD-banded over
+// the pass's own blocks), LDS at smem.
 template <int NY, int V, bool FIRST, int TMODE, bool PRUNE, int RT>
-__global__ void __launch_bounds__(RT)
-__attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_pass_kernel(RowPassArgs a)
+__device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* smem)
 {
     using P = PlanV<NY, V>;
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = RT / T;
     constexpr int LEN = lds_len(NY);
-    // one row per workgroup: nothing to share, tables are read from L2
-    constexpr bool LEAN = row_lean<NY, V>() || RPW == 1;
-    __shared__ float2 s_tw[LEAN ? 1 : plan_tw_len<P>()];
-    __shared__ float2 s_t[LEAN ? 1 : NY];
-    __shared__ float2 s_buf[RPW * LEN];
+    constexpr bool LEAN = row_is_lean<NY, V, RT>();
+    float2* s_tw = smem;
+    float2* s_t = smem + (LEAN ? 0 : plan_tw_len<P>());
+    float2* s_buf = smem + (LEAN ? 0 : plan_tw_len<P>() + NY);
 
     // workgroup-uniform decode (scalar unit)
     const int groups = (a.batch + RPW - 1) / RPW;
-    const int g = xcd_band((int)blockIdx.x, groups * a.nx);
+    const int g = xcd_band(blk, groups * a.nx);
     const int i = g / groups;
     const int b0 = (g - i * groups) * RPW;
 
@@ -210,6 +226,14 @@ __attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_pass_kernel(RowPa
     }
 }
 
+template <int NY, int V, bool FIRST, int TMODE, bool PRUNE, int RT>
+__global__ void __launch_bounds__(RT)
+__attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_pass_kernel(RowPassArgs a)
+{
+    __shared__ float2 smem[row_smem_len<NY, V, RT>()];
+    row_body<NY, V, FIRST, TMODE, PRUNE, RT>(a, (int)blockIdx.x, smem);
+}
+
 // --------------------------------------------------------------- column pass
 // A workgroup owns W adjacent kept columns of one position.  The tile is
 // loaded with lanes running along the columns (coalesced W*8-byte segments
@@ -259,8 +283,18 @@ template <int NX, int V, int W> __host__ __device__ constexpr int col_bound()
     return (V || PlanV<NX, V>::E >= 12) ? W * PlanV<NX, V>::T : 1024;
 }
 
+// LDS of a column-pass workgroup in float2 units: the staged twiddle and
+// propagator tables (unless lean), the exit roots (last slice) and the tile.
+template <int NX, int V, bool LAST, int W> __host__ __device__ constexpr int col_smem_len()
+{
+    return (col_lean<NX, V, W>() ? 0 : plan_tw_len<PlanV<NX, V>>() + NX) + (LAST ? NX : 0) +
+           W * col_len<NX, W>();
+}
+
+// One column-pass workgroup: logical block `blk` of `nblk` (XCD-banded over
+// the pass's own blocks), LDS at smem.
 template <int NX, int V, bool LAST, int W, bool PRUNE>
-__global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPassArgs a)
+__device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk, float2* smem)
 {
     using P = PlanV<NX, V>;
     constexpr int E = P::E, T = P::T;
@@ -270,10 +304,10 @@ __global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPa
     // spans whole wavefronts (workgroup barriers)
     static_assert(64 % T == 0 || T % 64 == 0, "a column transform must align with wavefronts");
     constexpr bool WAVE = T <= 64;
-    __shared__ float2 s_tw[LEAN ? 1 : plan_tw_len<P>()];
-    __shared__ float2 s_nat[LAST ? NX : 1]; // roots W_nx^e for the exit phase ramp
-    __shared__ float2 s_px[LEAN ? 1 : NX];
-    __shared__ float2 s_buf[W * LEN];
+    float2* s_tw = smem;
+    float2* s_px = smem + (LEAN ? 0 : plan_tw_len<P>());
+    float2* s_nat = smem + (LEAN ? 0 : plan_tw_len<P>() + NX); // roots W_nx^e (last slice)
+    float2* s_buf = s_nat + (LAST ? NX : 0);
 
     for (int e = threadIdx.x; e < NX; e += blockDim.x) {
         if constexpr (!LEAN) {
@@ -287,11 +321,13 @@ __global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPa
 
     // neighbouring tiles share 128-byte lines (W = 8 reads 64 B per row): keep
     // them on one XCD, next to each other in dispatch order
-    const int ntiles = a.ncp / W;
-    const int g = xcd_band((int)blockIdx.x, (int)gridDim.x);
+    // tiles cover the stored columns [0, nc); tiles of padding columns only
+    // (ncp is rounded up to 16) are not launched
+    const int nc = a.ka + a.kb;
+    const int ntiles = (nc + W - 1) / W;
+    const int g = xcd_band(blk, nblk);
     const int b = g / ntiles;
     const int j0 = (g - b * ntiles) * W;
-    const int nc = a.ka + a.kb;
     float2* tile = a.psi + (int64_t)b * NX * a.ncp + j0;
 
     {   // coalesced load of the tile (32-bit offsets from the uniform tile base).
@@ -379,6 +415,50 @@ __global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPa
             }
         }
     }
+}
+
+template <int NX, int V, bool LAST, int W, bool PRUNE>
+__global__ void __launch_bounds__((col_bound<NX, V, W>())) col_pass_kernel(ColPassArgs a)
+{
+    __shared__ float2 smem[col_smem_len<NX, V, LAST, W>()];
+    col_body<NX, V, LAST, W, PRUNE>(a, (int)blockIdx.x, (int)gridDim.x, smem);
+}
+
+// ------------------------------------------------------- fused pass pair
+// One launch runs a row pass of one batch and a column pass of another
+// (independent buffers), so that the VALU-bound row workgroups and the
+// memory-bound column workgroups share the CUs instead of following each
+// other.  Both passes need workgroups of the same size (RT = W * T).  Launch
+// blocks alternate in groups of eight, eight of each pass per sixteen, so
+// that a pass's logical block k runs on XCD k % 8 as its XCD banding
+// assumes (the row pass's block count, groups * nx, is a multiple of 8).
+__device__ __forceinline__ bool fused_decode(int b, int nr, int nc, int& id)
+{
+    const int m8 = (nr < nc ? nr : nc) & ~7;
+    if (b < 2 * m8) {
+        const int g = b >> 4, k = b & 15;
+        id = g * 8 + (k & 7);
+        return k < 8;
+    }
+    const int r = b - 2 * m8;
+    if (r < nr - m8) {
+        id = m8 + r;
+        return true;
+    }
+    id = m8 + (r - (nr - m8));
+    return false;
+}
+
+template <int N, int VR, int VC, bool FIRST, bool LAST, int TMODE, bool PR, bool PC, int W, int RT>
+__global__ void __launch_bounds__(RT) fused_pass_kernel(RowPassArgs ra, ColPassArgs ca, int nr, int nc)
+{
+    constexpr int LR = row_smem_len<N, VR, RT>(), LC = col_smem_len<N, VC, LAST, W>();
+    __shared__ float2 smem[LR > LC ? LR : LC];
+    int id = 0;
+    if (fused_decode((int)blockIdx.x, nr, nc, id))
+        row_body<N, VR, FIRST, TMODE, PR, RT>(ra, id, smem);
+    else
+        col_body<N, VC, LAST, W, PC>(ca, id, nc, smem);
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -565,7 +645,7 @@ static hipError_t col_pass_w(bool last, const ColPassArgs& a, hipStream_t s, hip
                              hipEvent_t ev1)
 {
     if (a.ncp % W) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((int64_t)a.batch * (a.ncp / W)));
+    const dim3 grid((unsigned)((int64_t)a.batch * ((a.ka + a.kb + W - 1) / W)));
     const dim3 block(W * PlanV<NX, V>::T);
 #define MSF_COL(L, PR)                                                                             \
     do {                                                                                           \
@@ -630,6 +710,99 @@ hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t 
     }
 }
 
+// Fused pass pairs exist for the lengths whose default row and column
+// workgroups have the same size: 256 (alternate plans: 16 rows x 16 lanes,
+// 16 columns x 16 lanes), 384 (8 x 32, 8 x 32) and 512 (8 x 64, 8 x 64),
+// with both passes pruned.
+template <int N> struct FusedShape { static constexpr bool exists = false; };
+template <> struct FusedShape<256> { static constexpr bool exists = true; static constexpr int VR = 1, VC = 1; };
+template <> struct FusedShape<384> { static constexpr bool exists = true; static constexpr int VR = 0, VC = 0; };
+template <> struct FusedShape<512> { static constexpr bool exists = true; static constexpr int VR = 0, VC = 0; };
+
+template <int N> static bool fused_ok(const RowPassArgs& ra, const ColPassArgs& ca)
+{
+    if constexpr (!FusedShape<N>::exists) {
+        return false;
+    } else {
+        constexpr int VR = FusedShape<N>::VR, VC = FusedShape<N>::VC;
+        constexpr int RT = row_threads<N, VR>(), W = ColTile<N, VC>::W;
+        static_assert(RT == W * PlanV<N, VC>::T, "fused workgroups must match");
+        const int w = (ca.tile == 4 || ca.tile == 8 || ca.tile == 16) ? ca.tile : W;
+        return ra.plan == VR && ca.plan == VC && w == W && ra.prune && ca.prune &&
+               ra.batch >= RT / PlanV<N, VR>::T && ca.ncp % W == 0 && ra.nx == N && ca.ny == N;
+    }
+}
+
+template <int N> static bool fused_shape_n(int vr, int vc, int tile)
+{
+    if constexpr (!FusedShape<N>::exists) {
+        return false;
+    } else {
+        constexpr int W = ColTile<N, FusedShape<N>::VC>::W;
+        return vr == FusedShape<N>::VR && vc == FusedShape<N>::VC && (tile == 0 || tile == W);
+    }
+}
+
+bool fused_shape_ok(int n, int vr, int vc, int tile)
+{
+    switch (n) {
+    case 256: return fused_shape_n<256>(vr, vc, tile);
+    case 384: return fused_shape_n<384>(vr, vc, tile);
+    case 512: return fused_shape_n<512>(vr, vc, tile);
+    default: return false;
+    }
+}
+
+bool fused_supported(int n, const RowPassArgs& ra, const ColPassArgs& ca)
+{
+    switch (n) {
+    case 256: return fused_ok<256>(ra, ca);
+    case 384: return fused_ok<384>(ra, ca);
+    case 512: return fused_ok<512>(ra, ca);
+    default: return false;
+    }
+}
+
+template <int N>
+static hipError_t fused_n(bool first, bool last, int tmode, const RowPassArgs& ra,
+                          const ColPassArgs& ca, hipStream_t s)
+{
+    if constexpr (!FusedShape<N>::exists) {
+        return hipErrorNotSupported;
+    } else {
+        if (!fused_ok<N>(ra, ca)) return hipErrorNotSupported;
+        constexpr int VR = FusedShape<N>::VR, VC = FusedShape<N>::VC;
+        constexpr int RT = row_threads<N, VR>(), W = ColTile<N, VC>::W;
+        constexpr int RPW = RT / PlanV<N, VR>::T;
+        const int nr = ((ra.batch + RPW - 1) / RPW) * ra.nx;
+        const int nc = ca.batch * ((ca.ka + ca.kb + W - 1) / W);
+        const dim3 grid((unsigned)(nr + nc)), block(RT);
+#define MSF_FUSED(F, L, M)                                                                         \
+    hipLaunchKernelGGL((fused_pass_kernel<N, VR, VC, F, L, M, true, true, W, RT>), grid, block, 0,  \
+                       s, ra, ca, nr, nc)
+        if (first) {
+            if (last) { if (tmode == 0) MSF_FUSED(true, true, 0); else MSF_FUSED(true, true, 1); }
+            else { if (tmode == 0) MSF_FUSED(true, false, 0); else MSF_FUSED(true, false, 1); }
+        } else {
+            if (last) { if (tmode == 0) MSF_FUSED(false, true, 0); else MSF_FUSED(false, true, 1); }
+            else { if (tmode == 0) MSF_FUSED(false, false, 0); else MSF_FUSED(false, false, 1); }
+        }
+#undef MSF_FUSED
+        return hipGetLastError();
+    }
+}
+
+hipError_t launch_fused_pass(int n, bool first, bool last, int tmode, const RowPassArgs& ra,
+                             const ColPassArgs& ca, hipStream_t s)
+{
+    switch (n) {
+    case 256: return fused_n<256>(first, last, tmode, ra, ca, s);
+    case 384: return fused_n<384>(first, last, tmode, ra, ca, s);
+    case 512: return fused_n<512>(first, last, tmode, ra, ca, s);
+    default: return hipErrorNotSupported;
+    }
+}
+
 // ------------------------------------------------------------- reductions
 // Each lane sums one segment of a ring's pixel list in FP64, in a fixed order;
 // the segments of a ring are then added in segment order.  Neither depends on
@@ -641,8 +814,13 @@ __global__ void __launch_bounds__(256) reduce_segments_kernel(const float2* psi,
                                                                int n_tiles, double* seg_out)
 {
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int b = blockIdx.y;
+    // Tiles follow the groups in ring order, and neighbouring rings read the
+    // same cache lines (a FEM ring is narrower than a pixel): give each XCD a
+    // contiguous range of (position, tile group) so those reads meet in one L2
+    const int gx = (int)gridDim.x;
+    const int lin = xcd_band((int)(blockIdx.y * gridDim.x + blockIdx.x), gx * (int)gridDim.y);
+    const int b = lin / gx;
+    const int w = (lin - b * gx) * 4 + (threadIdx.x >> 6);
     if (w >= n_tiles) return;
     const float2* wave = psi + (int64_t)b * plane;
     const int k0 = tile_off[w], k1 = tile_off[w + 1];

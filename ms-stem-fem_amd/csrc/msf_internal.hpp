@@ -116,6 +116,7 @@ struct RunPlan {
     // every pointer and size it was captured with is unchanged
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_sig;
+    uint64_t graph_fused = 0; // fused pass launches in the captured graph
     RunPlan() = default;
     RunPlan(const RunPlan&) = delete;
     RunPlan& operator=(const RunPlan&) = delete;

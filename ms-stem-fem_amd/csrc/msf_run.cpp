@@ -646,7 +646,11 @@ int launch_shape(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp,
     sh.n_batches = (P + B - 1) / B;
     const bool big_launch = B * f.plane >= (4ll << 20);
     const int chains_req = chains_cfg ? chains_cfg : (big_launch ? 2 : 1);
-    sh.chains = (chains_req == 2 && sh.n_batches >= 2 && !prof && !out.exit && !m.scratch()) ? 2 : 1;
+    // chains 3: batches in pairs on one stream, row and column passes of the
+    // pair's two batches fused into one launch per step
+    sh.chains = (chains_req >= 2 && sh.n_batches >= 2 && !prof && !out.exit && !m.scratch())
+                    ? chains_req
+                    : 1;
     sh.auto_eager = c->cfg.chains == 0 && sh.chains == 2;
     return MSF_OK;
 }
@@ -665,7 +669,7 @@ int prepare_buffers(msf_ctx* c, Mem& m, RunPlan& pl, const RunFlags& f, const Sh
         pl.exit_b.release();
         pl.cap_b = B;
     }
-    for (int k = 0; k < sh.chains; ++k) {
+    for (int k = 0; k < std::min(sh.chains, 2); ++k) {
         bool fresh = false;
         if (int r = take_as<float2>(c, m, pl.psi_b[k], (size_t)B * f.plane, &v.psi[k], &fresh))
             return r;
@@ -726,6 +730,7 @@ struct Prof {
     std::vector<hipEvent_t> ev;
     std::vector<int> kind;
     hipEvent_t ka0 = nullptr, ka1 = nullptr; // pair stamped by a slice pass's own dispatch
+    uint64_t fused = 0;                      // fused pass launches enqueued
     ~Prof()
     {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -792,34 +797,26 @@ int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const Run
     const float2* px = sp->px.as<float2>();
     const float2* py = sp->py.as<float2>();
     hipStream_t streams[2] = {c->stream, c->stream2};
-    if (chains == 2) { // fork: the second stream starts after everything issued so far
-        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-        HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    }
-    int64_t k = 0;
-    for (int64_t b0 = 0; b0 < P; b0 += B, ++k) {
-        const int nb = (int)std::min<int64_t>(B, P - b0);
-        const int ch = (int)(k % chains);
-        hipStream_t s = streams[ch];
+    auto bind = [&](int64_t b0, int nb, int ch, msf::RowPassArgs& r, msf::ColPassArgs& cc) {
+        r = ra;
+        cc = ca;
+        r.psi = cc.psi = v.psi[ch];
+        r.batch = cc.batch = nb;
+        r.shift = cc.shift = v.shift + b0;
+    };
+    auto row_at = [&](msf::RowPassArgs& r, int n) {
+        r.t = t32 + (int64_t)n * M;
+        r.u0 = pl.u0 + (int64_t)n * 2 * M;
+    };
+    auto col_at = [&](msf::ColPassArgs& cc, int n) {
+        cc.px = px + (int64_t)n * nx;
+        cc.py = py + (int64_t)n * ny;
+    };
+    // everything after the slice loop of batch k (positions b0.., chain ch)
+    auto finish_batch = [&](int64_t k, int64_t b0, int nb, int ch, hipStream_t s) -> int {
         float2* bpsi = v.psi[ch];
         double* bpart = v.part[ch];
         double* bsegp = v.segp[ch];
-        ra.psi = bpsi;
-        ra.batch = nb;
-        ra.shift = v.shift + b0;
-        ca.psi = bpsi;
-        ca.batch = nb;
-        ca.shift = v.shift + b0;
-        for (int n = 0; n < S; ++n) {
-            ra.t = t32 + (int64_t)n * M;
-            ra.u0 = pl.u0 + (int64_t)n * 2 * M;
-            HIPCHK(c, prof.kernel(n == 0 ? K_ROW0 : K_ROW));
-            HIPCHK(c, msf::launch_row_pass(ny, n == 0, f.tmode, ra, s, prof.ka0, prof.ka1));
-            ca.px = px + (int64_t)n * nx;
-            ca.py = py + (int64_t)n * ny;
-            HIPCHK(c, prof.kernel(n == S - 1 ? K_COLL : K_COL));
-            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, ca, s, prof.ka0, prof.ka1));
-        }
         HIPCHK(c, prof.begin(K_RED));
         HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off, pl.seg_kind,
                                               pl.n_tiles, nb, bsegp, s));
@@ -863,6 +860,90 @@ int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const Run
                                      (size_t)nb * nx * ny * sizeof(float2), hipMemcpyDeviceToHost,
                                      s));
         }
+        return MSF_OK;
+    };
+    // the slice loop of one batch alone
+    auto slices_alone = [&](msf::RowPassArgs& r, msf::ColPassArgs& cc, hipStream_t s) -> int {
+        for (int n = 0; n < S; ++n) {
+            row_at(r, n);
+            HIPCHK(c, prof.kernel(n == 0 ? K_ROW0 : K_ROW));
+            HIPCHK(c, msf::launch_row_pass(ny, n == 0, f.tmode, r, s, prof.ka0, prof.ka1));
+            col_at(cc, n);
+            HIPCHK(c, prof.kernel(n == S - 1 ? K_COLL : K_COL));
+            HIPCHK(c, msf::launch_col_pass(nx, n == S - 1, cc, s, prof.ka0, prof.ka1));
+        }
+        return MSF_OK;
+    };
+    if (chains == 3) {
+        // Paired batches on one stream: the row pass of one batch and the
+        // column pass of the other share a launch (fused_pass_kernel), so the
+        // VALU-bound row workgroups and the memory-bound column workgroups of
+        // the two batches run side by side on every CU.  Per pair (A, B):
+        // A.row0, then (A.col n + B.row n), (A.row n+1 + B.col n), ...,
+        // B.col S-1; each batch still sees its passes in slice order, and the
+        // reductions and FEM statistics follow in raster order.
+        hipStream_t s = c->stream;
+        auto pair = [&](bool first, bool last, const msf::RowPassArgs& r,
+                        const msf::ColPassArgs& cc) -> int {
+            if (nx == ny && msf::fused_supported(nx, r, cc)) {
+                HIPCHK(c, msf::launch_fused_pass(nx, first, last, f.tmode, r, cc, s));
+                prof.fused += 1;
+            } else {
+                HIPCHK(c, msf::launch_row_pass(ny, first, f.tmode, r, s));
+                HIPCHK(c, msf::launch_col_pass(nx, last, cc, s));
+            }
+            return MSF_OK;
+        };
+        int64_t k = 0;
+        for (int64_t b0 = 0; b0 < P;) {
+            const int nbA = (int)std::min<int64_t>(B, P - b0);
+            const int64_t b1 = b0 + nbA;
+            msf::RowPassArgs rA, rB;
+            msf::ColPassArgs cA, cB;
+            bind(b0, nbA, 0, rA, cA);
+            if (b1 >= P) { // an odd last batch
+                if (int r = slices_alone(rA, cA, s)) return r;
+                if (int r = finish_batch(k, b0, nbA, 0, s)) return r;
+                break;
+            }
+            const int nbB = (int)std::min<int64_t>(B, P - b1);
+            bind(b1, nbB, 1, rB, cB);
+            row_at(rA, 0);
+            HIPCHK(c, msf::launch_row_pass(ny, true, f.tmode, rA, s));
+            for (int n = 0; n < S; ++n) {
+                row_at(rB, n);
+                col_at(cA, n);
+                if (int r = pair(n == 0, n == S - 1, rB, cA)) return r;
+                col_at(cB, n);
+                if (n + 1 < S) {
+                    row_at(rA, n + 1);
+                    if (int r = pair(false, false, rA, cB)) return r;
+                } else {
+                    HIPCHK(c, msf::launch_col_pass(nx, true, cB, s));
+                }
+            }
+            if (int r = finish_batch(k, b0, nbA, 0, s)) return r;
+            if (int r = finish_batch(k + 1, b1, nbB, 1, s)) return r;
+            k += 2;
+            b0 = b1 + nbB;
+        }
+        if (f.want_d1) HIPCHK(c, msf::launch_fem_finalize(v.state, nbins, v.femv, c->stream));
+        return MSF_OK;
+    }
+    if (chains == 2) { // fork: the second stream starts after everything issued so far
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    }
+    int64_t k = 0;
+    for (int64_t b0 = 0; b0 < P; b0 += B, ++k) {
+        const int nb = (int)std::min<int64_t>(B, P - b0);
+        const int ch = (int)(k % chains);
+        hipStream_t s = streams[ch];
+        msf::RowPassArgs r;
+        msf::ColPassArgs cc;
+        bind(b0, nb, ch, r, cc);
+        if (int e = slices_alone(r, cc, s)) return e;
+        if (int e = finish_batch(k, b0, nb, ch, s)) return e;
     }
     if (chains == 2) { // join: the first stream waits for everything on the second
         HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
@@ -893,15 +974,23 @@ std::vector<uint64_t> graph_signature(const RunPlan& pl, const msf_specimen* sp,
             (uint64_t)pl.nbins, (uint64_t)pl.n_tiles, (uint64_t)sh.col_tile, (uint64_t)sh.plan_v};
 }
 
+// square grids whose slice passes can run as fused pass pairs (chains 3)
+static bool fused_shape(const RunFlags& f, int tile, int plan_v)
+{
+    return f.nx == f.ny && msf::fused_shape_ok(f.nx, (plan_v >> 1) & 1, plan_v & 1, tile) &&
+           msf::row_prunable(f.ny, f.ka, f.kb);
+}
+
 // The autotuner's candidates and trials.  A trial runs the job's own launch
 // sequence (enqueue_job: both slice passes of every slice, the reductions and
-// FEM statistics, with the candidate's chains) over the first two batches
-// per chain of the job's positions on trial buffers, once untimed and once
-// timed; its cost is seconds per position-slice.  Candidates: the rule's
-// batch depth, half and twice it; column tiles 4, 8 and 16; one or two
-// chains; with autotune = 1 and fft_plan = 0 also the all-primary and
-// all-alternate radix plans.  Every candidate of one radix plan gives
-// bit-identical results.
+// FEM statistics, with the candidate's chains) over the first four batches
+// of the job's positions on trial buffers, once untimed and once timed; its
+// cost is seconds per position-slice.  Candidates: the rule's batch depth,
+// half and twice it; column tiles 4, 8 and 16; one chain, two chains, or
+// batch pairs with fused pass launches where the grid has them; with
+// autotune = 1 and fft_plan = 0 also the all-primary and all-alternate radix
+// plans.  Every candidate of one radix plan gives bit-identical results.
+
 int tune_shape(msf_ctx* c, RunPlan& pl, const msf_specimen* sp, const std::vector<int2>& shifts,
                const RunFlags& f, const Twiddles& twx, const Twiddles& twy, int64_t B,
                int plan_rule, int all_alt, TunedShape& pick, int& state)
@@ -931,8 +1020,9 @@ int tune_shape(msf_ctx* c, RunPlan& pl, const msf_specimen* sp, const std::vecto
     for (int pv : pvs)
         for (int64_t b : bs)
             for (int w : {4, 8, 16})
-                for (int ch : {1, 2})
-                    if (ch == 1 || (P + b - 1) / b >= 2) cands.push_back({(int)b, w, ch, pv});
+                for (int ch : {1, 2, 3})
+                    if (ch == 1 || ((P + b - 1) / b >= 2 && (ch == 2 || fused_shape(f, w, pv))))
+                        cands.push_back({(int)b, w, ch, pv});
     RunPlan tb; // trial buffers, released on return
     tb.G = pl.G;
     tb.nbins = pl.nbins;
@@ -953,7 +1043,7 @@ int tune_shape(msf_ctx* c, RunPlan& pl, const msf_specimen* sp, const std::vecto
         ts.chains = cand[2];
         ts.plan_v = cand[3];
         RunFlags ft = f;
-        ft.P = std::min<int64_t>(P, 2 * ts.chains * ts.B);
+        ft.P = std::min<int64_t>(P, 4 * ts.B);
         ft.want_each = false;
         ts.n_batches = (ft.P + ts.B - 1) / ts.B;
         const std::vector<int2> sub(shifts.begin(), shifts.begin() + ft.P);
@@ -1062,7 +1152,7 @@ int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o
                 // A capture can be invalidated from outside, e.g. by another
                 // thread's allocation while it runs; nothing captured has
                 // executed, so one chain simply launches on the stream.
-                if (sh.chains != 1) return r != MSF_OK ? r : fail(c, MSF_ERR_DEVICE, hipGetErrorString(e));
+                if (sh.chains == 2) return r != MSF_OK ? r : fail(c, MSF_ERR_DEVICE, hipGetErrorString(e));
                 (void)hipGetLastError();
                 captured = false;
             } else {
@@ -1071,11 +1161,13 @@ int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o
                 if (ei != hipSuccess) pl.graph = nullptr;
                 HIPCHK(c, ei);
                 pl.graph_sig = sig;
+                pl.graph_fused = prof.fused;
                 graph_state = 1;
             }
         }
         if (captured) {
             HIPCHK(c, hipGraphLaunch(pl.graph, c->stream));
+            prof.fused = pl.graph_fused;
         } else {
             graph_state = 0;
             if (int r = enqueue_job(c, pl, sp, f, sh, v, out, prof)) return r;
@@ -1099,6 +1191,7 @@ int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o
     st.col_pass_bytes = st.row_pass_bytes;
     st.graph = graph_state;
     st.chains = sh.chains;
+    st.fused_launches = prof.fused;
     st.col_tile = sh.col_tile ? sh.col_tile
                               : msf::col_tile_auto(f.nx, sh.plan_v & 1, (int)sh.B, f.ncp);
     st.autotuned = sh.tuned_state;

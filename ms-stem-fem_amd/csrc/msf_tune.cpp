@@ -121,7 +121,7 @@ int msf_tuning_load(const char* path, uint32_t* entries)
             TunedShape s{};
             if (!(v >> s[0] >> s[1] >> s[2] >> s[3])) return MSF_ERR_INVALID;
             if (s[0] < 1 || (s[1] != 1 && s[1] != 4 && s[1] != 8 && s[1] != 16) ||
-                (s[2] != 1 && s[2] != 2) || s[3] < 0 || s[3] > 3)
+                (s[2] < 1 || s[2] > 3) || s[3] < 0 || s[3] > 3)
                 return MSF_ERR_INVALID;
             got[line.substr(0, eq)] = s;
         }

@@ -146,7 +146,7 @@ class Stats(C.Structure):
                 ("col_pass_count", C.c_uint64), ("row_pass_bytes", C.c_double),
                 ("col_pass_bytes", C.c_double), ("graph", C.c_int32), ("chains", C.c_int32),
                 ("col_tile", C.c_int32), ("autotuned", C.c_int32), ("row_fft_plan", C.c_int32),
-                ("col_fft_plan", C.c_int32)]
+                ("col_fft_plan", C.c_int32), ("fused_launches", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

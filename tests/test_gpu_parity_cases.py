@@ -211,6 +211,47 @@ def test_graph_replay_matches_stream_launches(msf, ctx):
         ctx.configure()
 
 
+@pytest.mark.parametrize("n,depth", [(256, 6.0), (384, 6.0), (512, 6.0), (512, 2.0)])
+def test_fused_pass_pairs_match_one_chain(msf, ctx, n, depth):
+    """Batches in pairs, the row pass of one and the column pass of the other
+    in one launch (chains 3), give results bit-identical to one chain: odd
+    and even batch counts, a smaller last batch, batches below one row
+    workgroup's positions (passes launched one after the other), one slice,
+    graph replay and stream launches."""
+    L = 0.05 * n
+    q, Z = random_atoms(int(0.02 * L * L * depth), (L, L, depth), seed=n + int(depth),
+                        species=(14, 29))
+    c = Case(nx=n, ny=n, cell=(L, L, depth), voltage=300e3, q=q, Z=Z, min_slice=2.0,
+             alpha_max=0.02, inner=0.04, outer=0.12, fem=True, dr=0.5)
+    spec = c.gpu_specimen(msf, ctx)
+    xp = np.linspace(0.3, L - 0.7, 8)
+    yp = np.linspace(0.1, L - 0.4, 6)
+
+    def run(**cfg):
+        ctx.configure(autotune=-1, **cfg)
+        out = ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
+                          fem=c.femcfg(msf))
+        return out, ctx.stats()
+
+    try:
+        for b in (16, 12, 20, 5):
+            (img_1, v_1), _ = run(batch=b, chains=1, graph=False)
+            for graph in (False, True, True):
+                (img_3, v_3), st = run(batch=b, chains=3, graph=graph)
+                assert st["chains"] == 3 and st["batch"] == b
+                # positions per row workgroup: 16 at 256 (alternate plan), else 8
+                assert (st["fused_launches"] > 0) == (b >= (16 if n == 256 else 8))
+                np.testing.assert_array_equal(img_3, img_1)
+                np.testing.assert_array_equal(v_3, v_1)
+        # the raster in one batch: one chain
+        (img_a, v_a), st = run(batch=48, chains=3)
+        assert st["chains"] == 1
+        np.testing.assert_array_equal(img_a, img_1)
+        np.testing.assert_array_equal(v_a, v_1)
+    finally:
+        ctx.configure()
+
+
 def test_exit_waves_for_explicit_shifts(msf, ctx):
     """msf_exit_waves_shifted on the first and last raster positions equals
     those positions of a whole-raster msf_exit_waves, bit for bit."""
