@@ -264,7 +264,10 @@ def uneven_batch(args, msf, SP):
     """48 uneven jobs (N 256/384/512): first run one after another on one
     context with the autotuner and graph replay (an untimed pass tunes each
     shape, a second is timed), then submitted together to the serving engine
-    (3 slots); each job builds its own specimen."""
+    (3 slots), once with every job alone and once with grouping (--group:
+    queued jobs of one grid size run together, their slice passes sharing
+    launches); each job builds its own specimen.  `value` is the submission
+    with the engine's defaults (jobs alone); `grouped` the second."""
     grids = [args.grid] if args.grid else None
     log("generating 48 uneven jobs (seeds 3000-3047)" + (f", grid {args.grid} only" if grids else ""))
     jobs = SP.uneven_jobs(48, seed=3000, grids=grids)
@@ -301,21 +304,29 @@ def uneven_batch(args, msf, SP):
               "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
                              "p99": round(1e3 * float(np.percentile(lat, 99)), 2)}}
     log(f"direct: {direct['props_per_s']:.4g} props/s")
-    # one batched submission through the serving engine
-    ctx = msf.Context(0)
-    try:
-        ctx.serve_configure(slots=3)
-        t0 = time.perf_counter()
-        tickets = [ctx.submit(g, j["q"], j["Z"], j["min_slice"], o, j["x_p"], j["y_p"], detector=d)
-                   for j, (g, o, d) in zip(jobs, prep)]
-        slat = []
-        for t in tickets:
-            _, _, info = ctx.wait(t, info=True)
-            slat.append(info["queue_ms"] + info["run_ms"])
-        span = time.perf_counter() - t0
-        sst = ctx.serve_stats()
-    finally:
-        ctx.close()
+    # one batched submission through the serving engine, jobs alone and then
+    # grouped (queued jobs of one grid size share slice-pass launches)
+    def served(group):
+        ctx = msf.Context(0)
+        try:
+            ctx.serve_configure(slots=3)
+            ctx.serve_group(group)
+            t0 = time.perf_counter()
+            tickets = [ctx.submit(g, j["q"], j["Z"], j["min_slice"], o, j["x_p"], j["y_p"],
+                                  detector=d) for j, (g, o, d) in zip(jobs, prep)]
+            slat = []
+            for t in tickets:
+                _, _, info = ctx.wait(t, info=True)
+                slat.append(info["queue_ms"] + info["run_ms"])
+            return time.perf_counter() - t0, slat, ctx.serve_stats()
+        finally:
+            ctx.close()
+
+    span1, slat1, sst1 = served(0)
+    log(f"served alone: {work / span1:.4g} props/s")
+    span, slat, sst = served(args.group)
+    log(f"served grouped ({args.group}): {work / span:.4g} props/s, {sst['groups']} groups, "
+        f"{sst['grouped_jobs']} jobs, {sst['group_launches']} launches")
     Ns = [j["nx"] for j in jobs]
     config = {"workload": "uneven_stem_48: 48 Si jobs, N 256/384/512, cubic 20-50 A at 0.05 A^-3, "
                           "minslice 2.0, rasters 16-48 per side, ADF 50-180 mrad, 100 kV, 20 mrad",
@@ -323,12 +334,18 @@ def uneven_batch(args, msf, SP):
               "atoms": int(sum(j["Z"].size for j in jobs)), "slices": int(sum(slices)),
               "positions": int(sum(j["x_p"].size * j["y_p"].size for j in jobs)),
               "probe_slices": int(work), "slots": 3, "seed": 3000}
-    return _base_line(work / span, "probe-slice propagations/s", 1, 0, 1e3 * span, config, None,
-                      {"props_512eq_per_s": round(work512 / span, 1),
-                       "latency_ms": {"p50": round(float(np.percentile(slat, 50)), 1),
-                                      "p99": round(float(np.percentile(slat, 99)), 1)},
-                       "max_running": sst["max_running"],
-                       "peak_admitted_gib": round(sst["peak_admitted_bytes"] / 2 ** 30, 2),
+    pct = lambda v, q: round(float(np.percentile(v, q)), 1)
+    return _base_line(work / span1, "probe-slice propagations/s", 1, 0, 1e3 * span1, config, None,
+                      {"props_512eq_per_s": round(work512 / span1, 1),
+                       "latency_ms": {"p50": pct(slat1, 50), "p99": pct(slat1, 99)},
+                       "max_running": sst1["max_running"],
+                       "peak_admitted_gib": round(sst1["peak_admitted_bytes"] / 2 ** 30, 2),
+                       "grouped": {"max_jobs": args.group, "props_per_s": round(work / span, 1),
+                                   "makespan_s": round(span, 3),
+                                   "latency_ms": {"p50": pct(slat, 50), "p99": pct(slat, 99)},
+                                   "groups": sst["groups"], "grouped_jobs": sst["grouped_jobs"],
+                                   "group_launches": sst["group_launches"],
+                                   "max_running": sst["max_running"]},
                        "direct_autotuned_graph": direct})
 
 
@@ -431,6 +448,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--autotune", type=int, default=None, choices=[-1, 0, 1],
                     help="msf_config.autotune (-1: the fixed rules, as for counter profiles)")
+    ap.add_argument("--group", type=int, default=16,
+                    help="uneven48: queued jobs of one grid size a served group may hold")
     ap.add_argument("--grid", type=int, default=0, choices=[0, 256, 384, 512],
                     help="uneven48: only the jobs of this grid (0: all 48)")
     ap.add_argument("--fft-plan", type=int, default=0,

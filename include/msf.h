@@ -150,6 +150,9 @@ typedef struct msf_stats {
     int32_t col_fft_plan;              /* of the column passes (nx): 1 primary, 2 alternate */
     uint64_t fused_launches;           /* launches that ran a row pass and a column pass
                                           together (chains 3) */
+    uint32_t group_runs;               /* msf_run_group: runs that took part */
+    uint64_t group_launches;           /* msf_run_group: slice-pass launches, each over the
+                                          current batches of up to 16 runs */
 } msf_stats;
 
 /* ------------------------------------------------------------ context */
@@ -428,6 +431,18 @@ typedef struct msf_run_job {
  * them, instead of per job; msf_specimen_free waits for those jobs. */
 int msf_submit_run(msf_ctx* ctx, const msf_run_job* job, uint64_t* id);
 
+/* No reference counterpart (the reference runs one job per process).  Runs
+ * n_runs runs on specimens built on ctx whose grids share nx and ny
+ * together: each slice step launches the row pass once over the current
+ * batches of up to 16 runs, then the column pass ("grouped launches"), while
+ * every run keeps its own specimen, vacuum waves, pixel groups, buffers and
+ * raster-order statistics.  Each run's outputs equal those of
+ * msf_run_stem_fem on it alone (autotune 0 or -1) bit for bit.  status
+ * (n_runs entries, may be NULL) gets each run's code; returns MSF_OK when
+ * every run succeeded, else the first failing run's code with its message in
+ * msf_last_error.  msf_get_stats reports group_runs and group_launches. */
+int msf_run_group(msf_ctx* ctx, const msf_run_job* runs, uint32_t n_runs, int32_t* status);
+
 /* MSF_OK or the job's error code once finished, MSF_PENDING while queued or
  * running, MSF_ERR_INVALID for an unknown id */
 int msf_poll(msf_ctx* ctx, uint64_t id);
@@ -465,8 +480,19 @@ typedef struct msf_serve_stats {
     uint64_t specimen_builds;          /* specimens built for msf_submit jobs so far */
     uint64_t resident_bytes;           /* stacks of the resident specimens queued or running
                                           msf_submit_run jobs refer to (counted once each) */
+    uint64_t groups;                   /* job groups run with shared launches (msf_serve_group) */
+    uint64_t grouped_jobs;             /* jobs that ran in those groups */
+    uint64_t group_launches;           /* their grouped slice-pass launches */
 } msf_serve_stats;
 int msf_serve_get_stats(msf_ctx* ctx, msf_serve_stats* out);
+
+/* No reference counterpart.  Grouping of served jobs, before the first
+ * msf_submit on the context: a worker that takes the head job of the queue
+ * also takes up to max_jobs - 1 further queued jobs of the same nx, ny and
+ * settings that fit the memory limit beside it, and runs them as one group
+ * (msf_run_group: shared slice-pass launches, each job's results as alone).
+ * 0 or 1 (the default): every job runs alone; at most 16. */
+int msf_serve_group(msf_ctx* ctx, uint32_t max_jobs);
 
 /* The admission estimate of a job's device footprint in bytes (specimen
  * stacks, vacuum waves, batch buffers, set-up temporaries, +10 %); host only. */

@@ -65,6 +65,33 @@ bool fused_supported(int n, const RowPassArgs& ra, const ColPassArgs& ca);
 // column tile `tile` (0: the plan's default)
 bool fused_shape_ok(int n, int vr, int vc, int tile);
 
+// Grouped launches: one launch runs the same pass over the batches of up to
+// kGroupMax runs of one transform length (each its own specimen, vacuum
+// waves, kept columns and buffers).  Entry k owns launch blocks
+// [off[k], off[k+1]), its first block a multiple of 8 (XCD banding within
+// the entry); blocks past an entry's own count return at once.
+constexpr int kGroupMax = 16;
+struct RowGroupArgs {
+    int n;
+    int off[kGroupMax + 1];
+    RowPassArgs a[kGroupMax];
+};
+struct ColGroupArgs {
+    int n;
+    int off[kGroupMax + 1];
+    ColPassArgs a[kGroupMax];
+};
+// blocks of one entry (before the padding to a multiple of 8)
+int row_group_blocks(int ny, const RowPassArgs& a);
+int col_group_blocks(int nx, const ColPassArgs& a);
+// Every entry: same length, plan, prune flag and (column pass) tile; the
+// rows' FIRST and the columns' LAST flags shared by the launch.
+hipError_t launch_row_group(int ny, bool first, int tmode, const RowGroupArgs& g, hipStream_t s);
+hipError_t launch_col_group(int nx, bool last, const ColGroupArgs& g, hipStream_t s);
+// the column tile a grouped column pass uses with a configured tile `tile`
+// (0: the plan's default)
+int col_group_tile(int nx, int v, int tile);
+
 bool slice_lengths_ok(int n);
 // concatenated per-stage twiddle tables of plan v (0 primary, 1 alternate) of
 // length n, (re, im) FP64

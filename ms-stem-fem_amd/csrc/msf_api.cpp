@@ -815,22 +815,7 @@ int msf_run_stem_fem(msf_ctx* c, const msf_specimen* sp, const msf_optics* o,
         std::vector<int2> shifts;
         if (!raster_shifts(sp->dom, x_p, n_xp, y_p, n_yp, shifts))
             return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
-        RunOut out;
-        out.adf = adf_image;
-        if (fo) {
-            out.fem = fo->v_omega;
-            out.vbar_r = fo->vbar_r;
-            out.v_re = fo->v_re;
-            out.omega_vimage = fo->omega_vimage;
-            out.v_gt17 = fo->v_gt17;
-            out.v_rbar = fo->v_rbar;
-            out.corr_avg = fo->correlograph_avg;
-            out.corr_sqr = fo->correlograph_sqr_sum;
-            out.corr_each = fo->correlograph_each;
-            out.mean_abs = fo->diffraction_mean_abs;
-            out.mean_sqr = fo->diffraction_mean_sqr;
-            out.var2d = fo->variance_2d;
-        }
+        const RunOut out = run_out(adf_image, fo);
         Mem heap;
         return run_internal(c, heap, sp, o, det, fem, shifts, out);
     }

@@ -217,6 +217,10 @@ struct ServeState {
     std::map<uint64_t, Resident> resident;
     uint64_t resident_bytes = 0;
     uint64_t specimen_builds = 0;
+    // grouping (msf_serve_group): a worker takes up to group_max queued jobs
+    // of one grid size and settings and runs them with shared launches
+    uint32_t group_max = 1;
+    uint64_t groups = 0, grouped_jobs = 0, group_launches = 0;
 };
 
 // Launch shape of a run: positions per batch, column-pass tile, batch
@@ -234,6 +238,7 @@ struct msf_ctx {
     std::unique_ptr<msf_impl::ServeState> serve;
     uint32_t serve_slots = 0;   // msf_serve_configure (0: automatic)
     uint64_t serve_limit = 0;
+    uint32_t serve_group = 0;   // msf_serve_group (0, 1: each job alone)
     int device = 0;
     hipStream_t stream = nullptr;
     // second stream and the events that order two batch chains (created on first use)
@@ -247,6 +252,7 @@ struct msf_ctx {
     std::string arch;           // device key of the tuning table
     msf_impl::RunPlan plan;     // ordinary runs
     msf_impl::RunPlan plan_na;  // the allocation-free run (device views rebuilt per call)
+    std::vector<std::unique_ptr<msf_impl::RunPlan>> gplans; // runs of a group, one each
     char* scratch = nullptr;    // this device's copy of the allocation-free run's array
     msf_ctx* owner = nullptr;   // serving slot: the context whose specimens it may run on
     std::set<msf_specimen*> specimens; // live specimens built on this context
@@ -373,11 +379,151 @@ struct RunOut {
     float* exit = nullptr;
 };
 
+// What one run computes, from its arguments.
+struct RunFlags {
+    bool want_adf = false, fem_on = false, want_d1 = false, want_d2 = false, want_pix = false,
+         want_corr = false, want_fem = false, want_mtf = false, want_each = false;
+    int tmode = 0;
+    int64_t P = 0;
+    int nx = 0, ny = 0, S = 0, ka = 0, kb = 0, ncp = 0;
+    int64_t M = 0, plane = 0;
+    int nphi = 0;
+    size_t ncorr = 1;
+};
+
+// Launch shape of one run.
+struct Shape {
+    int64_t B = 1, n_batches = 1;
+    int col_tile = 0;        // configured tile, 0: the plan's default
+    int chains = 1;
+    int plan_v = 0;          // radix plans, vx + 2 vy
+    int tuned_state = 0;
+    bool auto_eager = false; // two automatic chains launch on the streams
+};
+
+// Device addresses of one run's buffers.
+struct RunViews {
+    float2* psi[2] = {nullptr, nullptr};
+    double* part[2] = {nullptr, nullptr};
+    double* segp[2] = {nullptr, nullptr};
+    double2* mtf_full[2] = {nullptr, nullptr};
+    double *state = nullptr, *femv = nullptr, *adf = nullptr, *vsum = nullptr, *pm1 = nullptr,
+           *pm2 = nullptr, *csum = nullptr, *csqr = nullptr, *ceach = nullptr;
+    int2* shift = nullptr;
+    float2* exit = nullptr;
+    Twiddles twx, twy;
+};
+
+
+// Per-kernel timing with HIP events (profile_kernels).
+enum { K_ROW0, K_ROW, K_COL, K_COLL, K_RED, K_NUM };
+struct Prof {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    hipEvent_t ka0 = nullptr, ka1 = nullptr; // pair stamped by a slice pass's own dispatch
+    uint64_t fused = 0;                      // fused pass launches enqueued
+    ~Prof()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t begin(int k)
+    {
+        if (!on) return hipSuccess;
+        hipEvent_t a, b;
+        hipError_t e = hipEventCreate(&a);
+        if (e == hipSuccess) e = hipEventCreate(&b);
+        if (e != hipSuccess) return e;
+        ev.push_back(a);
+        ev.push_back(b);
+        kind.push_back(k);
+        return hipEventRecord(a, s);
+    }
+    hipError_t end() { return on ? hipEventRecord(ev.back(), s) : hipSuccess; }
+    hipError_t kernel(int k)
+    {
+        ka0 = ka1 = nullptr;
+        if (!on) return hipSuccess;
+        hipError_t e = hipEventCreate(&ka0);
+        if (e == hipSuccess) e = hipEventCreate(&ka1);
+        if (e != hipSuccess) return e;
+        ev.push_back(ka0);
+        ev.push_back(ka1);
+        kind.push_back(k);
+        return hipSuccess;
+    }
+};
+
+// Arguments of both slice passes of one run before a batch and slice are
+// bound (msf_run.cpp).
+struct PassArgs {
+    msf::RowPassArgs ra{};
+    msf::ColPassArgs ca{};
+    const float2 *t32 = nullptr, *px = nullptr, *py = nullptr, *u0 = nullptr;
+    int64_t M = 0;
+    int nx = 0, ny = 0;
+    // a batch of nb positions in wave buffer psi, shifts from `shift`
+    void bind(float2* psi, const int2* shift, int nb, msf::RowPassArgs& r,
+              msf::ColPassArgs& c) const;
+    void row_at(msf::RowPassArgs& r, int n) const; // transmission, vacuum wave of slice n
+    void col_at(msf::ColPassArgs& c, int n) const; // propagators of slice n
+};
+
+// ---- the units of a raster run (msf_run.cpp), in the order a run calls them
+int check_run(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const msf_detector* det,
+              const msf_fem* fem, const std::vector<int2>& shifts, const RunOut& out, RunFlags& f);
+int prepare_groups(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, const msf_optics* o,
+                   const msf_detector* det, const msf_fem* fem, RunFlags& f);
+int prepare_vacuum(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, const msf_optics* o,
+                   const RunFlags& f);
+// positions per batch by the rule: the fewest batches whose wave buffer stays
+// within 96 MiB, in whole groups of 16 positions
+int64_t batch_rule(const RunFlags& f);
+int prepare_buffers(msf_ctx* c, Mem& m, RunPlan& pl, const RunFlags& f, const Shape& sh,
+                    const std::vector<int2>& shifts, const RunOut& out, RunViews& v);
+PassArgs pass_args(const RunPlan& pl, const msf_specimen* sp, const RunFlags& f, const Shape& sh,
+                   const RunViews& v);
+// everything after the slice loop of batch k (positions b0 .. b0+nb-1, wave
+// buffer `ch`): reductions, ADF, MTF, FEM statistics (after batch k-1's when
+// chains == 2), exit waves
+int enqueue_batch_tail(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const RunFlags& f,
+                       const RunViews& v, const RunOut& out, Prof& prof, int chains, int64_t k,
+                       int64_t b0, int nb, int ch, hipStream_t s);
+int finish_outputs(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const msf_optics* o,
+                   const RunFlags& f, const RunViews& v, const RunOut& out);
+
 // The raster run behind msf_run_stem, msf_run_stem_fem, msf_exit_waves(_shifted)
 // and the allocation-free run (m.scratch()).
 int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o,
                  const msf_detector* det, const msf_fem* fem, const std::vector<int2>& shifts,
                  RunOut out);
+
+// ---- grouped runs (msf_group.cpp)
+// the caller buffers of msf_run_stem_fem's arguments
+RunOut run_out(double* adf, const msf_fem_outputs* fo);
+// One run of a group: arguments, outputs and, after run_group, its status.
+struct GroupRun {
+    const msf_specimen* sp = nullptr;
+    const msf_optics* o = nullptr;
+    const msf_detector* det = nullptr;
+    const msf_fem* fem = nullptr;
+    std::vector<int2> shifts;
+    RunOut out;
+    int rc = MSF_OK;
+    std::string err;
+};
+struct GroupInfo {
+    uint32_t runs = 0;     // runs that took part
+    uint64_t launches = 0; // grouped slice-pass launches
+    uint64_t steps = 0;    // slice steps (a row and a column launch per flag class)
+};
+// Runs specimens of one grid size together, their slice passes sharing
+// launches; each run's rc/err tells its outcome.  Returns MSF_OK unless a
+// device error ended the group.
+int run_group(msf_ctx* c, std::vector<GroupRun>& runs, GroupInfo* info);
+// frees the work buffers of the group's run plans (pixel groups stay)
+void release_group_plans(msf_ctx* c);
 
 // ---- serving (msf_serve.cpp)
 // Blocks until no queued or running served job refers to the specimen.

@@ -239,8 +239,6 @@ bool raster_shifts(const msf::Domain& d, const double* x_p, uint32_t n_xp, const
     return true;
 }
 
-namespace {
-
 // V_re, Omega_vimage, V_gt17 and V_rbar from the per-pixel sums of |psi|
 // (s1) and |psi|^2 (s2) over P positions, on the compact plane
 // (adfstem.cpp:2942-2995 averages; :3169-3466 ring ratios; :3468-3605 the
@@ -310,41 +308,6 @@ void fem_pixel_images(const msf::Domain& d, const msf::RowRanges& keep, int ncp,
             }
         }
 }
-
-// What one run computes, from its arguments.
-struct RunFlags {
-    bool want_adf = false, fem_on = false, want_d1 = false, want_d2 = false, want_pix = false,
-         want_corr = false, want_fem = false, want_mtf = false, want_each = false;
-    int tmode = 0;
-    int64_t P = 0;
-    int nx = 0, ny = 0, S = 0, ka = 0, kb = 0, ncp = 0;
-    int64_t M = 0, plane = 0;
-    int nphi = 0;
-    size_t ncorr = 1;
-};
-
-// Launch shape of one run.
-struct Shape {
-    int64_t B = 1, n_batches = 1;
-    int col_tile = 0;        // configured tile, 0: the plan's default
-    int chains = 1;
-    int plan_v = 0;          // radix plans, vx + 2 vy
-    int tuned_state = 0;
-    bool auto_eager = false; // two automatic chains launch on the streams
-};
-
-// Device addresses of one run's buffers.
-struct RunViews {
-    float2* psi[2] = {nullptr, nullptr};
-    double* part[2] = {nullptr, nullptr};
-    double* segp[2] = {nullptr, nullptr};
-    double2* mtf_full[2] = {nullptr, nullptr};
-    double *state = nullptr, *femv = nullptr, *adf = nullptr, *vsum = nullptr, *pm1 = nullptr,
-           *pm2 = nullptr, *csum = nullptr, *csqr = nullptr, *ceach = nullptr;
-    int2* shift = nullptr;
-    float2* exit = nullptr;
-    Twiddles twx, twy;
-};
 
 int check_run(msf_ctx* c, const msf_specimen* sp, const msf_optics* o, const msf_detector* det,
               const msf_fem* fem, const std::vector<int2>& shifts, const RunOut& out, RunFlags& f)
@@ -565,6 +528,18 @@ int tune_shape(msf_ctx* c, RunPlan& pl, const msf_specimen* sp, const std::vecto
                const RunFlags& f, const Twiddles& twx, const Twiddles& twy, int64_t B,
                int plan_rule, int all_alt, TunedShape& pick, int& state);
 
+int64_t batch_rule(const RunFlags& f)
+{
+    const int64_t P = f.P;
+    const int64_t per_pos = f.plane * (int64_t)sizeof(float2);
+    int64_t bmax = std::max<int64_t>(1, (96ll << 20) / per_pos);
+    if (bmax >= 32) bmax -= bmax % 16; // whole groups of 16 positions per launch
+    const int64_t nbat = (P + bmax - 1) / bmax;
+    int64_t B = (P + nbat - 1) / nbat;
+    if (B > 16 && B % 16) B = std::min<int64_t>(bmax, B + 16 - B % 16);
+    return std::max<int64_t>(1, std::min<int64_t>(B, P));
+}
+
 // Batch depth, column tile, chains and radix plans: configured, by the rules
 // below or by the autotuner.
 int launch_shape(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp,
@@ -584,15 +559,7 @@ int launch_shape(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp,
     // partial sums then stay resident in the 256 MB Infinity Cache across the
     // two passes of a slice (512^2: 64 positions per batch, measured best of
     // 64..256).
-    int64_t B = c->cfg.batch;
-    if (B == 0) {
-        const int64_t per_pos = f.plane * (int64_t)sizeof(float2);
-        int64_t bmax = std::max<int64_t>(1, (96ll << 20) / per_pos);
-        if (bmax >= 32) bmax -= bmax % 16; // whole groups of 16 positions per launch
-        const int64_t nbat = (P + bmax - 1) / bmax;
-        B = (P + nbat - 1) / nbat;
-        if (B > 16 && B % 16) B = std::min<int64_t>(bmax, B + 16 - B % 16);
-    }
+    int64_t B = c->cfg.batch ? (int64_t)c->cfg.batch : batch_rule(f);
     B = std::max<int64_t>(1, std::min<int64_t>(B, P));
     sh.col_tile = c->cfg.col_tile;
     int chains_cfg = c->cfg.chains;
@@ -722,69 +689,25 @@ int prepare_buffers(msf_ctx* c, Mem& m, RunPlan& pl, const RunFlags& f, const Sh
     return MSF_OK;
 }
 
-// Per-kernel timing with HIP events (profile_kernels).
-enum { K_ROW0, K_ROW, K_COL, K_COLL, K_RED, K_NUM };
-struct Prof {
-    bool on = false;
-    hipStream_t s = nullptr;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> kind;
-    hipEvent_t ka0 = nullptr, ka1 = nullptr; // pair stamped by a slice pass's own dispatch
-    uint64_t fused = 0;                      // fused pass launches enqueued
-    ~Prof()
-    {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    }
-    hipError_t begin(int k)
-    {
-        if (!on) return hipSuccess;
-        hipEvent_t a, b;
-        hipError_t e = hipEventCreate(&a);
-        if (e == hipSuccess) e = hipEventCreate(&b);
-        if (e != hipSuccess) return e;
-        ev.push_back(a);
-        ev.push_back(b);
-        kind.push_back(k);
-        return hipEventRecord(a, s);
-    }
-    hipError_t end() { return on ? hipEventRecord(ev.back(), s) : hipSuccess; }
-    hipError_t kernel(int k)
-    {
-        ka0 = ka1 = nullptr;
-        if (!on) return hipSuccess;
-        hipError_t e = hipEventCreate(&ka0);
-        if (e == hipSuccess) e = hipEventCreate(&ka1);
-        if (e != hipSuccess) return e;
-        ev.push_back(ka0);
-        ev.push_back(ka1);
-        kind.push_back(k);
-        return hipSuccess;
-    }
-};
-
-// Every launch of the job (all batches, reductions, FEM statistics), issued
-// on the streams in raster order.
-int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const RunFlags& f,
-                const Shape& sh, const RunViews& v, const RunOut& out, Prof& prof)
+PassArgs pass_args(const RunPlan& pl, const msf_specimen* sp, const RunFlags& f, const Shape& sh,
+                   const RunViews& v)
 {
     const msf::Domain& d = sp->dom;
-    const int nx = f.nx, ny = f.ny, S = f.S, ka = f.ka, kb = f.kb, ncp = f.ncp;
-    const int64_t M = f.M, plane = f.plane, P = f.P, B = sh.B;
-    const int G = pl.G, nbins = pl.nbins, nphi = f.nphi, chains = sh.chains;
-    msf::RowPassArgs ra{};
-    ra.nx = nx;
-    ra.ncp = ncp;
-    ra.ka = ka;
-    ra.kb = kb;
+    PassArgs p;
+    msf::RowPassArgs& ra = p.ra;
+    ra.nx = f.nx;
+    ra.ncp = f.ncp;
+    ra.ka = f.ka;
+    ra.kb = f.kb;
     ra.plan = (sh.plan_v >> 1) & 1;
     ra.tw = ra.plan ? v.twy.alt : v.twy.f32;
     ra.inv_s = (float)(1.0 / d.sqrt_nxny);
-    ra.prune = msf::row_prunable(ny, ka, kb) ? 1 : 0;
-    msf::ColPassArgs ca{};
-    ca.ny = ny;
-    ca.ncp = ncp;
-    ca.ka = ka;
-    ca.kb = kb;
+    ra.prune = msf::row_prunable(f.ny, f.ka, f.kb) ? 1 : 0;
+    msf::ColPassArgs& ca = p.ca;
+    ca.ny = f.ny;
+    ca.ncp = f.ncp;
+    ca.ka = f.ka;
+    ca.kb = f.kb;
     ca.colkeep = sp->dcolkeep.as<int2>();
     ca.plan = sh.plan_v & 1;
     ca.tw = ca.plan ? v.twx.alt : v.twx.f32;
@@ -792,75 +715,113 @@ int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const Run
     ca.twx_nat = v.twx.nat;
     ca.twy = v.twy.nat;
     ca.tile = sh.col_tile;
-    ca.prune = msf::col_prunable(nx, sp->keep.ab.data()) ? 1 : 0;
-    const float2* t32 = sp->t32.as<float2>();
-    const float2* px = sp->px.as<float2>();
-    const float2* py = sp->py.as<float2>();
-    hipStream_t streams[2] = {c->stream, c->stream2};
-    auto bind = [&](int64_t b0, int nb, int ch, msf::RowPassArgs& r, msf::ColPassArgs& cc) {
-        r = ra;
-        cc = ca;
-        r.psi = cc.psi = v.psi[ch];
-        r.batch = cc.batch = nb;
-        r.shift = cc.shift = v.shift + b0;
-    };
-    auto row_at = [&](msf::RowPassArgs& r, int n) {
-        r.t = t32 + (int64_t)n * M;
-        r.u0 = pl.u0 + (int64_t)n * 2 * M;
-    };
-    auto col_at = [&](msf::ColPassArgs& cc, int n) {
-        cc.px = px + (int64_t)n * nx;
-        cc.py = py + (int64_t)n * ny;
-    };
-    // everything after the slice loop of batch k (positions b0.., chain ch)
-    auto finish_batch = [&](int64_t k, int64_t b0, int nb, int ch, hipStream_t s) -> int {
-        float2* bpsi = v.psi[ch];
-        double* bpart = v.part[ch];
-        double* bsegp = v.segp[ch];
-        HIPCHK(c, prof.begin(K_RED));
-        HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off, pl.seg_kind,
-                                              pl.n_tiles, nb, bsegp, s));
+    ca.prune = msf::col_prunable(f.nx, sp->keep.ab.data()) ? 1 : 0;
+    p.t32 = sp->t32.as<float2>();
+    p.px = sp->px.as<float2>();
+    p.py = sp->py.as<float2>();
+    p.u0 = pl.u0;
+    p.M = f.M;
+    p.nx = f.nx;
+    p.ny = f.ny;
+    return p;
+}
+
+void PassArgs::bind(float2* psi, const int2* shift, int nb, msf::RowPassArgs& r,
+                    msf::ColPassArgs& c) const
+{
+    r = ra;
+    c = ca;
+    r.psi = c.psi = psi;
+    r.batch = c.batch = nb;
+    r.shift = c.shift = shift;
+}
+
+void PassArgs::row_at(msf::RowPassArgs& r, int n) const
+{
+    r.t = t32 + (int64_t)n * M;
+    r.u0 = u0 + (int64_t)n * 2 * M;
+}
+
+void PassArgs::col_at(msf::ColPassArgs& c, int n) const
+{
+    c.px = px + (int64_t)n * nx;
+    c.py = py + (int64_t)n * ny;
+}
+
+int enqueue_batch_tail(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const RunFlags& f,
+                       const RunViews& v, const RunOut& out, Prof& prof, int chains, int64_t k,
+                       int64_t b0, int nb, int ch, hipStream_t s)
+{
+    const msf::Domain& d = sp->dom;
+    const int nx = f.nx, ny = f.ny, ka = f.ka, kb = f.kb, ncp = f.ncp;
+    const int64_t M = f.M, plane = f.plane;
+    const int G = pl.G, nbins = pl.nbins, nphi = f.nphi;
+    float2* bpsi = v.psi[ch];
+    double* bpart = v.part[ch];
+    double* bsegp = v.segp[ch];
+    HIPCHK(c, prof.begin(K_RED));
+    HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off, pl.seg_kind,
+                                          pl.n_tiles, nb, bsegp, s));
+    HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64, pl.seg_of_group, G, nb,
+                                           bpart, s));
+    HIPCHK(c, prof.end());
+    if (f.want_adf) HIPCHK(c, msf::launch_collect_adf(bpart, G, nb, v.adf + b0, s));
+    if (f.want_mtf) {
+        // adfstem.cpp:2272-2385 after the ADF sum: |psi| outside the
+        // aperture, BACKWARD FFT, x MTF / s (in-place update), FORWARD
+        // FFT, / s, bw_limit; then the group sums again for the FEM
+        double2* full = v.mtf_full[ch];
+        HIPCHK(c, msf::launch_mtf_magnitude(bpsi, nx, ny, ncp, ka, kb, pl.mtf_out, nb, full, s));
+        HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, true, v.twx.f64, v.twy.f64, s));
+        HIPCHK(c, msf::launch_transmit_f64(full, pl.mtf2d, M, nb, 0, d.sqrt_nxny, s));
+        HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, false, v.twx.f64, v.twy.f64, s));
+        HIPCHK(c, msf::launch_mtf_compact(full, nx, ny, ncp, ka, kb, sp->dkeep.as<int2>(),
+                                          d.sqrt_nxny, nb, bpsi, s));
+        HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off,
+                                              pl.seg_kind, pl.n_tiles, nb, bsegp, s));
         HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64, pl.seg_of_group, G, nb,
                                                bpart, s));
-        HIPCHK(c, prof.end());
-        if (f.want_adf) HIPCHK(c, msf::launch_collect_adf(bpart, G, nb, v.adf + b0, s));
-        if (f.want_mtf) {
-            // adfstem.cpp:2272-2385 after the ADF sum: |psi| outside the
-            // aperture, BACKWARD FFT, x MTF / s (in-place update), FORWARD
-            // FFT, / s, bw_limit; then the group sums again for the FEM
-            double2* full = v.mtf_full[ch];
-            HIPCHK(c, msf::launch_mtf_magnitude(bpsi, nx, ny, ncp, ka, kb, pl.mtf_out, nb, full, s));
-            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, true, v.twx.f64, v.twy.f64, s));
-            HIPCHK(c, msf::launch_transmit_f64(full, pl.mtf2d, M, nb, 0, d.sqrt_nxny, s));
-            HIPCHK(c, msf::fft2d_f64(full, nx, ny, nb, false, v.twx.f64, v.twy.f64, s));
-            HIPCHK(c, msf::launch_mtf_compact(full, nx, ny, ncp, ka, kb, sp->dkeep.as<int2>(),
-                                              d.sqrt_nxny, nb, bpsi, s));
-            HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off,
-                                                  pl.seg_kind, pl.n_tiles, nb, bsegp, s));
-            HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64, pl.seg_of_group, G, nb,
-                                                   bpart, s));
-        }
-        // the FEM accumulators are shared: batch k updates them after batch k-1
-        if (chains == 2 && k > 0) HIPCHK(c, hipStreamWaitEvent(s, c->ev_fem[(k - 1) % 2], 0));
-        if (f.want_d1)
-            HIPCHK(c, msf::launch_fem_welford(bpart, G, nb, nbins, pl.cnt, pl.zero, v.state, s));
-        if (f.want_d2)
-            HIPCHK(c, msf::launch_fem_vbar(bpart, G, nb, nbins, pl.hg.fem_abs, pl.hg.fem_sqr, pl.cnt,
-                                           pl.zero, v.vsum, s));
-        if (f.want_corr)
-            HIPCHK(c, msf::launch_fem_correlograph(
-                          bpart, G, nb, nbins, nphi, pl.hg.fem_abs, pl.hg.corr, pl.cnt, pl.zero,
-                          v.csum, v.csqr, f.want_each ? v.ceach + b0 * nbins * nphi : nullptr,
-                          s));
-        if (f.want_pix) HIPCHK(c, msf::launch_pixel_moments(bpsi, plane, nb, v.pm1, v.pm2, s));
-        if (chains == 2) HIPCHK(c, hipEventRecord(c->ev_fem[k % 2], s));
-        if (out.exit) {
-            HIPCHK(c, msf::launch_expand_exit(bpsi, nx, ny, ncp, ka, kb, nb, v.exit, s));
-            HIPCHK(c, hipMemcpyAsync(out.exit + (size_t)b0 * nx * ny * 2, v.exit,
-                                     (size_t)nb * nx * ny * sizeof(float2), hipMemcpyDeviceToHost,
-                                     s));
-        }
-        return MSF_OK;
+    }
+    // the FEM accumulators are shared: batch k updates them after batch k-1
+    if (chains == 2 && k > 0) HIPCHK(c, hipStreamWaitEvent(s, c->ev_fem[(k - 1) % 2], 0));
+    if (f.want_d1)
+        HIPCHK(c, msf::launch_fem_welford(bpart, G, nb, nbins, pl.cnt, pl.zero, v.state, s));
+    if (f.want_d2)
+        HIPCHK(c, msf::launch_fem_vbar(bpart, G, nb, nbins, pl.hg.fem_abs, pl.hg.fem_sqr, pl.cnt,
+                                       pl.zero, v.vsum, s));
+    if (f.want_corr)
+        HIPCHK(c, msf::launch_fem_correlograph(
+                      bpart, G, nb, nbins, nphi, pl.hg.fem_abs, pl.hg.corr, pl.cnt, pl.zero,
+                      v.csum, v.csqr, f.want_each ? v.ceach + b0 * nbins * nphi : nullptr,
+                      s));
+    if (f.want_pix) HIPCHK(c, msf::launch_pixel_moments(bpsi, plane, nb, v.pm1, v.pm2, s));
+    if (chains == 2) HIPCHK(c, hipEventRecord(c->ev_fem[k % 2], s));
+    if (out.exit) {
+        HIPCHK(c, msf::launch_expand_exit(bpsi, nx, ny, ncp, ka, kb, nb, v.exit, s));
+        HIPCHK(c, hipMemcpyAsync(out.exit + (size_t)b0 * nx * ny * 2, v.exit,
+                                 (size_t)nb * nx * ny * sizeof(float2), hipMemcpyDeviceToHost,
+                                 s));
+    }
+    return MSF_OK;
+}
+
+// Every launch of the job (all batches, reductions, FEM statistics), issued
+// on the streams in raster order.
+int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const RunFlags& f,
+                const Shape& sh, const RunViews& v, const RunOut& out, Prof& prof)
+{
+    const int nx = f.nx, ny = f.ny, S = f.S;
+    const int64_t P = f.P, B = sh.B;
+    const int chains = sh.chains;
+    const PassArgs pa = pass_args(pl, sp, f, sh, v);
+    hipStream_t streams[2] = {c->stream, c->stream2};
+    auto bind = [&](int64_t b0, int nb, int ch, msf::RowPassArgs& r, msf::ColPassArgs& cc) {
+        pa.bind(v.psi[ch], v.shift + b0, nb, r, cc);
+    };
+    auto row_at = [&](msf::RowPassArgs& r, int n) { pa.row_at(r, n); };
+    auto col_at = [&](msf::ColPassArgs& cc, int n) { pa.col_at(cc, n); };
+    auto finish_batch = [&](int64_t k, int64_t b0, int nb, int ch, hipStream_t s) -> int {
+        return enqueue_batch_tail(c, pl, sp, f, v, out, prof, chains, k, b0, nb, ch, s);
     };
     // the slice loop of one batch alone
     auto slices_alone = [&](msf::RowPassArgs& r, msf::ColPassArgs& cc, hipStream_t s) -> int {
@@ -927,7 +888,7 @@ int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const Run
             k += 2;
             b0 = b1 + nbB;
         }
-        if (f.want_d1) HIPCHK(c, msf::launch_fem_finalize(v.state, nbins, v.femv, c->stream));
+        if (f.want_d1) HIPCHK(c, msf::launch_fem_finalize(v.state, pl.nbins, v.femv, c->stream));
         return MSF_OK;
     }
     if (chains == 2) { // fork: the second stream starts after everything issued so far
@@ -949,7 +910,7 @@ int enqueue_job(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const Run
         HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
-    if (f.want_d1) HIPCHK(c, msf::launch_fem_finalize(v.state, nbins, v.femv, c->stream));
+    if (f.want_d1) HIPCHK(c, msf::launch_fem_finalize(v.state, pl.nbins, v.femv, c->stream));
     return MSF_OK;
 }
 
@@ -1107,8 +1068,6 @@ int finish_outputs(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, const 
     }
     return MSF_OK;
 }
-
-} // namespace
 
 int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o,
                  const msf_detector* det, const msf_fem* fem, const std::vector<int2>& shifts,

@@ -55,12 +55,112 @@ uint64_t job_footprint(const ServeJob& j)
 // for the next job on the same grid and settings
 void reset_plan(msf_ctx* c) { c->plan.release_work(); }
 
+// Jobs that may share slice-pass launches: one grid size, the same settings.
+bool groupable(const ServeJob& a, const ServeJob& b)
+{
+    const msf_grid& ga = a.spec ? a.spec->grid : a.grid;
+    const msf_grid& gb = b.spec ? b.spec->grid : b.grid;
+    return ga.nx == gb.nx && ga.ny == gb.ny && a.cfg.transmit_mode == b.cfg.transmit_mode &&
+           a.cfg.fft_plan == b.cfg.fft_plan && a.cfg.col_tile == b.cfg.col_tile &&
+           a.cfg.batch == b.cfg.batch;
+}
+
+// One job on the slot's context: its specimen (unless resident), the run,
+// the specimen freed.
+void run_one(msf_ctx* sc, ServeJob& job, ServeDone& d, bool& built)
+{
+    msf_specimen* sp = nullptr;
+    d.rc = msf_set_config(sc, &job.cfg);
+    if (d.rc == MSF_OK && !job.spec) {
+        d.rc = msf_specimen_build(sc, &job.grid, job.q.data(), job.Z.data(),
+                                  (uint32_t)job.Z.size(), job.min_slice, &sp);
+        built = d.rc == MSF_OK;
+    }
+    if (!job.mtf.empty()) {
+        job.fem.mtf = job.mtf.data();
+        job.fem.mtf_domain = job.mtf_dom.data();
+    }
+    const msf_specimen* use = job.spec ? job.spec : sp;
+    if (d.rc == MSF_OK)
+        d.rc = msf_run_stem_fem(sc, use, &job.optics, &job.det, &job.fem, job.xp.data(),
+                                (uint32_t)job.xp.size(), job.yp.data(), (uint32_t)job.yp.size(),
+                                job.adf, &job.fo);
+    if (d.rc != MSF_OK) d.msg = sc->err;
+    if (sp) msf_specimen_free(sp);
+    std::lock_guard<std::recursive_mutex> lk(sc->mu);
+    reset_plan(sc);
+}
+
+// Several jobs of one grid size as one group (run_group): specimens built
+// first, then every run's slice passes in shared launches.
+void run_grouped(msf_ctx* sc, std::vector<ServeJob>& jobs, std::vector<ServeDone>& d,
+                 std::vector<bool>& built, GroupInfo& info)
+{
+    const size_t n = jobs.size();
+    std::vector<msf_specimen*> own(n, nullptr);
+    std::vector<GroupRun> runs;
+    std::vector<size_t> idx;
+    const int rc0 = msf_set_config(sc, &jobs[0].cfg);
+    for (size_t k = 0; k < n; ++k) {
+        ServeJob& job = jobs[k];
+        d[k].rc = rc0;
+        if (d[k].rc == MSF_OK && !job.spec) {
+            d[k].rc = msf_specimen_build(sc, &job.grid, job.q.data(), job.Z.data(),
+                                         (uint32_t)job.Z.size(), job.min_slice, &own[k]);
+            built[k] = d[k].rc == MSF_OK;
+        }
+        if (!job.mtf.empty()) {
+            job.fem.mtf = job.mtf.data();
+            job.fem.mtf_domain = job.mtf_dom.data();
+        }
+        GroupRun g;
+        g.sp = job.spec ? job.spec : own[k];
+        if (d[k].rc == MSF_OK &&
+            !raster_shifts(g.sp->dom, job.xp.data(), (uint32_t)job.xp.size(), job.yp.data(),
+                           (uint32_t)job.yp.size(), g.shifts)) {
+            d[k].rc = MSF_ERR_DOMAIN;
+            sc->err = "probe position below the domain minimum";
+        }
+        if (d[k].rc != MSF_OK) {
+            d[k].msg = sc->err;
+            continue;
+        }
+        g.o = &job.optics;
+        g.det = &job.det;
+        g.fem = &job.fem;
+        g.out = run_out(job.adf, &job.fo);
+        runs.push_back(std::move(g));
+        idx.push_back(k);
+    }
+    {
+        std::lock_guard<std::recursive_mutex> lk(sc->mu);
+        if (hipSetDevice(sc->device) == hipSuccess) {
+            (void)run_group(sc, runs, &info);
+        } else {
+            (void)hipGetLastError();
+            for (GroupRun& g : runs) {
+                g.rc = MSF_ERR_DEVICE;
+                g.err = "hipSetDevice failed";
+            }
+        }
+        release_group_plans(sc);
+        reset_plan(sc);
+    }
+    for (size_t r = 0; r < runs.size(); ++r) {
+        d[idx[r]].rc = runs[r].rc;
+        if (runs[r].rc != MSF_OK) d[idx[r]].msg = runs[r].err;
+    }
+    for (msf_specimen* sp : own)
+        if (sp) msf_specimen_free(sp);
+}
+
 void serve_worker(msf_ctx* c, uint32_t slot)
 {
     ServeState& st = *c->serve;
     msf_ctx* sc = st.slot_ctx[slot];
     for (;;) {
-        ServeJob job;
+        std::vector<ServeJob> jobs;
+        uint64_t fp = 0;
         double t_start = 0;
         {
             std::unique_lock<std::mutex> lk(st.m);
@@ -70,59 +170,68 @@ void serve_worker(msf_ctx* c, uint32_t slot)
                        st.admitted + st.resident_bytes + st.queue.front().footprint <= st.limit;
             });
             if (st.queue.empty()) return; // stop requested and nothing left
-            job = std::move(st.queue.front());
+            jobs.push_back(std::move(st.queue.front()));
             st.queue.pop_front();
-            st.admitted += job.footprint;
-            st.running += 1;
+            fp = jobs[0].footprint;
+            // grouping: further queued jobs of the head job's grid size and
+            // settings that fit beside it, in queue order
+            for (auto it = st.queue.begin(); it != st.queue.end() && jobs.size() < st.group_max;) {
+                if (groupable(jobs[0], *it) &&
+                    st.admitted + st.resident_bytes + fp + it->footprint <= st.limit) {
+                    fp += it->footprint;
+                    jobs.push_back(std::move(*it));
+                    it = st.queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            st.admitted += fp;
+            st.running += (uint32_t)jobs.size();
             st.max_running = std::max(st.max_running, st.running);
             st.peak_admitted = std::max(st.peak_admitted, st.admitted + st.resident_bytes);
             t_start = now_ms();
         }
         st.work_cv.notify_all(); // the next head job may fit as well
-        ServeDone d;
-        bool built = false;
-        {
-            msf_specimen* sp = nullptr;
-            d.rc = msf_set_config(sc, &job.cfg);
-            if (d.rc == MSF_OK && !job.spec) {
-                d.rc = msf_specimen_build(sc, &job.grid, job.q.data(), job.Z.data(),
-                                          (uint32_t)job.Z.size(), job.min_slice, &sp);
-                built = d.rc == MSF_OK;
-            }
-            if (!job.mtf.empty()) {
-                job.fem.mtf = job.mtf.data();
-                job.fem.mtf_domain = job.mtf_dom.data();
-            }
-            const msf_specimen* use = job.spec ? job.spec : sp;
-            if (d.rc == MSF_OK)
-                d.rc = msf_run_stem_fem(sc, use, &job.optics, &job.det, &job.fem, job.xp.data(),
-                                        (uint32_t)job.xp.size(), job.yp.data(),
-                                        (uint32_t)job.yp.size(), job.adf, &job.fo);
-            if (d.rc != MSF_OK) d.msg = sc->err;
-            if (sp) msf_specimen_free(sp);
-            std::lock_guard<std::recursive_mutex> lk(sc->mu);
-            reset_plan(sc);
+        const size_t n = jobs.size();
+        std::vector<ServeDone> d(n);
+        std::vector<bool> built(n, false);
+        GroupInfo info{};
+        if (n == 1) {
+            bool b = false;
+            run_one(sc, jobs[0], d[0], b);
+            built[0] = b;
+        } else {
+            run_grouped(sc, jobs, d, built, info);
         }
-        d.info.queue_ms = t_start - job.t_submit;
-        d.info.run_ms = ms_since(t_start);
-        d.info.footprint = job.footprint;
-        d.info.slot = slot;
+        const double t_end = now_ms();
         {
             std::lock_guard<std::mutex> lk(st.m);
-            st.admitted -= job.footprint;
-            st.running -= 1;
-            st.jobs_done += 1;
-            if (built) st.specimen_builds += 1;
-            if (job.spec) { // the specimen stays resident while jobs refer to it
-                auto it = st.resident.find(job.spec->id);
-                if (it != st.resident.end() && --it->second.jobs == 0) {
-                    st.resident_bytes -= it->second.bytes;
-                    st.resident.erase(it);
-                }
-                job.spec->served.fetch_sub(1);
+            st.admitted -= fp;
+            st.running -= (uint32_t)n;
+            if (n > 1) {
+                st.groups += 1;
+                st.grouped_jobs += n;
+                st.group_launches += info.launches;
             }
-            st.pending.erase(job.id);
-            st.done[job.id] = std::move(d);
+            for (size_t k = 0; k < n; ++k) {
+                ServeJob& job = jobs[k];
+                d[k].info.queue_ms = t_start - job.t_submit;
+                d[k].info.run_ms = t_end - t_start;
+                d[k].info.footprint = job.footprint;
+                d[k].info.slot = slot;
+                st.jobs_done += 1;
+                if (built[k]) st.specimen_builds += 1;
+                if (job.spec) { // the specimen stays resident while jobs refer to it
+                    auto it = st.resident.find(job.spec->id);
+                    if (it != st.resident.end() && --it->second.jobs == 0) {
+                        st.resident_bytes -= it->second.bytes;
+                        st.resident.erase(it);
+                    }
+                    job.spec->served.fetch_sub(1);
+                }
+                st.pending.erase(job.id);
+                st.done[job.id] = std::move(d[k]);
+            }
         }
         st.work_cv.notify_all();
         st.done_cv.notify_all();
@@ -136,6 +245,7 @@ int start_serving(msf_ctx* c)
     if (c->serve) return MSF_OK;
     std::unique_ptr<ServeState> st(new ServeState);
     st->slots = c->serve_slots ? c->serve_slots : 3;
+    st->group_max = std::max<uint32_t>(1, c->serve_group);
     st->limit = c->serve_limit;
     if (!st->limit) {
         size_t free_b = 0, total_b = 0;
@@ -331,6 +441,21 @@ int msf_serve_configure(msf_ctx* c, uint32_t slots, uint64_t mem_limit)
     MSF_CATCH_ALL(c)
 }
 
+int msf_serve_group(msf_ctx* c, uint32_t max_jobs)
+{
+    try {
+        if (!c) return MSF_ERR_INVALID;
+        std::lock_guard<std::mutex> lk(c->serve_mu);
+        if (c->serve) return fail(c, MSF_ERR_INVALID, "the serving engine has already started");
+        if (max_jobs > (uint32_t)msf::kGroupMax)
+            return fail(c, MSF_ERR_INVALID,
+                        "a group holds at most " + std::to_string(msf::kGroupMax) + " jobs");
+        c->serve_group = max_jobs;
+        return MSF_OK;
+    }
+    MSF_CATCH_ALL(c)
+}
+
 int msf_serve_get_stats(msf_ctx* c, msf_serve_stats* out)
 {
     try {
@@ -350,6 +475,9 @@ int msf_serve_get_stats(msf_ctx* c, msf_serve_stats* out)
         out->peak_admitted_bytes = st.peak_admitted;
         out->specimen_builds = st.specimen_builds;
         out->resident_bytes = st.resident_bytes;
+        out->groups = st.groups;
+        out->grouped_jobs = st.grouped_jobs;
+        out->group_launches = st.group_launches;
         return MSF_OK;
     }
     MSF_CATCH_ALL(c)

@@ -126,7 +126,9 @@ class ServeStats(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("running", C.c_uint32), ("max_running", C.c_uint32),
                 ("queued", C.c_uint64), ("jobs_done", C.c_uint64), ("mem_limit", C.c_uint64),
                 ("admitted_bytes", C.c_uint64), ("peak_admitted_bytes", C.c_uint64),
-                ("specimen_builds", C.c_uint64), ("resident_bytes", C.c_uint64)]
+                ("specimen_builds", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("groups", C.c_uint64), ("grouped_jobs", C.c_uint64),
+                ("group_launches", C.c_uint64)]
 
 
 class Config(C.Structure):
@@ -146,7 +148,8 @@ class Stats(C.Structure):
                 ("col_pass_count", C.c_uint64), ("row_pass_bytes", C.c_double),
                 ("col_pass_bytes", C.c_double), ("graph", C.c_int32), ("chains", C.c_int32),
                 ("col_tile", C.c_int32), ("autotuned", C.c_int32), ("row_fft_plan", C.c_int32),
-                ("col_fft_plan", C.c_int32), ("fused_launches", C.c_uint64)]
+                ("col_fft_plan", C.c_int32), ("fused_launches", C.c_uint64),
+                ("group_runs", C.c_uint32), ("group_launches", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -172,7 +175,8 @@ EXPORTS = ["msf_create", "msf_destroy", "msf_last_error", "msf_set_config", "msf
            "msf_fem_accumulate", "msf_mtf_2d", "msf_apply_mtf", "msf_wait_info",
            "msf_serve_configure", "msf_serve_get_stats", "msf_job_footprint",
            "msf_run_stem_noalloc", "msf_copy_rate", "msf_submit_run", "msf_tuning_save",
-           "msf_tuning_load", "msf_tuning_clear"]
+           "msf_tuning_load", "msf_tuning_clear", "msf_run_group",
+           "msf_serve_group"]
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -243,10 +247,12 @@ def lib():
         L.msf_mtf_table_free.argtypes = [C.POINTER(MtfTable)]
         L.msf_submit.argtypes = [vp, C.POINTER(Job), C.POINTER(C.c_uint64)]
         L.msf_submit_run.argtypes = [vp, C.POINTER(RunJob), C.POINTER(C.c_uint64)]
+        L.msf_run_group.argtypes = [vp, C.POINTER(RunJob), C.c_uint32, C.POINTER(C.c_int32)]
         L.msf_poll.argtypes = [vp, C.c_uint64]
         L.msf_wait.argtypes = [vp, C.c_uint64]
         L.msf_wait_info.argtypes = [vp, C.c_uint64, C.POINTER(JobInfo)]
         L.msf_serve_configure.argtypes = [vp, C.c_uint32, C.c_uint64]
+        L.msf_serve_group.argtypes = [vp, C.c_uint32]
         L.msf_serve_get_stats.argtypes = [vp, C.POINTER(ServeStats)]
         L.msf_job_footprint.argtypes = [C.POINTER(Job)]
         L.msf_job_footprint.restype = C.c_uint64
@@ -609,6 +615,38 @@ class Context:
         t.footprint = lib().msf_job_footprint(C.byref(job))
         return t
 
+    def run_group(self, runs, modes=("v_omega",)):
+        """msf_run_group: runs on resident specimens of one grid size with
+        shared slice-pass launches.  `runs`: dicts with spec, optics, x_p,
+        y_p and optionally detector, fem.  Returns (results, status): per run
+        (adf image [n_xp, n_yp] or None, {mode: values}) and its status code."""
+        keep, jobs, res = [], [], []
+        for r in runs:
+            x_p = np.ascontiguousarray(r["x_p"], np.float64)
+            y_p = np.ascontiguousarray(r["y_p"], np.float64)
+            det = r.get("detector") or Detector(0, 0.0, 0.0, 0)
+            fm = r.get("fem") or Fem(0, 0.0)
+            img = np.zeros(x_p.size * y_p.size) if det.enabled else None
+            outs, nb = {}, 0
+            if fm.enabled:
+                nb = fem_bins(r["spec"].grid, fm.dr).size
+                outs = {m: np.zeros(max(nb, 1)) for m in modes}
+            fo = FemOutputs(*[outs[m].ctypes.data if m in outs else None for m in FEM_MODES])
+            jobs.append(RunJob(r["spec"]._h, r["optics"], x_p.ctypes.data, x_p.size,
+                               y_p.ctypes.data, y_p.size, det, fm,
+                               img.ctypes.data if img is not None else None, fo))
+            keep.append((x_p, y_p))
+            res.append((img, outs, nb, x_p.size, y_p.size))
+        arr = (RunJob * max(len(jobs), 1))(*jobs)
+        status = (C.c_int32 * max(len(jobs), 1))()
+        rc = lib().msf_run_group(self._h, arr, len(jobs), status)
+        out = [((img.reshape(nxp, nyp) if img is not None else None),
+                {m: v[:nb] for m, v in outs.items()}) for img, outs, nb, nxp, nyp in res]
+        st = [int(status[k]) for k in range(len(jobs))]
+        if rc and all(x != 0 for x in st):
+            self._check(rc, "msf_run_group")
+        return out, st
+
     def submit_run(self, spec, optics, x_p, y_p, detector=None, fem=None, modes=("v_omega",)):
         """msf_submit_run: queue a run on a resident specimen of this context
         on the serving engine; returns a Ticket for poll() / wait()."""
@@ -647,6 +685,11 @@ class Context:
     def serve_configure(self, slots=0, mem_limit=0):
         """msf_serve_configure (before the first submit)."""
         self._check(lib().msf_serve_configure(self._h, slots, mem_limit), "msf_serve_configure")
+
+    def serve_group(self, max_jobs):
+        """msf_serve_group (before the first submit): run up to max_jobs queued
+        jobs of one grid size as a group with shared slice-pass launches."""
+        self._check(lib().msf_serve_group(self._h, max_jobs), "msf_serve_group")
 
     def serve_stats(self):
         st = ServeStats()

@@ -191,3 +191,55 @@ def test_runs_on_a_resident_specimen_share_one_build(msf):
             np.testing.assert_array_equal(v4["v_omega"], v["v_omega"])
     finally:
         eng.close()
+
+
+def test_grouped_jobs_match_direct_runs(msf):
+    """With grouping (msf_serve_group), queued jobs of one grid size run as a
+    group with shared slice-pass launches; every job, built from atoms or on a
+    resident specimen, equals the same run made directly, bit for bit, and a
+    job with bad atoms fails alone."""
+    own = msf.Context(0)
+    try:
+        own.serve_configure(slots=1)
+        own.serve_group(8)
+        # a larger head job keeps the one slot busy while the rest queue up
+        (c0, _), = _jobs(1, seed=21)
+        head_xp = np.linspace(0.0, c0.cell[0] * 0.9, 12)
+        jobs = _jobs(9, seed=22)
+        spec_r = jobs[0][0].gpu_specimen(msf, own)
+        tickets = [own.submit(c0.grid(msf), c0.q, c0.Z, c0.min_slice, c0.optics(msf), head_xp,
+                              head_xp, detector=c0.detector(msf), fem=c0.femcfg(msf))]
+        for c, xp in jobs:
+            tickets.append(own.submit(c.grid(msf), c.q, c.Z, c.min_slice, c.optics(msf), xp, xp,
+                                      detector=c.detector(msf), fem=c.femcfg(msf),
+                                      modes=("v_omega", "v_re")))
+        c, xp = jobs[0]
+        t_res = own.submit_run(spec_r, c.optics(msf), xp[::-1].copy(), xp,
+                               detector=c.detector(msf), fem=c.femcfg(msf))
+        bad = own.submit(c.grid(msf), c.q, np.full(c.Z.size, 200, np.uint32), c.min_slice,
+                         c.optics(msf), xp, xp, detector=c.detector(msf))
+        results = [own.wait(t) for t in tickets[1:]]
+        img_res, fem_res = own.wait(t_res)
+        with pytest.raises(msf.MsfError):
+            own.wait(bad)
+        own.wait(tickets[0])
+        st = own.serve_stats()
+        print("groups", st["groups"], "grouped jobs", st["grouped_jobs"], "launches",
+              st["group_launches"], flush=True)
+        assert st["groups"] >= 1 and st["grouped_jobs"] >= 2 and st["group_launches"] > 0
+        for (c, xp), (img, fem) in zip(jobs, results):
+            spec = c.gpu_specimen(msf, own)
+            img_d, fem_d = own.adfstem_fem(spec, c.optics(msf), xp, xp, c.femcfg(msf),
+                                           modes=("v_omega", "v_re"), detector=c.detector(msf))
+            np.testing.assert_array_equal(img, img_d)
+            for m in fem:
+                np.testing.assert_array_equal(fem[m], fem_d[m])
+            spec.free()
+        c, xp = jobs[0]
+        img_d, fem_d = own.adfstem_fem(spec_r, c.optics(msf), xp[::-1].copy(), xp, c.femcfg(msf),
+                                       modes=("v_omega",), detector=c.detector(msf))
+        np.testing.assert_array_equal(img_res, img_d)
+        np.testing.assert_array_equal(fem_res["v_omega"], fem_d["v_omega"])
+        spec_r.free()
+    finally:
+        own.close()
