@@ -123,9 +123,14 @@ hipError_t launch_col_pass(int nx, bool last, const ColPassArgs& a, hipStream_t 
 hipError_t launch_reduce_segments(const float2* psi, int64_t plane, const int32_t* pix,
                                   const int32_t* tile_off, const int32_t* seg_kind, int n_tiles,
                                   int batch, double* seg_out, hipStream_t s);
+// (small: the groups of at most 64 segments, one lane each; large: the rest,
+// one wavefront each; combine_split forms both lists from seg_of_group)
 hipError_t launch_combine_segments(const double* seg_in, int n_seg_stride,
-                                   const int32_t* seg_of_group, int groups, int batch,
+                                   const int32_t* seg_of_group, const int32_t* small, int n_small,
+                                   const int32_t* large, int n_large, int groups, int batch,
                                    double* out, hipStream_t s);
+void combine_split(const std::vector<int32_t>& seg_of_group, std::vector<int32_t>& small,
+                   std::vector<int32_t>& large);
 // ADF column of the partial sums -> adf[pos0 + b]
 hipError_t launch_collect_adf(const double* partial, int groups, int batch, double* adf,
                               hipStream_t s);

@@ -442,28 +442,69 @@ hipError_t launch_reduce_segments(const float2* psi, int64_t plane, const int32_
     return hipGetLastError();
 }
 
-__global__ void combine_segments_kernel(const double* seg_in, int n_seg_stride,
-                                        const int32_t* seg_of_group, int groups, int batch,
-                                        double* out)
+// Group sums from the segment sums, per (group, position), in an order that
+// depends on the group only.  Groups of at most kWaveGroup segments: one lane
+// adds them in segment order.  Larger groups (the ADF annulus at 1024^2:
+// ~5,500 segments): one wavefront, lane l adding segments l, l + 64, ... in
+// order, the lanes' sums meeting in a fixed butterfly.
+constexpr int kWaveGroup = 64;
+
+__global__ void combine_small_kernel(const double* seg_in, int n_seg_stride,
+                                     const int32_t* seg_of_group, const int32_t* ids, int n_ids,
+                                     int groups, int batch, double* out)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)groups * batch) return;
-    const int g = (int)(idx % groups);
-    const int b = (int)(idx / groups);
+    if (idx >= (int64_t)n_ids * batch) return;
+    const int g = ids[(int)(idx % n_ids)];
+    const int b = (int)(idx / n_ids);
     const double* in = seg_in + (int64_t)b * n_seg_stride;
     double acc = 0.0;
     for (int k = seg_of_group[g]; k < seg_of_group[g + 1]; ++k) acc += in[k];
     out[(int64_t)b * groups + g] = acc;
 }
 
+__global__ void __launch_bounds__(256) combine_large_kernel(const double* seg_in, int n_seg_stride,
+                                                            const int32_t* seg_of_group,
+                                                            const int32_t* ids, int n_ids,
+                                                            int groups, int batch, double* out)
+{
+    const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= (int64_t)n_ids * batch) return; // whole wavefronts
+    const int lane = threadIdx.x & 63;
+    const int g = ids[(int)(pair % n_ids)];
+    const int b = (int)(pair / n_ids);
+    const double* in = seg_in + (int64_t)b * n_seg_stride;
+    const int s1 = seg_of_group[g + 1];
+    double acc = 0.0;
+    for (int k = seg_of_group[g] + lane; k < s1; k += 64) acc += in[k];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) out[(int64_t)b * groups + g] = acc;
+}
+
+void combine_split(const std::vector<int32_t>& seg_of_group, std::vector<int32_t>& small,
+                   std::vector<int32_t>& large)
+{
+    small.clear();
+    large.clear();
+    for (size_t g = 0; g + 1 < seg_of_group.size(); ++g)
+        (seg_of_group[g + 1] - seg_of_group[g] > kWaveGroup ? large : small).push_back((int32_t)g);
+}
+
 hipError_t launch_combine_segments(const double* seg_in, int n_seg_stride,
-                                   const int32_t* seg_of_group, int groups, int batch,
+                                   const int32_t* seg_of_group, const int32_t* small, int n_small,
+                                   const int32_t* large, int n_large, int groups, int batch,
                                    double* out, hipStream_t s)
 {
-    const int64_t n = (int64_t)groups * batch;
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(combine_segments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       seg_in, n_seg_stride, seg_of_group, groups, batch, out);
+    if (batch <= 0) return hipSuccess;
+    const int64_t ns = (int64_t)n_small * batch, nl = (int64_t)n_large * batch;
+    if (ns > 0)
+        hipLaunchKernelGGL(combine_small_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                           s, seg_in, n_seg_stride, seg_of_group, small, n_small, groups, batch,
+                           out);
+    if (nl > 0)
+        hipLaunchKernelGGL(combine_large_kernel, dim3((unsigned)((nl + 3) / 4)), dim3(256), 0, s,
+                           seg_in, n_seg_stride, seg_of_group, large, n_large, groups, batch, out);
     return hipGetLastError();
 }
 

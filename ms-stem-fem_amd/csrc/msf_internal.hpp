@@ -95,9 +95,11 @@ struct RunPlan {
     msf::Groups hg;          // host groups (also for the per-pixel FEM modes)
     msf::SegmentTiles st;    // host segment tables
     int n_tiles = 0;
-    DevBuf seg_pix_b, tile_off_b, seg_kind_b, seg_of_group_b, cnt_b, zero_b;
+    DevBuf seg_pix_b, tile_off_b, seg_kind_b, seg_of_group_b, cnt_b, zero_b, grp_small_b,
+        grp_large_b;
     int32_t *seg_pix = nullptr, *tile_off = nullptr, *seg_kind = nullptr,
-            *seg_of_group = nullptr, *cnt = nullptr;
+            *seg_of_group = nullptr, *cnt = nullptr, *grp_small = nullptr, *grp_large = nullptr;
+    std::vector<int32_t> small_ids, large_ids; // groups summed by one lane / one wavefront
     uint8_t* zero = nullptr;
     // detector MTF: the table it was built from, the 2-D MTF [nx][ny] and the
     // outside-aperture map

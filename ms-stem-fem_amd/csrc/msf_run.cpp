@@ -429,6 +429,7 @@ int prepare_groups(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, cons
         pl.G = (int)pl.hg.kind.size();
         pl.nbins = pl.hg.nbins;
         pl.n_tiles = pl.st.n_tiles;
+        msf::combine_split(pl.st.seg_of_group, pl.small_ids, pl.large_ids);
         pl.fem_sqr = f.want_d2;
         pl.corr = f.want_corr;
         pl.cap_b = 0; // partial-sum buffers depend on G
@@ -452,6 +453,8 @@ int prepare_groups(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, cons
         if (int r = put(c, m, pl.tile_off_b, pl.st.tile_off, &pl.tile_off)) return r;
         if (int r = put(c, m, pl.seg_kind_b, pl.st.seg_kind, &pl.seg_kind)) return r;
         if (int r = put(c, m, pl.seg_of_group_b, pl.st.seg_of_group, &pl.seg_of_group)) return r;
+        if (int r = put(c, m, pl.grp_small_b, pl.small_ids, &pl.grp_small)) return r;
+        if (int r = put(c, m, pl.grp_large_b, pl.large_ids, &pl.grp_large)) return r;
         if (int r = put(c, m, pl.cnt_b, pl.hg.counts, &pl.cnt)) return r;
         if (int r = put(c, m, pl.zero_b, pl.hg.zeroed, &pl.zero)) return r;
         if (f.want_mtf) {
@@ -759,11 +762,15 @@ int enqueue_batch_tail(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, co
     float2* bpsi = v.psi[ch];
     double* bpart = v.part[ch];
     double* bsegp = v.segp[ch];
+    auto combine = [&](const double* sg, int n, double* out, hipStream_t st) {
+        return msf::launch_combine_segments(sg, pl.n_tiles * 64, pl.seg_of_group, pl.grp_small,
+                                            (int)pl.small_ids.size(), pl.grp_large,
+                                            (int)pl.large_ids.size(), G, n, out, st);
+    };
     HIPCHK(c, prof.begin(K_RED));
     HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off, pl.seg_kind,
                                           pl.n_tiles, nb, bsegp, s));
-    HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64, pl.seg_of_group, G, nb,
-                                           bpart, s));
+    HIPCHK(c, combine(bsegp, nb, bpart, s));
     HIPCHK(c, prof.end());
     if (f.want_adf) HIPCHK(c, msf::launch_collect_adf(bpart, G, nb, v.adf + b0, s));
     if (f.want_mtf) {
@@ -779,8 +786,7 @@ int enqueue_batch_tail(msf_ctx* c, const RunPlan& pl, const msf_specimen* sp, co
                                           d.sqrt_nxny, nb, bpsi, s));
         HIPCHK(c, msf::launch_reduce_segments(bpsi, plane, pl.seg_pix, pl.tile_off,
                                               pl.seg_kind, pl.n_tiles, nb, bsegp, s));
-        HIPCHK(c, msf::launch_combine_segments(bsegp, pl.n_tiles * 64, pl.seg_of_group, G, nb,
-                                               bpart, s));
+        HIPCHK(c, combine(bsegp, nb, bpart, s));
     }
     // the FEM accumulators are shared: batch k updates them after batch k-1
     if (chains == 2 && k > 0) HIPCHK(c, hipStreamWaitEvent(s, c->ev_fem[(k - 1) % 2], 0));
@@ -929,7 +935,8 @@ std::vector<uint64_t> graph_signature(const RunPlan& pl, const msf_specimen* sp,
             u(pl.mtf_out), u(v.mtf_full[0]), u(v.twx.f64), u(v.twy.f64), u(v.vsum), u(v.pm1),
             u(v.pm2), (uint64_t)pl.hg.fem_abs, (uint64_t)pl.hg.fem_sqr, (uint64_t)f.plane,
             u(v.psi[0]), u(v.part[0]), u(v.segp[0]), u(v.adf), u(v.shift), u(v.state), u(v.femv),
-            u(pl.u0), u(pl.psi0), u(pl.seg_pix), u(pl.tile_off), u(pl.seg_kind),
+            u(pl.u0), u(pl.psi0), u(pl.seg_pix), u(pl.tile_off), u(pl.seg_kind), u(pl.grp_small),
+            u(pl.grp_large), (uint64_t)pl.small_ids.size(),
             u(pl.seg_of_group), u(pl.cnt), u(pl.zero), u(v.twx.f32), u(v.twx.alt), u(v.twy.f32),
             u(v.twy.alt), u(v.twx.nat), u(v.twy.nat), (uint64_t)f.ncp, (uint64_t)pl.G,
             (uint64_t)pl.nbins, (uint64_t)pl.n_tiles, (uint64_t)sh.col_tile, (uint64_t)sh.plan_v};

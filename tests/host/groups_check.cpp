@@ -6,6 +6,7 @@
 // AddressSanitizer + UBSan and under ThreadSanitizer by test_host_sanitizers.py;
 // two builds also run concurrently, as two serving slots do.  Exits non-zero
 // on any violated invariant.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
@@ -30,6 +31,38 @@ static int check(const msf::Domain& d, int ncp, const msf::Groups& g, const msf:
         listed += p >= 0;
     }
     if (listed != g.pixels.size()) return 7;
+    // the sums the reduction forms (segments in lane order, then a group's
+    // segments in order) equal each group's direct sum of |psi|^2 or |psi|
+    std::vector<float> re((size_t)plane), im((size_t)plane);
+    uint64_t x = 88172645463325252ull;
+    for (int64_t k = 0; k < plane; ++k) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        re[(size_t)k] = (float)((x >> 11) * (1.0 / 9007199254740992.0)) - 0.5f;
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        im[(size_t)k] = (float)((x >> 11) * (1.0 / 9007199254740992.0)) - 0.5f;
+    }
+    auto val = [&](int64_t k, bool amp) {
+        const double a = re[(size_t)k], b = im[(size_t)k], i2 = a * a + b * b;
+        return amp ? std::sqrt(i2) : i2;
+    };
+    // (segment w * 64 + l sums its lane's pixels in order; a group adds its
+    // consecutive segments)
+    std::vector<double> seg((size_t)st.n_tiles * 64, 0.0);
+    for (int w = 0; w < st.n_tiles; ++w)
+        for (int l = 0; l < 64; ++l) {
+            double acc = 0.0;
+            for (int k = st.tile_off[w]; k < st.tile_off[w + 1]; ++k) {
+                const int32_t p = st.pix[(size_t)k * 64 + l];
+                if (p >= 0) acc += val(p, st.seg_kind[(size_t)w * 64 + l] != 0);
+            }
+            seg[(size_t)w * 64 + l] = acc;
+        }
+    for (size_t k = 0; k < G; ++k) {
+        double a = 0.0, b = 0.0;
+        for (int32_t s = st.seg_of_group[k]; s < st.seg_of_group[k + 1]; ++s) a += seg[(size_t)s];
+        for (int32_t q = g.offsets[k]; q < g.offsets[k + 1]; ++q) b += val(g.pixels[(size_t)q], g.kind[k] != 0);
+        if (std::fabs(a - b) > 1e-9 * std::max(1.0, std::fabs(b))) return 15;
+    }
     if ((int)g.bin_off.size() != g.nbins + 1) return 8;
     for (size_t k = 0; k < g.bin_full.size(); ++k)
         if (g.bin_full[k] < 0 || g.bin_full[k] >= d.nx * d.ny || g.bin_compact[k] >= plane) return 9;
