@@ -124,9 +124,11 @@ typedef struct msf_config {
     int32_t fft_plan;                  /* radix plans of the slice passes: 1 primary (every
                                           length), 2 the alternate plans with 16 values per lane
                                           and one fewer stage on lengths 256, 512 and 1024 (the
-                                          primary elsewhere); 0 = automatic (per length: the
+                                          primary elsewhere), 3 the alternate plan for the row
+                                          passes (length ny) and the primary for the column
+                                          passes, 4 the reverse; 0 = automatic (per length: the
                                           alternate at 256 and 1024, else the primary; or the
-                                          autotuner's choice among these three).  Plans differ by
+                                          autotuner's choice among these).  Plans differ by
                                           FP32 rounding only. */
 } msf_config;
 

@@ -109,7 +109,8 @@ int msf_set_config(msf_ctx* c, const msf_config* cfg)
             return fail(c, MSF_ERR_INVALID, "col_tile must be 0, 4, 8 or 16");
         if (cfg->chains < 0 || cfg->chains > 3)
             return fail(c, MSF_ERR_INVALID, "chains must be 0, 1, 2 or 3");
-        if (cfg->fft_plan < 0 || cfg->fft_plan > 2) return fail(c, MSF_ERR_INVALID, "fft_plan must be 0, 1 or 2");
+        if (cfg->fft_plan < 0 || cfg->fft_plan > 4)
+            return fail(c, MSF_ERR_INVALID, "fft_plan must be 0 .. 4");
         c->cfg = *cfg;
         return MSF_OK;
     }

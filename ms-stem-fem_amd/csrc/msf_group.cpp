@@ -85,10 +85,7 @@ int run_group(msf_ctx* c, std::vector<GroupRun>& runs, GroupInfo* info)
         s.v.twx = twx;
         s.v.twy = twy;
         // launch shape: the rules (the radix plans of a run on its own), one chain
-        const int all_alt = (twx.alt ? 1 : 0) + (twy.alt ? 2 : 0);
-        s.sh.plan_v = c->cfg.fft_plan == 1   ? 0
-                      : c->cfg.fft_plan == 2 ? all_alt
-                                             : fft_plan_rule(nx) + 2 * fft_plan_rule(ny);
+        s.sh.plan_v = plan_choice(c->cfg, nx, ny, twx, twy);
         s.sh.B = std::min<int64_t>(c->cfg.batch ? (int64_t)c->cfg.batch : batch_rule(s.f), s.f.P);
         s.sh.n_batches = (s.f.P + s.sh.B - 1) / s.sh.B;
         s.sh.col_tile = c->cfg.col_tile;

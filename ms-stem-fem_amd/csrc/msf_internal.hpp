@@ -354,6 +354,9 @@ int check_optics(msf_ctx* c, const msf_optics* o);
 int check_mtf(msf_ctx* c, const msf_fem* fem);
 std::vector<int2> to_int2(const msf::RowRanges& r);
 int fft_plan_rule(int n);
+// radix plans of a run, vx + 2 vy (per axis 0 primary, 1 alternate), from
+// msf_config.fft_plan and the lengths' alternate tables
+int plan_choice(const msf_config& cfg, int nx, int ny, const Twiddles& twx, const Twiddles& twy);
 // H(k) -> BACKWARD FFT: h = IFFT(H) (unnormalised, [nx][ny] double2) and its
 // squared norm, in memory from m
 int probe_device(msf_ctx* c, Mem& m, const msf::Domain& d, const msf_optics* o, DevBuf& hb,

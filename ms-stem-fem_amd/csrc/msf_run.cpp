@@ -129,6 +129,18 @@ int fft_plan_rule(int n)
     return (msf::has_alt_plan(n) && n != 512) ? 1 : 0;
 }
 
+int plan_choice(const msf_config& cfg, int nx, int ny, const Twiddles& twx, const Twiddles& twy)
+{
+    const int ax = twx.alt ? 1 : 0, ay = twy.alt ? 2 : 0;
+    switch (cfg.fft_plan) {
+    case 1: return 0;
+    case 2: return ax + ay;
+    case 3: return ay;  // alternate rows, primary columns
+    case 4: return ax;  // primary rows, alternate columns
+    default: return fft_plan_rule(nx) + 2 * fft_plan_rule(ny);
+    }
+}
+
 int check_grid(msf_ctx* c, const msf_grid* g)
 {
     if (!g) return fail(c, MSF_ERR_INVALID, "grid is NULL");
@@ -553,9 +565,7 @@ int launch_shape(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp,
     // radix plans as vx + 2 vy (per axis 0 primary, 1 alternate): configured
     // (1 primary, 2 alternate where the length has one) or by the rule
     const int all_alt = (twx.alt ? 1 : 0) + (twy.alt ? 2 : 0);
-    const int plan_rule = c->cfg.fft_plan == 1   ? 0
-                          : c->cfg.fft_plan == 2 ? all_alt
-                                                 : fft_plan_rule(f.nx) + 2 * fft_plan_rule(f.ny);
+    const int plan_rule = plan_choice(c->cfg, f.nx, f.ny, twx, twy);
 
     // batch depth: the fewest batches whose wave buffer stays within 96 MiB,
     // spread evenly over the raster.  The buffer, the t and u0 slices and the
