@@ -204,6 +204,17 @@ template <bool INV, typename C> struct Dft<16, INV, C> {
 // Stockham write pattern across LDS banks
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 3); }
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3) + 1; }
+// lds_pad(r + S * p) for r >= 0: with S a multiple of 8 the padding of the
+// p-th element is (r >> 3) + p S / 8, so the address is one per-lane base
+// plus a constant per p (an immediate offset, no address register per p)
+template <int S> __device__ __forceinline__ int lds_pad_at(int r, int p)
+{
+    if constexpr (S % 8 == 0) {
+        return (r + (r >> 3)) + p * (S + S / 8);
+    } else {
+        return lds_pad(r + S * p);
+    }
+}
 // twiddle tables in LDS: stage reads W^(q k N/(NS R)) step through the table
 // with power-of-two strides; one slot every 16 entries spreads them over banks
 __host__ __device__ constexpr int tw_pad(int e) { return e + (e >> 4); }
@@ -265,11 +276,11 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
             const int j = t + T * m;
             const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-            for (int q = 0; q < R; ++q) lds[lds_pad(base + q * NS)] = y[m * R + q];
+            for (int q = 0; q < R; ++q) lds[lds_pad_at<NS>(base, q)] = y[m * R + q];
         }
         lds_sync<WAVE>();
 #pragma unroll
-        for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
+        for (int p = 0; p < E; ++p) x[p] = lds[lds_pad_at<T>(t, p)];
     }
 }
 

@@ -353,7 +353,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
         for (int p = 0; p < E; ++p) v[p] = tile[o0 + (uint32_t)p * step];
 #pragma unroll
         for (int p = 0; p < E; ++p)
-            s_buf[c * LEN + lds_pad(r0 + T * p)] = ok ? v[p] : float2{0.f, 0.f};
+            s_buf[c * LEN + lds_pad_at<T>(r0, p)] = ok ? v[p] : float2{0.f, 0.f};
     }
     __syncthreads();
 
@@ -365,7 +365,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
     float2* lds = s_buf + c * LEN;
     float2 x[E];
 #pragma unroll
-    for (int p = 0; p < E; ++p) x[p] = lds[lds_pad(t + T * p)];
+    for (int p = 0; p < E; ++p) x[p] = lds[lds_pad_at<T>(t, p)];
 
     P::template run<false, float2, WAVE>(x, lds, tw, t);
 
@@ -409,7 +409,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
 
     lds_sync<WAVE>();
 #pragma unroll
-    for (int p = 0; p < E; ++p) lds[lds_pad(t + T * p)] = x[p];
+    for (int p = 0; p < E; ++p) lds[lds_pad_at<T>(t, p)] = x[p];
     __syncthreads();
 
     {   // coalesced store of the tile
@@ -419,10 +419,8 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
         const uint32_t step = (uint32_t)(T * a.ncp);
         if (ok) {
 #pragma unroll
-            for (int p = 0; p < E; ++p) {
-                const int i = r0 + T * p;
-                tile[o0 + (uint32_t)p * step] = s_buf[c2 * LEN + lds_pad(i)];
-            }
+            for (int p = 0; p < E; ++p)
+                tile[o0 + (uint32_t)p * step] = s_buf[c2 * LEN + lds_pad_at<T>(r0, p)];
         }
     }
 }

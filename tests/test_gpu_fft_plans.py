@@ -76,17 +76,18 @@ def test_lengths_without_alternate_use_the_primary_plan(msf, ctx):
     spec = c.gpu_specimen(msf, ctx)
     outs = []
     try:
-        for fp in (1, 2):
+        for fp in (1, 2, 3, 4):
             ctx.configure(fft_plan=fp)
             outs.append(ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
                                     fem=c.femcfg(msf)))
             assert _plans(ctx) == (1, 1)
     finally:
         ctx.configure()
-    np.testing.assert_array_equal(outs[0][0], outs[1][0])
-    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    for o in outs[1:]:
+        np.testing.assert_array_equal(outs[0][0], o[0])
+        np.testing.assert_array_equal(outs[0][1], o[1])
     with pytest.raises(msf.MsfError):
-        ctx.configure(fft_plan=3)
+        ctx.configure(fft_plan=5)
 
 
 def test_mixed_lengths_follow_each_axis(msf, ctx, O):
@@ -104,6 +105,34 @@ def test_mixed_lengths_follow_each_axis(msf, ctx, O):
     assert _plans(ctx) == (2, 1)
     assert rel_l2(img, img_o) < 1e-4
     assert rel_l2(v, v_o) < 1e-4
+
+
+@pytest.mark.parametrize("fp,plans", [(3, (2, 1)), (4, (1, 2))])
+def test_per_axis_plans(msf, ctx, O, fp, plans):
+    """fft_plan 3 (alternate rows, primary columns) and 4 (the reverse) at
+    1024^2, within the oracle's tolerances; the mixes differ from the
+    all-alternate plan by FP32 rounding only."""
+    q, Z = random_atoms(30, (40.0, 40.0, 3.0), seed=74, species=(29,))
+    c = Case(nx=1024, ny=1024, cell=(40.0, 40.0, 3.0), voltage=200e3, q=q, Z=Z, min_slice=2.0,
+             alpha_max=0.02, inner=0.03, outer=0.2, fem=True, dr=0.5)
+    xp = np.array([5.0, 31.3])
+    yp = np.array([20.0])
+    _, (img_o, v_o, _) = c.run_oracle(O, xp, yp)
+    spec = c.gpu_specimen(msf, ctx)
+    try:
+        ctx.configure(fft_plan=fp)
+        img, v = ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
+                             fem=c.femcfg(msf))
+        assert _plans(ctx) == plans
+        ctx.configure(fft_plan=2)
+        img2, v2 = ctx.adfstem(spec, c.optics(msf), xp, yp, detector=c.detector(msf),
+                               fem=c.femcfg(msf))
+    finally:
+        ctx.configure()
+        spec.free()
+    assert rel_l2(img, img_o) < 1e-4
+    assert rel_l2(v, v_o) < 1e-4
+    assert rel_l2(img, img2) < 1e-6
 
 
 def test_autotuned_plan_reproduces_when_set(msf, ctx):
