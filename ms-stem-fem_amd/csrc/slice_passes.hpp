@@ -23,6 +23,15 @@
 
 namespace msf {
 
+// Element k of an array whose byte offsets fit 32 bits (a row or one
+// position's plane of the wave): the offset is formed in 32 bits, so the
+// load or store takes a uniform base and a 32-bit per-lane offset rather
+// than a 64-bit address per element.
+template <typename T> __device__ __forceinline__ T& elem32(T* base, uint32_t k)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + k * (uint32_t)sizeof(T));
+}
+
 // ------------------------------------------------------------------ row pass
 // Transmission of one row.  With t' = t/s = (tau + 1/s, ti) and the
 // reference's in-place update (TMODE 0: Im uses the updated Re), for the
@@ -170,13 +179,13 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
             if (PRUNE && in_zero_band<NY, E>(p)) {
                 x[p] = float2{0.f, 0.f};
             } else if (PRUNE && always_low<NY, E>(p)) {
-                x[p] = row[j];
+                x[p] = elem32(row, j);
             } else if (PRUNE && always_high<NY, E>(p)) {
-                x[p] = row[j + (uint32_t)off];
+                x[p] = elem32(row, j + (uint32_t)off);
             } else {
                 const bool lo = (int)j < a.ka, up = (int)j >= hi;
                 // lanes outside the kept band all read element 0 (one shared line)
-                const float2 v = row[lo ? j : (up ? j + (uint32_t)off : 0u)];
+                const float2 v = elem32(row, lo ? j : (up ? j + (uint32_t)off : 0u));
                 x[p] = (lo || up) ? v : float2{0.f, 0.f};
             }
         }
@@ -216,12 +225,12 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
             const uint32_t j = (uint32_t)(t + T * p);
             if (PRUNE && in_zero_band<NY, E>(p)) continue;
             if (PRUNE && always_low<NY, E>(p)) {
-                row[j] = x[p];
+                elem32(row, j) = x[p];
             } else if (PRUNE && always_high<NY, E>(p)) {
-                row[j + (uint32_t)off] = x[p];
+                elem32(row, j + (uint32_t)off) = x[p];
             } else {
-                if ((int)j < a.ka) row[j] = x[p];
-                else if ((int)j >= hi) row[j + (uint32_t)off] = x[p];
+                if ((int)j < a.ka) elem32(row, j) = x[p];
+                else if ((int)j >= hi) elem32(row, j + (uint32_t)off) = x[p];
             }
         }
     }
@@ -350,7 +359,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
         const uint32_t step = (uint32_t)(T * a.ncp);
         float2 v[E];
 #pragma unroll
-        for (int p = 0; p < E; ++p) v[p] = tile[o0 + (uint32_t)p * step];
+        for (int p = 0; p < E; ++p) v[p] = elem32(tile, o0 + (uint32_t)p * step);
 #pragma unroll
         for (int p = 0; p < E; ++p)
             s_buf[c * LEN + lds_pad_at<T>(r0, p)] = ok ? v[p] : float2{0.f, 0.f};
@@ -420,7 +429,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
         if (ok) {
 #pragma unroll
             for (int p = 0; p < E; ++p)
-                tile[o0 + (uint32_t)p * step] = s_buf[c2 * LEN + lds_pad_at<T>(r0, p)];
+                elem32(tile, o0 + (uint32_t)p * step) = s_buf[c2 * LEN + lds_pad_at<T>(r0, p)];
         }
     }
 }
