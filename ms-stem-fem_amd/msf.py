@@ -453,6 +453,43 @@ def write_correlograph_txt(path, data, k_edges, phi_edges):
         raise MsfError(rc, lib().msf_io_last_error().decode())
 
 
+def _fem_outputs(spec, fem, P, modes=FEM_MODES, correlograph=False, correlograph_each=False,
+                 pixel_images=False):
+    """Output arrays of msf_run_stem_fem for a specimen and FEM settings:
+    (msf_fem_outputs, the arrays, a function giving the result dict)."""
+    nb = fem_bins(spec.grid, fem.dr).size if fem.enabled else 0
+    outs = {m: np.zeros(max(nb, 1)) for m in modes}
+    ptrs = [outs[m].ctypes.data if m in outs else None for m in FEM_MODES]
+    nphi = 0
+    if correlograph:
+        nphi = correlograph_bins(spec.grid, fem.dr)[0]
+        for m in ("correlograph_avg", "correlograph_sqr_sum"):
+            outs[m] = np.zeros(max(nb * nphi, 1))
+            ptrs.append(outs[m].ctypes.data)
+        if correlograph_each:
+            outs["correlograph_each"] = np.zeros(max(P * nb * nphi, 1))
+            ptrs.append(outs["correlograph_each"].ctypes.data)
+    fo = FemOutputs(*ptrs)
+    img_names = ("diffraction_mean_abs", "diffraction_mean_sqr", "variance_2d")
+    if pixel_images:
+        for m in img_names:
+            outs[m] = np.zeros((spec.grid.nx, spec.grid.ny))
+            setattr(fo, m, outs[m].ctypes.data)
+
+    def finish():
+        res = {m: v[:nb] for m, v in outs.items() if m in FEM_MODES}
+        if pixel_images:
+            res.update({m: outs[m] for m in img_names})
+        if correlograph:
+            for m in ("correlograph_avg", "correlograph_sqr_sum"):
+                res[m] = outs[m][:nb * nphi].reshape(nphi, nb)
+            if correlograph_each:
+                res["correlograph_each"] = outs["correlograph_each"][:P * nb * nphi].reshape(
+                    P, nphi, nb)
+        return res
+    return fo, outs, finish
+
+
 class Context:
     """One device context (msf_create); one per host thread."""
 
@@ -555,40 +592,14 @@ class Context:
         y_p = np.ascontiguousarray(y_p, np.float64)
         det = detector if detector is not None else Detector(0, 0.0, 0.0, 0)
         img = np.zeros(x_p.size * y_p.size) if det.enabled else None
-        nb = fem_bins(spec.grid, fem.dr).size
-        outs = {m: np.zeros(max(nb, 1)) for m in modes}
-        ptrs = [outs[m].ctypes.data if m in outs else None for m in FEM_MODES]
-        nphi = 0
-        if correlograph:
-            nphi = correlograph_bins(spec.grid, fem.dr)[0]
-            for m in ("correlograph_avg", "correlograph_sqr_sum"):
-                outs[m] = np.zeros(max(nb * nphi, 1))
-                ptrs.append(outs[m].ctypes.data)
-            if correlograph_each:
-                outs["correlograph_each"] = np.zeros(max(x_p.size * y_p.size * nb * nphi, 1))
-                ptrs.append(outs["correlograph_each"].ctypes.data)
-        fo = FemOutputs(*ptrs)
-        img_names = ("diffraction_mean_abs", "diffraction_mean_sqr", "variance_2d")
-        if pixel_images:
-            for m in img_names:
-                outs[m] = np.zeros((spec.grid.nx, spec.grid.ny))
-                setattr(fo, m, outs[m].ctypes.data)
+        fo, outs, finish = _fem_outputs(spec, fem, x_p.size * y_p.size, modes, correlograph,
+                                        correlograph_each, pixel_images)
         self._check(lib().msf_run_stem_fem(self._h, spec._h, C.byref(optics), C.byref(det),
                                            C.byref(fem), _p(x_p), x_p.size, _p(y_p), y_p.size,
                                            _p(img), C.byref(fo)), "msf_run_stem_fem")
         if img is not None:
             img = img.reshape(x_p.size, y_p.size)
-        res = {m: v[:nb] for m, v in outs.items() if m in FEM_MODES}
-        if pixel_images:
-            res.update({m: outs[m] for m in img_names})
-        if correlograph:
-            for m in ("correlograph_avg", "correlograph_sqr_sum"):
-                res[m] = outs[m][:nb * nphi].reshape(nphi, nb)
-            if correlograph_each:
-                P = x_p.size * y_p.size
-                res["correlograph_each"] = outs["correlograph_each"][:P * nb * nphi].reshape(
-                    P, nphi, nb)
-        return img, res
+        return img, finish()
 
     def submit(self, grid, q, Z, min_slice, optics, x_p, y_p, detector=None, fem=None,
                modes=("v_omega",)):
@@ -618,8 +629,10 @@ class Context:
     def run_group(self, runs, modes=("v_omega",)):
         """msf_run_group: runs on resident specimens of one grid size with
         shared slice-pass launches.  `runs`: dicts with spec, optics, x_p,
-        y_p and optionally detector, fem.  Returns (results, status): per run
-        (adf image [n_xp, n_yp] or None, {mode: values}) and its status code."""
+        y_p and optionally detector, fem, modes, correlograph,
+        correlograph_each, pixel_images (as adfstem_fem).  Returns (results,
+        status): per run (adf image [n_xp, n_yp] or None, {output: values})
+        as adfstem_fem, and its status code."""
         keep, jobs, res = [], [], []
         for r in runs:
             x_p = np.ascontiguousarray(r["x_p"], np.float64)
@@ -627,21 +640,21 @@ class Context:
             det = r.get("detector") or Detector(0, 0.0, 0.0, 0)
             fm = r.get("fem") or Fem(0, 0.0)
             img = np.zeros(x_p.size * y_p.size) if det.enabled else None
-            outs, nb = {}, 0
-            if fm.enabled:
-                nb = fem_bins(r["spec"].grid, fm.dr).size
-                outs = {m: np.zeros(max(nb, 1)) for m in modes}
-            fo = FemOutputs(*[outs[m].ctypes.data if m in outs else None for m in FEM_MODES])
+            fo, outs, finish = _fem_outputs(r["spec"], fm, x_p.size * y_p.size,
+                                            r.get("modes", modes) if fm.enabled else (),
+                                            r.get("correlograph", False),
+                                            r.get("correlograph_each", False),
+                                            r.get("pixel_images", False))
             jobs.append(RunJob(r["spec"]._h, r["optics"], x_p.ctypes.data, x_p.size,
                                y_p.ctypes.data, y_p.size, det, fm,
                                img.ctypes.data if img is not None else None, fo))
-            keep.append((x_p, y_p))
-            res.append((img, outs, nb, x_p.size, y_p.size))
+            keep.append((x_p, y_p, outs))
+            res.append((img, finish, x_p.size, y_p.size))
         arr = (RunJob * max(len(jobs), 1))(*jobs)
         status = (C.c_int32 * max(len(jobs), 1))()
         rc = lib().msf_run_group(self._h, arr, len(jobs), status)
-        out = [((img.reshape(nxp, nyp) if img is not None else None),
-                {m: v[:nb] for m, v in outs.items()}) for img, outs, nb, nxp, nyp in res]
+        out = [((img.reshape(nxp, nyp) if img is not None else None), finish())
+               for img, finish, nxp, nyp in res]
         st = [int(status[k]) for k in range(len(jobs))]
         if rc and all(x != 0 for x in st):
             self._check(rc, "msf_run_group")

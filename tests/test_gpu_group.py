@@ -101,3 +101,32 @@ def test_group_reports_failing_runs(msf, ctx):
     finally:
         for r in runs + [other]:
             r["spec"].free()
+
+
+def test_group_with_every_fem_output(msf, ctx):
+    """Grouped runs with the other FEM modes, the correlograph (per position
+    too), the per-pixel images and the detector MTF, equal to each run alone."""
+    runs = _runs(msf, ctx, 128, 3, seed=91)
+    dom = np.linspace(0.0, 1.0, 11)
+    mtf = 1.0 / (1.0 + (dom / 0.4) ** 2)
+    extra = dict(modes=msf.FEM_MODES, correlograph=True, correlograph_each=True,
+                 pixel_images=True)
+    try:
+        ctx.configure(batch=4)
+        for r in runs:
+            r.update(extra)
+        runs[1]["fem"] = msf.make_fem(runs[1]["fem"].dr, mtf, dom, 12.0)
+        alone = [ctx.adfstem_fem(r["spec"], r["optics"], r["x_p"], r["y_p"], r["fem"],
+                                 modes=msf.FEM_MODES, detector=r["detector"], correlograph=True,
+                                 correlograph_each=True, pixel_images=True) for r in runs]
+        grouped, status = ctx.run_group(runs)
+        assert status == [0, 0, 0]
+        for (img_a, out_a), (img_g, out_g) in zip(alone, grouped):
+            np.testing.assert_array_equal(img_g, img_a)
+            assert sorted(out_a) == sorted(out_g)
+            for k in out_a:
+                np.testing.assert_array_equal(out_g[k], out_a[k])
+    finally:
+        ctx.configure()
+        for r in runs:
+            r["spec"].free()
