@@ -10,6 +10,7 @@
 #include "msf_internal.hpp"
 
 #include <algorithm>
+#include <functional>
 
 namespace msf_impl {
 
@@ -45,42 +46,49 @@ int run_group(msf_ctx* c, std::vector<GroupRun>& runs, GroupInfo* info)
     std::vector<GroupState> gs((size_t)R);
     while ((int)c->gplans.size() < R) c->gplans.emplace_back(new RunPlan);
     Mem heap;
-    const int nx = runs[0].sp ? runs[0].sp->dom.nx : 0, ny = runs[0].sp ? runs[0].sp->dom.ny : 0;
+    // Runs are set up one at a time on a second stream, each joining the
+    // step loop as soon as it is ready: the host work of a set-up (specimen
+    // build, pixel-group tables) and its device work overlap the steps of
+    // the runs already going.  An event orders each run's first launch after
+    // its set-up.
+    hipStream_t st = c->stream; // the steps
+    if (int r = ensure_stream2(c)) return r;
+    hipStream_t su = c->stream2; // the set-ups
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev()
+        {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } ready;
+    HIPCHK(c, hipEventCreateWithFlags(&ready.e, hipEventDisableTiming));
+    int nx = 0, ny = 0;
     Twiddles twx, twy;
-    int setup_rc = MSF_OK;
-    if (nx > 0) {
-        setup_rc = twiddles(c, heap, nx, twx);
-        if (setup_rc == MSF_OK) setup_rc = twiddles(c, heap, ny, twy);
-    }
     auto failed = [&](int r, int rc) {
         runs[(size_t)r].rc = rc;
         runs[(size_t)r].err = c->err;
     };
-    // set-up of every run: checks, pixel groups, vacuum waves, buffers
-    for (int r = 0; r < R; ++r) {
+    // set-up of run r on the set-up stream: its specimen (if it builds one),
+    // checks, pixel groups, vacuum waves, buffers
+    auto setup_on = [&](int r) -> int {
         GroupRun& g = runs[(size_t)r];
         GroupState& s = gs[(size_t)r];
         RunPlan& pl = *c->gplans[(size_t)r];
-        if (setup_rc != MSF_OK) {
-            failed(r, setup_rc);
-            continue;
+        if (g.prepare)
+            if (int rc = g.prepare(c, g)) return rc;
+        if (int rc = check_run(c, g.sp, g.o, g.det, g.fem, g.shifts, g.out, s.f)) return rc;
+        if (nx == 0) { // the first run fixes the group's lengths
+            if (int rc = twiddles(c, heap, s.f.nx, twx)) return rc;
+            if (int rc = twiddles(c, heap, s.f.ny, twy)) return rc;
+            nx = s.f.nx;
+            ny = s.f.ny;
         }
-        if (int rc = check_run(c, g.sp, g.o, g.det, g.fem, g.shifts, g.out, s.f)) {
-            failed(r, rc);
-            continue;
-        }
-        if (s.f.nx != nx || s.f.ny != ny || g.out.exit) {
-            failed(r, fail(c, MSF_ERR_INVALID,
-                           "the runs of a group share nx and ny and write no exit waves"));
-            continue;
-        }
-        int rc = prepare_groups(c, heap, pl, g.sp, g.o, g.det, g.fem, s.f);
+        if (s.f.nx != nx || s.f.ny != ny || g.out.exit)
+            return fail(c, MSF_ERR_INVALID,
+                        "the runs of a group share nx and ny and write no exit waves");
+        if (int rc = prepare_groups(c, heap, pl, g.sp, g.o, g.det, g.fem, s.f)) return rc;
         s.f.want_each = s.f.want_corr && g.out.corr_each && pl.nbins * s.f.nphi > 0;
-        if (rc == MSF_OK) rc = prepare_vacuum(c, heap, pl, g.sp, g.o, s.f);
-        if (rc != MSF_OK) {
-            failed(r, rc);
-            continue;
-        }
+        if (int rc = prepare_vacuum(c, heap, pl, g.sp, g.o, s.f)) return rc;
         pl.ncp = s.f.ncp;
         s.v.twx = twx;
         s.v.twy = twy;
@@ -90,17 +98,14 @@ int run_group(msf_ctx* c, std::vector<GroupRun>& runs, GroupInfo* info)
         s.sh.n_batches = (s.f.P + s.sh.B - 1) / s.sh.B;
         s.sh.col_tile = c->cfg.col_tile;
         s.sh.chains = 1;
-        if (int rb = prepare_buffers(c, heap, pl, s.f, s.sh, g.shifts, g.out, s.v)) {
-            failed(r, rb);
-            continue;
-        }
+        if (int rc = prepare_buffers(c, heap, pl, s.f, s.sh, g.shifts, g.out, s.v)) return rc;
         s.pa = pass_args(pl, g.sp, s.f, s.sh, s.v);
         s.ok = true;
-    }
+        return MSF_OK;
+    };
 
     // the step loop: per step one row launch and one column launch for each
     // class of shared flags (first slice / last slice, pruning)
-    hipStream_t st = c->stream;
     Prof prof; // off
     std::vector<Active> act;
     auto start = [&](int r, int64_t k) {
@@ -114,11 +119,38 @@ int run_group(msf_ctx* c, std::vector<GroupRun>& runs, GroupInfo* info)
         s.pa.bind(s.v.psi[0], s.v.shift + a.b0, a.nb, a.ra, a.ca);
         act.push_back(a);
     };
-    for (int r = 0; r < R; ++r)
-        if (gs[(size_t)r].ok) start(r, 0);
+    // sets up run r on the set-up stream and lets it join the steps
+    auto join = [&](int r) -> int {
+        struct OnSetup { // the context's stream is the set-up stream meanwhile
+            msf_ctx* c;
+            hipStream_t st;
+            ~OnSetup() { c->stream = st; }
+        } on{c, st};
+        c->stream = su;
+        const int rc = setup_on(r);
+        c->stream = st;
+        if (rc != MSF_OK) {
+            failed(r, rc);
+            gs[(size_t)r].ok = false;
+            return MSF_OK; // the run fails alone
+        }
+        HIPCHK(c, hipEventRecord(ready.e, su));
+        HIPCHK(c, hipStreamWaitEvent(st, ready.e, 0));
+        start(r, 0);
+        return MSF_OK;
+    };
     uint64_t launches = 0, steps = 0;
     int rc = MSF_OK;
-    while (!act.empty() && rc == MSF_OK) {
+    int next = 0; // the next run to set up
+    while (rc == MSF_OK) {
+        // one set-up per step: its host work runs while the GPU works through
+        // the steps already queued
+        if (next < R) rc = join(next++);
+        if (rc != MSF_OK) break;
+        if (act.empty()) {
+            if (next < R) continue;
+            break;
+        }
         ++steps;
         // row passes, classes (first slice, pruned)
         for (int cls = 0; cls < 4 && rc == MSF_OK; ++cls) {
@@ -190,17 +222,20 @@ int run_group(msf_ctx* c, std::vector<GroupRun>& runs, GroupInfo* info)
                      ? MSF_OK
                      : fail(c, MSF_ERR_DEVICE, "fem_finalize launch failed");
     }
-    if (rc == MSF_OK) {
-        const hipError_t e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
+    for (hipStream_t q : {su, st}) {
+        const hipError_t e = hipStreamSynchronize(q);
+        if (e != hipSuccess && rc == MSF_OK) {
             (void)hipGetLastError();
             rc = fail(c, MSF_ERR_DEVICE, std::string("group run: ") + hipGetErrorString(e));
         }
     }
-    // a device error ends every run that was still going
+    // a device error ends every run that was still going or not yet set up
     for (int r = 0; r < R; ++r) {
         GroupState& s = gs[(size_t)r];
-        if (!s.ok) continue;
+        if (!s.ok) {
+            if (r >= next && rc != MSF_OK) failed(r, rc);
+            continue;
+        }
         if (rc != MSF_OK) {
             failed(r, rc);
             continue;

@@ -485,6 +485,9 @@ int prepare_vacuum(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, cons
 // positions per batch by the rule: the fewest batches whose wave buffer stays
 // within 96 MiB, in whole groups of 16 positions
 int64_t batch_rule(const RunFlags& f);
+// the context's second stream and the events that order two batch chains
+// (created on first use)
+int ensure_stream2(msf_ctx* c);
 int prepare_buffers(msf_ctx* c, Mem& m, RunPlan& pl, const RunFlags& f, const Shape& sh,
                     const std::vector<int2>& shifts, const RunOut& out, RunViews& v);
 PassArgs pass_args(const RunPlan& pl, const msf_specimen* sp, const RunFlags& f, const Shape& sh,
@@ -509,6 +512,9 @@ int run_internal(msf_ctx* c, Mem& m, const msf_specimen* sp, const msf_optics* o
 RunOut run_out(double* adf, const msf_fem_outputs* fo);
 // One run of a group: arguments, outputs and, after run_group, its status.
 struct GroupRun {
+    // set-up before the checks, on the set-up stream (may be empty): e.g.
+    // builds the run's specimen and its shifts
+    std::function<int(msf_ctx*, GroupRun&)> prepare;
     const msf_specimen* sp = nullptr;
     const msf_optics* o = nullptr;
     const msf_detector* det = nullptr;

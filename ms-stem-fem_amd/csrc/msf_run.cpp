@@ -635,6 +635,14 @@ int launch_shape(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp,
     return MSF_OK;
 }
 
+int ensure_stream2(msf_ctx* c)
+{
+    if (!c->stream2) HIPCHK(c, hipStreamCreate(&c->stream2)); // default priority, as the first
+    for (hipEvent_t* e : {&c->ev_fork, &c->ev_join, &c->ev_fem[0], &c->ev_fem[1]})
+        if (!*e) HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return MSF_OK;
+}
+
 // Batch-chain buffers (grown as needed and kept for the next ordinary run)
 // and the run's accumulators, zeroed outside any captured graph.
 int prepare_buffers(msf_ctx* c, Mem& m, RunPlan& pl, const RunFlags& f, const Shape& sh,
@@ -694,11 +702,8 @@ int prepare_buffers(msf_ctx* c, Mem& m, RunPlan& pl, const RunFlags& f, const Sh
     }
     HIPCHK(c, hipMemcpyAsync(v.shift, shifts.data(), (size_t)P * sizeof(int2),
                              hipMemcpyHostToDevice, c->stream));
-    if (sh.chains == 2 && !c->stream2) {
-        HIPCHK(c, hipStreamCreate(&c->stream2)); // default priority, as the first stream
-        for (hipEvent_t* e : {&c->ev_fork, &c->ev_join, &c->ev_fem[0], &c->ev_fem[1]})
-            HIPCHK(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
-    }
+    if (sh.chains == 2)
+        if (int r = ensure_stream2(c)) return r;
     return MSF_OK;
 }
 

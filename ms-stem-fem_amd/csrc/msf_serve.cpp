@@ -91,50 +91,51 @@ void run_one(msf_ctx* sc, ServeJob& job, ServeDone& d, bool& built)
     reset_plan(sc);
 }
 
-// Several jobs of one grid size as one group (run_group): specimens built
-// first, then every run's slice passes in shared launches.
+// Several jobs of one grid size as one group (run_group): each job's
+// specimen is built as part of its set-up, which overlaps the slice passes of
+// the jobs set up before it.
 void run_grouped(msf_ctx* sc, std::vector<ServeJob>& jobs, std::vector<ServeDone>& d,
                  std::vector<bool>& built, GroupInfo& info)
 {
     const size_t n = jobs.size();
     std::vector<msf_specimen*> own(n, nullptr);
-    std::vector<GroupRun> runs;
-    std::vector<size_t> idx;
+    std::vector<GroupRun> runs(n);
     const int rc0 = msf_set_config(sc, &jobs[0].cfg);
     for (size_t k = 0; k < n; ++k) {
         ServeJob& job = jobs[k];
-        d[k].rc = rc0;
-        if (d[k].rc == MSF_OK && !job.spec) {
-            d[k].rc = msf_specimen_build(sc, &job.grid, job.q.data(), job.Z.data(),
-                                         (uint32_t)job.Z.size(), job.min_slice, &own[k]);
-            built[k] = d[k].rc == MSF_OK;
-        }
         if (!job.mtf.empty()) {
             job.fem.mtf = job.mtf.data();
             job.fem.mtf_domain = job.mtf_dom.data();
         }
-        GroupRun g;
-        g.sp = job.spec ? job.spec : own[k];
-        if (d[k].rc == MSF_OK &&
-            !raster_shifts(g.sp->dom, job.xp.data(), (uint32_t)job.xp.size(), job.yp.data(),
-                           (uint32_t)job.yp.size(), g.shifts)) {
-            d[k].rc = MSF_ERR_DOMAIN;
-            sc->err = "probe position below the domain minimum";
-        }
-        if (d[k].rc != MSF_OK) {
-            d[k].msg = sc->err;
-            continue;
-        }
+        GroupRun& g = runs[k];
         g.o = &job.optics;
         g.det = &job.det;
         g.fem = &job.fem;
         g.out = run_out(job.adf, &job.fo);
-        runs.push_back(std::move(g));
-        idx.push_back(k);
+        msf_specimen** slot = &own[k];
+        g.prepare = [&job, slot](msf_ctx* c, GroupRun& r) -> int {
+            const msf_specimen* sp = job.spec;
+            if (!sp) {
+                if (int e = msf_specimen_build(c, &job.grid, job.q.data(), job.Z.data(),
+                                               (uint32_t)job.Z.size(), job.min_slice, slot))
+                    return e;
+                sp = *slot;
+            }
+            r.sp = sp;
+            if (!raster_shifts(sp->dom, job.xp.data(), (uint32_t)job.xp.size(), job.yp.data(),
+                               (uint32_t)job.yp.size(), r.shifts))
+                return fail(c, MSF_ERR_DOMAIN, "probe position below the domain minimum");
+            return MSF_OK;
+        };
     }
     {
         std::lock_guard<std::recursive_mutex> lk(sc->mu);
-        if (hipSetDevice(sc->device) == hipSuccess) {
+        if (rc0 != MSF_OK) {
+            for (GroupRun& g : runs) {
+                g.rc = rc0;
+                g.err = sc->err;
+            }
+        } else if (hipSetDevice(sc->device) == hipSuccess) {
             (void)run_group(sc, runs, &info);
         } else {
             (void)hipGetLastError();
@@ -146,9 +147,10 @@ void run_grouped(msf_ctx* sc, std::vector<ServeJob>& jobs, std::vector<ServeDone
         release_group_plans(sc);
         reset_plan(sc);
     }
-    for (size_t r = 0; r < runs.size(); ++r) {
-        d[idx[r]].rc = runs[r].rc;
-        if (runs[r].rc != MSF_OK) d[idx[r]].msg = runs[r].err;
+    for (size_t k = 0; k < n; ++k) {
+        d[k].rc = runs[k].rc;
+        if (runs[k].rc != MSF_OK) d[k].msg = runs[k].err;
+        built[k] = own[k] != nullptr;
     }
     for (msf_specimen* sp : own)
         if (sp) msf_specimen_free(sp);
