@@ -44,7 +44,7 @@ struct ColPassArgs {
     const float2* py;       // [ny] propagator along y of this slice, scaled by 1/s
     const int2* colkeep;    // [ncp] (c, d): kept rows [0, c) U [nx - d, nx) of compact column jc
     const float2* tw;       // stage twiddle table of length nx (stage_twiddles)
-    const float2* psi0;     // [nx][ny] vacuum exit wave in k-space (last pass only)
+    const float2* psi0;     // [ny][nx] vacuum exit wave in k-space, column-major (last pass only)
     const int2* shift;      // [B] probe shifts (last pass only)
     const float2* twx_nat;  // [nx] roots W_nx^e (last pass only)
     const float2* twy;      // [ny] roots W_ny^e (last pass only)
@@ -211,8 +211,8 @@ hipError_t launch_vacuum_step(double2* a, int nx, int ny, const double2* px, con
 // FP64 transmission update of `count` real-space waves by one slice's t
 hipError_t launch_transmit_f64(double2* a, const double2* t, int64_t m, int count, int mode,
                                double s, hipStream_t st);
-// out = (float2) a
-hipError_t launch_to_f32(const double2* a, int64_t n, float2* out, hipStream_t s);
+// FP64 [nx][ny] -> FP32 [ny][nx]
+hipError_t launch_to_f32_transposed(const double2* a, int nx, int ny, float2* out, hipStream_t s);
 // out[i][j'] = (float2) a[i][j' mod ny], j' < 2*ny
 hipError_t launch_to_f32_doubled_rows(const double2* a, int nx, int ny, float2* out,
                                       hipStream_t s);

@@ -386,15 +386,25 @@ hipError_t launch_vacuum_step(double2* a, int nx, int ny, const double2* px, con
     return hipGetLastError();
 }
 
-__global__ void to_f32_kernel(const double2* a, int64_t n, float2* out)
+
+
+// out[j * nx + i] = a[i * ny + j]: the column-major copy a column pass reads
+// along its column
+__global__ void to_f32_transposed_kernel(const double2* a, int nx, int ny, float2* out)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) out[k] = float2{(float)a[k].x, (float)a[k].y};
+    if (k >= (int64_t)nx * ny) return;
+    const int64_t j = k / nx;
+    const int i = (int)(k % nx);
+    const double2 v = a[(int64_t)i * ny + j];
+    out[k] = float2{(float)v.x, (float)v.y};
 }
 
-hipError_t launch_to_f32(const double2* a, int64_t n, float2* out, hipStream_t s)
+hipError_t launch_to_f32_transposed(const double2* a, int nx, int ny, float2* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, out);
+    const int64_t n = (int64_t)nx * ny;
+    hipLaunchKernelGGL(to_f32_transposed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       s, a, nx, ny, out);
     return hipGetLastError();
 }
 

@@ -77,7 +77,7 @@ struct RunPlan {
     bool valid = false;      // vacuum waves of (spec_id, optics) are current
     int ncp = 0, G = 0, nbins = 0;
     // vacuum waves: u0 of every slice [S][nx][2ny] (s * real space, rows
-    // doubled) and the k-space vacuum exit wave [nx][ny], FP32
+    // doubled) and the k-space vacuum exit wave, column-major [ny][nx], FP32
     DevBuf U0, Psi0;
     float2* u0 = nullptr;
     float2* psi0 = nullptr;

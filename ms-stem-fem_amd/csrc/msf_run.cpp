@@ -517,7 +517,7 @@ int prepare_vacuum(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, cons
                                           c->stream));
         if (n + 1 < S) HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, true, twx.f64, twy.f64, c->stream));
     }
-    HIPCHK(c, msf::launch_to_f32(v, M, pl.psi0, c->stream));
+    HIPCHK(c, msf::launch_to_f32_transposed(v, nx, ny, pl.psi0, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     pl.spec_id = sp->id;
     pl.optics = *o;

@@ -410,7 +410,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
             const uint32_t i = (uint32_t)(t + T * p);
             const float2 wx = s_nat[(i * (uint32_t)sh.x) % (uint32_t)NX];
             const float2 w = float2{wx.x * wy.x - wx.y * wy.y, wx.x * wy.y + wx.y * wy.x};
-            const float2 v0 = a.psi0[(int64_t)i * a.ny + j];
+            const float2 v0 = a.psi0[(int64_t)j * NX + i];
             x[p].x += v0.x * w.x - v0.y * w.y;
             x[p].y += v0.x * w.y + v0.y * w.x;
         }
