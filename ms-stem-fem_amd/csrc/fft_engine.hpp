@@ -202,17 +202,19 @@ template <bool INV, typename C> struct Dft<16, INV, C> {
 // ------------------------------------------------------------ LDS layout
 // one padding slot every 8 elements breaks the power-of-two strides of the
 // Stockham write pattern across LDS banks
-__host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 3); }
-__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3) + 1; }
-// lds_pad(r + S * p) for r >= 0: with S a multiple of 8 the padding of the
-// p-th element is (r >> 3) + p S / 8, so the address is one per-lane base
-// plus a constant per p (an immediate offset, no address register per p)
-template <int S> __device__ __forceinline__ int lds_pad_at(int r, int p)
+// (PS: one padding slot every 2^PS elements; 3 unless a kernel asks for a
+// sparser padding to fit more rows in LDS)
+template <int PS = 3> __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> PS); }
+template <int PS = 3> __host__ __device__ constexpr int lds_len(int n) { return n + (n >> PS) + 1; }
+// lds_pad(r + S * p) for r >= 0: with S a multiple of 2^PS the padding of
+// the p-th element is (r >> PS) + p S / 2^PS, so the address is one per-lane
+// base plus a constant per p (an immediate offset, no address register per p)
+template <int S, int PS = 3> __device__ __forceinline__ int lds_pad_at(int r, int p)
 {
-    if constexpr (S % 8 == 0) {
-        return (r + (r >> 3)) + p * (S + S / 8);
+    if constexpr (S % (1 << PS) == 0) {
+        return (r + (r >> PS)) + p * (S + (S >> PS));
     } else {
-        return lds_pad(r + S * p);
+        return lds_pad<PS>(r + S * p);
     }
 }
 // twiddle tables in LDS: stage reads W^(q k N/(NS R)) step through the table
@@ -240,7 +242,8 @@ template <bool WAVE> __device__ __forceinline__ void lds_sync()
 // [q-1][k], entry W_{NS R}^{q k}, q = 1..R-1, k = 0..NS-1, at offset TWO in
 // the concatenated table of all stages (stage_twiddles): lanes with
 // consecutive k read consecutive words, and the q offsets are immediates.
-template <typename C, int N, int E, int R, int NS, int TWO, bool INV, bool LAST, bool WAVE>
+template <typename C, int N, int E, int R, int NS, int TWO, bool INV, bool LAST, bool WAVE,
+          int PS = 3>
 __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
 {
     constexpr int T = N / E;
@@ -282,22 +285,24 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
             const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
             for (int q = 0; q < R; ++q)
-                lds[FOLD ? lds_pad_at<NS>(base, q) : lds_pad(base + q * NS)] = y[m * R + q];
+                lds[FOLD ? lds_pad_at<NS, PS>(base, q) : lds_pad<PS>(base + q * NS)] = y[m * R + q];
         }
         lds_sync<WAVE>();
 #pragma unroll
-        for (int p = 0; p < E; ++p) x[p] = lds[FOLD ? lds_pad_at<T>(t, p) : lds_pad(t + T * p)];
+        for (int p = 0; p < E; ++p)
+            x[p] = lds[FOLD ? lds_pad_at<T, PS>(t, p) : lds_pad<PS>(t + T * p)];
     }
 }
 
 // stage offsets in the concatenated twiddle table: stage s with span NS > 1
 // adds (R-1)*NS entries; the first stage (NS = 1) needs none
-template <typename C, int N, int E, bool INV, bool WAVE, int NS, int TWO, int R, int... Rest>
+template <typename C, int N, int E, bool INV, bool WAVE, int PS, int NS, int TWO, int R,
+          int... Rest>
 __device__ __forceinline__ void run_stages(C (&x)[E], C* lds, const C* tw, int t)
 {
-    fft_stage<C, N, E, R, NS, TWO, INV, sizeof...(Rest) == 0, WAVE>(x, lds, tw, t);
+    fft_stage<C, N, E, R, NS, TWO, INV, sizeof...(Rest) == 0, WAVE, PS>(x, lds, tw, t);
     if constexpr (sizeof...(Rest) > 0)
-        run_stages<C, N, E, INV, WAVE, NS * R, (NS > 1 ? TWO + (R - 1) * NS : 0), Rest...>(
+        run_stages<C, N, E, INV, WAVE, PS, NS * R, (NS > 1 ? TWO + (R - 1) * NS : 0), Rest...>(
             x, lds, tw, t);
 }
 
@@ -315,10 +320,10 @@ template <int N> struct AltPlan { static constexpr bool exists = false; };
         static constexpr int N = N_, E = E_, T = N_ / E_;                                      \
         static constexpr int kRadix[] = {__VA_ARGS__};                                         \
         static constexpr int kStages = sizeof(kRadix) / sizeof(int);                           \
-        template <bool INV, typename C, bool WAVE = false>                                     \
+        template <bool INV, typename C, bool WAVE = false, int PS = 3>                         \
         static __device__ __forceinline__ void run(C (&x)[E_], C* lds, const C* tw, int t)     \
         {                                                                                      \
-            run_stages<C, N_, E_, INV, WAVE, 1, 0, __VA_ARGS__>(x, lds, tw, t);                \
+            run_stages<C, N_, E_, INV, WAVE, PS, 1, 0, __VA_ARGS__>(x, lds, tw, t);            \
         }                                                                                      \
     };
 #define MSF_PLAN(N_, E_, ...) MSF_PLAN_AS(Plan, N_, E_, __VA_ARGS__)

@@ -123,10 +123,18 @@ template <int NY, int V, int RT> __host__ __device__ constexpr bool row_is_lean(
     // one row per workgroup: nothing to share, tables are read from L2
     return row_lean<NY, V>() || RT / PlanV<NY, V>::T == 1;
 }
+// padding of a row's exchange buffer: one slot every 2^row_pad_shift values.
+// The alternate 1024 row pass pads every 32 (8.3 KB per row instead of
+// 9.2 KB): its pass takes as long, but the smaller workgroups leave room for
+// the other batch chain's column pass on the same CUs (1024^2 job +3 %).
+template <int NY, int V> __host__ __device__ constexpr int row_pad_shift()
+{
+    return (NY == 1024 && V == 1) ? 5 : 3;
+}
 template <int NY, int V, int RT> __host__ __device__ constexpr int row_smem_len()
 {
     return (row_is_lean<NY, V, RT>() ? 0 : plan_tw_len<PlanV<NY, V>>() + NY) +
-           (RT / PlanV<NY, V>::T) * lds_len(NY);
+           (RT / PlanV<NY, V>::T) * lds_len<row_pad_shift<NY, V>()>(NY);
 }
 
 // One row-pass workgroup: logical block `blk` of the pass (XCD-banded over
@@ -137,7 +145,8 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
     using P = PlanV<NY, V>;
     constexpr int E = P::E, T = P::T;
     constexpr int RPW = RT / T;
-    constexpr int LEN = lds_len(NY);
+    constexpr int PS = row_pad_shift<NY, V>();
+    constexpr int LEN = lds_len<PS>(NY);
     constexpr bool LEAN = row_is_lean<NY, V, RT>();
     float2* s_tw = smem;
     float2* s_t = smem + (LEAN ? 0 : plan_tw_len<P>());
@@ -189,7 +198,7 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
                 x[p] = (lo || up) ? v : float2{0.f, 0.f};
             }
         }
-        P::template run<true, float2, (T <= 64)>(x, lds, tw, t);
+        P::template run<true, float2, (T <= 64), PS>(x, lds, tw, t);
     }
 
     const int2 sh = a.shift[b];
@@ -217,7 +226,7 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
         }
     }
 
-    P::template run<false, float2, (T <= 64)>(x, lds, tw, t);
+    P::template run<false, float2, (T <= 64), PS>(x, lds, tw, t);
 
     if (active) {
 #pragma unroll
