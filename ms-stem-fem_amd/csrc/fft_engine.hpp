@@ -12,7 +12,9 @@
 // for butterfly j = t + T*m, m = 0..E/R-1.
 //
 // Twiddles come from per-stage tables (see fft_stage) built on the host in
-// FP64 and rounded once to the working precision, held in LDS.
+// FP64 and rounded once to the working precision, held in LDS or read from
+// L2; the plans with 16 values per lane load a few entries per stage and
+// derive the rest by one complex product each.
 // FFT sign conventions match FFTW: forward = exp(-2 pi i), inverse =
 // exp(+2 pi i), both unnormalised.
 #pragma once
@@ -168,6 +170,19 @@ template <int EXP, bool INV, typename C> __device__ __forceinline__ C mul_w16(C 
     }
 }
 
+// multiplication by W16^e for an exponent known after unrolling
+template <bool INV, typename C> __device__ __forceinline__ C mul_w16_at(C a, int e)
+{
+    using R = decltype(a.x);
+    e &= 15;
+    if (e == 0) return a;
+    if (e == 4) return mul_mi<INV>(a);
+    if (e == 8) return C{-a.x, -a.y};
+    if (e == 12) return mul_mi<!INV>(a);
+    const R c = R(cos16(e)), s = R(cos16(e - 4));
+    return INV ? C{a.x * c - a.y * s, a.y * c + a.x * s} : C{a.x * c + a.y * s, a.y * c - a.x * s};
+}
+
 // radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2,
 // X[k] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2]
 template <bool INV, typename C> struct Dft<16, INV, C> {
@@ -249,13 +264,52 @@ __device__ __forceinline__ void fft_stage(C (&x)[E], C* lds, const C* tw, int t)
     constexpr int T = N / E;
     constexpr int BPT = E / R; // butterflies per lane
     static_assert(E % R == 0, "radix must divide the per-lane element count");
+    // Derived twiddles (16-value plans): a lane's butterflies m >= 1 of a
+    // stage whose spans hold them whole take W^{q(t + T m)} = W^{q t} times
+    // the constant W16^{16 q T m / (NS R)} (MDER), or the twiddles of
+    // butterfly 0 when T is a multiple of NS (SAME: (t + T m) % NS = t % NS);
+    // a radix-16 stage with one butterfly per lane takes W^{q k}, q = 4a + b,
+    // as W^{4a k} W^{b k} (QDER).  Each derived twiddle is one product of
+    // table entries or of a table entry and a W16 constant (about one ulp
+    // more than the table's); the 1024 alternate plan loads 9 instead of 27
+    // per transform, and its row pass, bound by load latency at 4 waves per
+    // SIMD, goes from 71.7 to 59.9 us per 17 positions.
+    constexpr bool DERIVE = E >= 16 && NS > 1;
+    constexpr bool SAME = DERIVE && BPT > 1 && T % NS == 0;
+    constexpr bool MDER = DERIVE && !SAME && BPT > 1 && T * BPT <= NS && (16 * T) % (NS * R) == 0;
+    constexpr bool QDER = DERIVE && R == 16 && BPT == 1;
+    C w0[R];
     C y[E];
 #pragma unroll
     for (int m = 0; m < BPT; ++m) {
         C v[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) v[q] = x[m + q * BPT];
-        if constexpr (NS > 1) {
+        if constexpr (MDER || SAME) {
+            if (m == 0) {
+                const C* twk = tw + TWO + t % NS;
+#pragma unroll
+                for (int q = 1; q < R; ++q) w0[q] = twk[(q - 1) * NS];
+            }
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const C w = SAME ? w0[q] : mul_w16_at<false>(w0[q], (16 * q * T * m) / (NS * R));
+                v[q] = INV ? cmulc(v[q], w) : cmul(v[q], w);
+            }
+        } else if constexpr (QDER) {
+            const C* twk = tw + TWO + t % NS;
+            C wb[4], wa[4];
+#pragma unroll
+            for (int b = 1; b < 4; ++b) wb[b] = twk[(b - 1) * NS];
+#pragma unroll
+            for (int a4 = 1; a4 < 4; ++a4) wa[a4] = twk[(4 * a4 - 1) * NS];
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const int a4 = q / 4, b = q % 4;
+                const C w = a4 == 0 ? wb[b] : (b == 0 ? wa[a4] : cmul(wa[a4], wb[b]));
+                v[q] = INV ? cmulc(v[q], w) : cmul(v[q], w);
+            }
+        } else if constexpr (NS > 1) {
             const C* twk = tw + TWO + (t + T * m) % NS;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
