@@ -24,11 +24,21 @@
 
 namespace msf {
 
+// Pair layout of the rows the row pass reads beside the wave (ny even): a
+// lane holding elements j and j + ny/2 of a row fetches both with one 16-byte
+// read.  Transmission rows store element j at pair_pos(j, ny); u0 rows hold,
+// for every shift r in [0, ny), the pair (r, (r + ny/2) mod ny).
+__host__ __device__ inline int pair_pos(int j, int n)
+{
+    return j < n / 2 ? 2 * j : 2 * (j - n / 2) + 1;
+}
+
 struct RowPassArgs {
     float2* psi;            // [B][nx][ncp] scattered wave phi
-    const float2* t;        // [nx][ny] (t_re/s - 1/s, t_im/s) of this slice
-    const float2* u0;       // [nx][2*ny] vacuum wave entering this slice, s * real space,
-                            // each row stored twice in a row
+    const float2* t;        // [nx][ny] (t_re/s - 1/s, t_im/s) of this slice, rows in the
+                            // pair layout
+    const float2* u0;       // [nx][ny][2] vacuum wave entering this slice, s * real space,
+                            // rows as shift pairs (pair_pos)
 
     const int2* shift;      // [B] probe shifts of the batch
     const float2* tw;       // stage twiddle table of length ny (stage_twiddles)
@@ -202,9 +212,10 @@ hipError_t launch_exp_transmit(double2* a, int64_t n, double scale, hipStream_t 
 // zero outside the kept ranges
 hipError_t launch_apply_keep(double2* a, int nx, int ny, int count, const int2* keep,
                              hipStream_t s);
-// t = a / nxny (FP64, kept for queries); t32 = (t_re/s - 1/s, t_im/s) in FP32
-hipError_t launch_finish_transmit(double2* a, int64_t n, double nxny, double s, float2* t32,
-                                  hipStream_t st);
+// t = a / nxny (FP64, kept for queries); t32 = (t_re/s - 1/s, t_im/s) in FP32,
+// each row of ny in the pair layout (pair_pos)
+hipError_t launch_finish_transmit(double2* a, int64_t n, int ny, double nxny, double s,
+                                  float2* t32, hipStream_t st);
 // one vacuum slice step in k-space: a *= keep ? P_x[i] P_y[j] * scale : 0
 hipError_t launch_vacuum_step(double2* a, int nx, int ny, const double2* px, const double2* py,
                               const int2* keep, double scale, hipStream_t s);
@@ -213,9 +224,9 @@ hipError_t launch_transmit_f64(double2* a, const double2* t, int64_t m, int coun
                                double s, hipStream_t st);
 // FP64 [nx][ny] -> FP32 [ny][nx]
 hipError_t launch_to_f32_transposed(const double2* a, int nx, int ny, float2* out, hipStream_t s);
-// out[i][j'] = (float2) a[i][j' mod ny], j' < 2*ny
-hipError_t launch_to_f32_doubled_rows(const double2* a, int nx, int ny, float2* out,
-                                      hipStream_t s);
+// out[i][2r + h] = (float2) a[i][(r + h ny/2) mod ny], r < ny, h = 0, 1: the
+// pair of elements r and r + ny/2 for every shift r of a row (u0 rows)
+hipError_t launch_to_f32_shift_pairs(const double2* a, int nx, int ny, float2* out, hipStream_t s);
 // a *= f / sqrt(*norm2)
 hipError_t launch_scale_f64(double2* a, int64_t n, const double* norm2, double f, hipStream_t s);
 

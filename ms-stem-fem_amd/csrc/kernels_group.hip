@@ -23,7 +23,7 @@ template <int NY, int V, bool FIRST, int TMODE, bool PRUNE, int RT>
 __global__ void __launch_bounds__(RT)
 __attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_group_kernel(RowGroupArgs g)
 {
-    __shared__ float2 smem[row_smem_len<NY, V, RT>()];
+    __shared__ __align__(16) float2 smem[row_smem_len<NY, V, RT>()];
     constexpr int RPW = RT / PlanV<NY, V>::T;
     const int k = group_entry(g.off, g.n, (int)blockIdx.x);
     const RowPassArgs& a = g.a[k];

@@ -341,20 +341,22 @@ hipError_t launch_apply_keep(double2* a, int nx, int ny, int count, const int2* 
 // slice.cpp:1741-1748 (/ NxNy).  The FP32 copy used by the slice loop holds
 // t' = t/s (adfstem.cpp:1936,1949 divide by sqrtNxNy) as (t're - 1/s, t'im):
 // the deviation from vacuum keeps full FP32 relative precision.
-__global__ void finish_transmit_kernel(double2* a, int64_t n, double nxny, double s, float2* t32)
+__global__ void finish_transmit_kernel(double2* a, int64_t n, int ny, double nxny, double s,
+                                       float2* t32)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double2 t{a[k].x / nxny, a[k].y / nxny};
     a[k] = t;
-    t32[k] = float2{(float)(t.x / s - 1.0 / s), (float)(t.y / s)};
+    const int j = (int)(k % ny);
+    t32[k - j + pair_pos(j, ny)] = float2{(float)(t.x / s - 1.0 / s), (float)(t.y / s)};
 }
 
-hipError_t launch_finish_transmit(double2* a, int64_t n, double nxny, double s, float2* t32,
-                                  hipStream_t st)
+hipError_t launch_finish_transmit(double2* a, int64_t n, int ny, double nxny, double s,
+                                  float2* t32, hipStream_t st)
 {
     hipLaunchKernelGGL(finish_transmit_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       a, n, nxny, s, t32);
+                       a, n, ny, nxny, s, t32);
     return hipGetLastError();
 }
 
@@ -408,22 +410,24 @@ hipError_t launch_to_f32_transposed(const double2* a, int nx, int ny, float2* ou
     return hipGetLastError();
 }
 
-__global__ void to_f32_doubled_kernel(const double2* a, int nx, int ny, float2* out)
+__global__ void to_f32_shift_pairs_kernel(const double2* a, int nx, int ny, float2* out)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int64_t)nx * 2 * ny) return;
     const int64_t i = k / (2 * ny);
-    const int j = (int)(k % (2 * ny));
-    const double2 v = a[i * ny + (j < ny ? j : j - ny)];
+    const int m = (int)(k % (2 * ny));
+    // pair r = m / 2 holds a[i][r] and a[i][(r + ny/2) mod ny]
+    int j = m / 2 + (m % 2) * (ny / 2);
+    if (j >= ny) j -= ny;
+    const double2 v = a[i * ny + j];
     out[k] = float2{(float)v.x, (float)v.y};
 }
 
-hipError_t launch_to_f32_doubled_rows(const double2* a, int nx, int ny, float2* out,
-                                      hipStream_t s)
+hipError_t launch_to_f32_shift_pairs(const double2* a, int nx, int ny, float2* out, hipStream_t s)
 {
     const int64_t n = (int64_t)nx * 2 * ny;
-    hipLaunchKernelGGL(to_f32_doubled_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       a, nx, ny, out);
+    hipLaunchKernelGGL(to_f32_shift_pairs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       s, a, nx, ny, out);
     return hipGetLastError();
 }
 

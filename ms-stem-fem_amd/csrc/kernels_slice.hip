@@ -43,7 +43,7 @@ template <int N, int VR, int VC, bool FIRST, bool LAST, int TMODE, bool PR, bool
 __global__ void __launch_bounds__(RT) fused_pass_kernel(RowPassArgs ra, ColPassArgs ca, int nr, int nc)
 {
     constexpr int LR = row_smem_len<N, VR, RT>(), LC = col_smem_len<N, VC, LAST, W>();
-    __shared__ float2 smem[LR > LC ? LR : LC];
+    __shared__ __align__(16) float2 smem[LR > LC ? LR : LC];
     int id = 0;
     if (fused_decode((int)blockIdx.x, nr, nc, id))
         row_body<N, VR, FIRST, TMODE, PR, RT>(ra, id, smem);

@@ -361,7 +361,7 @@ int msf_specimen_build(msf_ctx* c, const msf_grid* grid, const double* q, const 
             if (!e) e = msf::fft2d_f64(sig, nx, ny, k, false, twx, twy, c->stream);
             if (!e) e = msf::launch_apply_keep(sig, nx, ny, k, dbw.as<int2>(), c->stream);
             if (!e) e = msf::fft2d_f64(sig, nx, ny, k, true, twx, twy, c->stream);
-            if (!e) e = msf::launch_finish_transmit(sig, (int64_t)k * M, d.nxny, d.sqrt_nxny,
+            if (!e) e = msf::launch_finish_transmit(sig, (int64_t)k * M, ny, d.nxny, d.sqrt_nxny,
                                                     sp->t32.as<float2>() + (int64_t)s0 * M, c->stream);
             // the atom list buffer is reused by the next chunk
             if (!e) e = hipStreamSynchronize(c->stream);

@@ -267,7 +267,7 @@ struct msf_specimen {
     int n_slices = 0;
     std::vector<double> lower, thick;
     std::vector<uint32_t> counts;
-    msf_impl::DevBuf t32;  // [S][nx][ny] float2, t / sqrtNxNy
+    msf_impl::DevBuf t32;  // [S][nx][ny] float2, t / sqrtNxNy, rows in the pair layout (pair_pos)
     msf_impl::DevBuf t64;  // [S][nx][ny] double2, t as built
     msf_impl::DevBuf px;   // [S][nx] float2
     msf_impl::DevBuf py;   // [S][ny] float2, P_y / sqrtNxNy

@@ -509,7 +509,7 @@ int prepare_vacuum(msf_ctx* c, Mem& m, RunPlan& pl, const msf_specimen* sp, cons
     if (int r = take_as<float2>(c, m, pl.Psi0, (size_t)M, &pl.psi0)) return r;
     HIPCHK(c, msf::launch_scale_f64(v, M, norm2, d.sqrt_nxny, c->stream));
     for (int n = 0; n < S; ++n) {
-        HIPCHK(c, msf::launch_to_f32_doubled_rows(v, nx, ny, pl.u0 + (int64_t)n * 2 * M, c->stream));
+        HIPCHK(c, msf::launch_to_f32_shift_pairs(v, nx, ny, pl.u0 + (int64_t)n * 2 * M, c->stream));
         HIPCHK(c, msf::fft2d_f64(v, nx, ny, 1, false, twx.f64, twy.f64, c->stream));
         HIPCHK(c, msf::launch_vacuum_step(v, nx, ny, sp->px64.as<double2>() + (int64_t)n * nx,
                                           sp->py64.as<double2>() + (int64_t)n * ny,

@@ -204,15 +204,8 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
     const int2 sh = a.shift[b];
     int ii = i - sh.x;
     if (ii < 0) ii += a.nx;
-    // u0 rows are stored twice over (length 2*NY): the circular shift by sy is
-    // a plain offset
-    const float2* urow = a.u0 + (int64_t)ii * (2 * NY) + (NY - sh.y) + t;
     const float is = a.inv_s;
-#pragma unroll
-    for (int p = 0; p < E; ++p) {
-        const int j = t + T * p;
-        const float2 tv = tt[j];
-        const float2 u0 = urow[T * p];
+    auto transmit = [&](int p, float2 tv, float2 u0) {
         const float da = tv.x * u0.x - tv.y * u0.y;
         const float db = (TMODE == 0) ? tv.x * u0.y + tv.y * (u0.x * is + da) : tv.x * u0.y + tv.y * u0.x;
         if constexpr (FIRST) {
@@ -224,6 +217,20 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
             const float im = (TMODE == 0) ? tr * u.y + tv.y * re : tr * u.y + tv.y * u.x;
             x[p] = float2{re + da, im + db};
         }
+    };
+    // Transmission and u0 rows are stored as pairs (pair_pos): a lane's
+    // elements p and p + E/2 are j and j + NY/2, and one 16-byte read fetches
+    // both, of t at j and of u0 at the shifted r = (j - sy) mod NY.
+    const float2* urow = a.u0 + (int64_t)ii * (2 * NY);
+#pragma unroll
+    for (int p = 0; p < E / 2; ++p) {
+        const int j = t + T * p;
+        int r = j - sh.y;
+        if (r < 0) r += NY;
+        const float4 tv = *reinterpret_cast<const float4*>(tt + 2 * j);
+        const float4 uv = *reinterpret_cast<const float4*>(urow + 2 * r);
+        transmit(p, float2{tv.x, tv.y}, float2{uv.x, uv.y});
+        transmit(p + E / 2, float2{tv.z, tv.w}, float2{uv.z, uv.w});
     }
 
     P::template run<false, float2, (T <= 64), PS>(x, lds, tw, t);
@@ -249,7 +256,7 @@ template <int NY, int V, bool FIRST, int TMODE, bool PRUNE, int RT>
 __global__ void __launch_bounds__(RT)
 __attribute__((amdgpu_waves_per_eu((row_waves<NY, V>())))) row_pass_kernel(RowPassArgs a)
 {
-    __shared__ float2 smem[row_smem_len<NY, V, RT>()];
+    __shared__ __align__(16) float2 smem[row_smem_len<NY, V, RT>()];
     row_body<NY, V, FIRST, TMODE, PRUNE, RT>(a, (int)blockIdx.x, smem);
 }
 
