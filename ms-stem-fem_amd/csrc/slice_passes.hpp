@@ -131,6 +131,16 @@ template <int NY, int V> __host__ __device__ constexpr int row_pad_shift()
 {
     return (NY == 1024 && V == 1) ? 5 : 3;
 }
+// Row passes that read every transmission and u0 pair of a lane before the
+// first product (the alternate 1024 plan: row pass 57.5 -> 53.4 us; the
+// primary 512 plan: the 512^2 job +0.7 %); the others interleave reads and
+// products, which keeps their registers lower (384: 128 against 149, the
+// alternate 256 plan: 101 against 128 with spills; the uneven 256-512 job
+// set +1 %).
+template <int NY, int V> __host__ __device__ constexpr bool row_reads_ahead()
+{
+    return (NY == 1024 && V == 1) || (NY == 512 && V == 0);
+}
 template <int NY, int V, int RT> __host__ __device__ constexpr int row_smem_len()
 {
     return (row_is_lean<NY, V, RT>() ? 0 : plan_tw_len<PlanV<NY, V>>() + NY) +
@@ -182,24 +192,17 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
 
     // Transmission and u0 rows are stored as pairs (pair_pos): a lane's
     // elements p and p + E/2 are j and j + NY/2, and one 16-byte read fetches
-    // both, of t at j and of u0 at the shifted r = (j - sy) mod NY.  The
-    // shift and the u0 row address are formed before the inverse transform,
-    // so the reads after it do not wait for the shift (1024^2 row pass
-    // 57.5 -> 53.4 us); issuing the reads themselves earlier spills.
+    // both, of t at j and of u0 at the shifted r = (j - sy) mod NY.
     const int2 sh = a.shift[b];
     int ii = i - sh.x;
     if (ii < 0) ii += a.nx;
     const float2* urow = a.u0 + (int64_t)ii * (2 * NY);
-    float4 tq[E / 2], uq[E / 2];
-    auto fetch = [&]() {
-#pragma unroll
-        for (int p = 0; p < E / 2; ++p) {
-            const int j = t + T * p;
-            int r = j - sh.y;
-            if (r < 0) r += NY;
-            tq[p] = *reinterpret_cast<const float4*>(tt + 2 * j);
-            uq[p] = *reinterpret_cast<const float4*>(urow + 2 * r);
-        }
+    auto pair_at = [&](int p, float4& tv, float4& uv) {
+        const int j = t + T * p;
+        int r = j - sh.y;
+        if (r < 0) r += NY;
+        tv = *reinterpret_cast<const float4*>(tt + 2 * j);
+        uv = *reinterpret_cast<const float4*>(urow + 2 * r);
     };
 
     float2 x[E];
@@ -222,7 +225,6 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
         }
         P::template run<true, float2, (T <= 64), PS>(x, lds, tw, t);
     }
-    fetch();
 
     const float is = a.inv_s;
     auto transmit = [&](int p, float2 tv, float2 u0) {
@@ -238,10 +240,25 @@ __device__ __forceinline__ void row_body(const RowPassArgs& a, int blk, float2* 
             x[p] = float2{re + da, im + db};
         }
     };
+    if constexpr (row_reads_ahead<NY, V>()) {
+        // all reads issued before the first product: more of them in flight,
+        // for more registers
+        float4 tq[E / 2], uq[E / 2];
 #pragma unroll
-    for (int p = 0; p < E / 2; ++p) {
-        transmit(p, float2{tq[p].x, tq[p].y}, float2{uq[p].x, uq[p].y});
-        transmit(p + E / 2, float2{tq[p].z, tq[p].w}, float2{uq[p].z, uq[p].w});
+        for (int p = 0; p < E / 2; ++p) pair_at(p, tq[p], uq[p]);
+#pragma unroll
+        for (int p = 0; p < E / 2; ++p) {
+            transmit(p, float2{tq[p].x, tq[p].y}, float2{uq[p].x, uq[p].y});
+            transmit(p + E / 2, float2{tq[p].z, tq[p].w}, float2{uq[p].z, uq[p].w});
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < E / 2; ++p) {
+            float4 tv, uv;
+            pair_at(p, tv, uv);
+            transmit(p, float2{tv.x, tv.y}, float2{uv.x, uv.y});
+            transmit(p + E / 2, float2{tv.z, tv.w}, float2{uv.z, uv.w});
+        }
     }
 
     P::template run<false, float2, (T <= 64), PS>(x, lds, tw, t);
