@@ -308,9 +308,18 @@ template <int NX, int V = 0> struct ColTile {
 // column stride of the LDS tile: a 16-lane group of the coalesced load/store
 // phases covers W' = min(W, 16) columns x 16/W' rows; a stride congruent to
 // 16/W' modulo 16 (8-byte words) puts those 16 lanes on distinct bank pairs
-template <int NX, int W> __host__ __device__ constexpr int col_len()
+// padding of a column in the tile: one slot every 2^col_pad_shift values.
+// The alternate 1024 column pass pads every 16 (a 4-column tile of 34.9 KB
+// instead of 37 KB): the pass 45.5 -> 44.0 us and the 1024^2 job +3-4 %
+// (every 32: +2 %), the smaller tiles sharing CUs with the other batch
+// chain's row pass.
+template <int NX, int V> __host__ __device__ constexpr int col_pad_shift()
 {
-    constexpr int base = lds_len(NX);
+    return (NX == 1024 && V == 1) ? 4 : 3;
+}
+template <int NX, int V, int W> __host__ __device__ constexpr int col_len()
+{
+    constexpr int base = lds_len<col_pad_shift<NX, V>()>(NX);
     constexpr int want = (16 / (W < 16 ? W : 16)) % 16;
     return base + ((want - base % 16) + 16) % 16;
 }
@@ -323,7 +332,7 @@ template <int NX, int V, int W> __host__ __device__ constexpr bool col_lean()
 {
     if (W == 1) return true;      // one column: nothing to share, tables are read from L2
     if (W == 4) return NX >= 512; // small tiles: tables would outweigh the tile
-    return 2 * ((size_t)W * col_len<NX, W>() * 8 + (size_t)plan_tw_len<PlanV<NX, V>>() * 8 +
+    return 2 * ((size_t)W * col_len<NX, V, W>() * 8 + (size_t)plan_tw_len<PlanV<NX, V>>() * 8 +
                 (size_t)NX * 8) > 160 * 1024;
 }
 
@@ -342,7 +351,7 @@ template <int NX, int V, int W> __host__ __device__ constexpr int col_bound()
 template <int NX, int V, bool LAST, int W> __host__ __device__ constexpr int col_smem_len()
 {
     return (col_lean<NX, V, W>() ? 0 : plan_tw_len<PlanV<NX, V>>() + NX) + (LAST ? NX : 0) +
-           W * col_len<NX, W>();
+           W * col_len<NX, V, W>();
 }
 
 // tile sizes whose LDS and workgroup fit a CU
@@ -351,7 +360,7 @@ template <int NX, int V, int W> __host__ __device__ constexpr bool col_fits()
     using P = PlanV<NX, V>;
     const size_t tables = col_lean<NX, V, W>() ? (size_t)NX * 8  // s_nat of the last slice
                                                 : (size_t)NX * 16 + (size_t)plan_tw_len<P>() * 8;
-    return (size_t)W * col_len<NX, W>() * 8 + tables <= 160 * 1024 && W * P::T <= 1024;
+    return (size_t)W * col_len<NX, V, W>() * 8 + tables <= 160 * 1024 && W * P::T <= 1024;
 }
 
 // One column-pass workgroup: logical block `blk` of `nblk` (XCD-banded over
@@ -361,7 +370,8 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
 {
     using P = PlanV<NX, V>;
     constexpr int E = P::E, T = P::T;
-    constexpr int LEN = col_len<NX, W>();
+    constexpr int LEN = col_len<NX, V, W>();
+    constexpr int PS = col_pad_shift<NX, V>();
     constexpr bool LEAN = col_lean<NX, V, W>();
     // a column transform lies in one wavefront (wave-local exchanges) or
     // spans whole wavefronts (workgroup barriers)
@@ -406,7 +416,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
         for (int p = 0; p < E; ++p) v[p] = elem32(tile, o0 + (uint32_t)p * step);
 #pragma unroll
         for (int p = 0; p < E; ++p)
-            s_buf[c * LEN + lds_pad_at<T>(r0, p)] = ok ? v[p] : float2{0.f, 0.f};
+            s_buf[c * LEN + lds_pad_at<T, PS>(r0, p)] = ok ? v[p] : float2{0.f, 0.f};
     }
     __syncthreads();
 
@@ -418,9 +428,9 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
     float2* lds = s_buf + c * LEN;
     float2 x[E];
 #pragma unroll
-    for (int p = 0; p < E; ++p) x[p] = lds[lds_pad_at<T>(t, p)];
+    for (int p = 0; p < E; ++p) x[p] = lds[lds_pad_at<T, PS>(t, p)];
 
-    P::template run<false, float2, WAVE>(x, lds, tw, t);
+    P::template run<false, float2, WAVE, PS>(x, lds, tw, t);
 
     // band limit (bw_limit + propagate's k^2 test, decided on the host) and
     // propagator P_x[i] * P_y[j]; P_y carries the 1/sqrtNxNy of the FFT
@@ -443,7 +453,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
     }
 
     if constexpr (!LAST) {
-        P::template run<true, float2, WAVE>(x, lds, tw, t);
+        P::template run<true, float2, WAVE, PS>(x, lds, tw, t);
     } else if (active) {
         // add the vacuum exit wave, shifted: psi0(k) e^{-2 pi i (i sx / nx + j sy / ny)}
         const int2 sh = a.shift[b];
@@ -462,7 +472,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
 
     lds_sync<WAVE>();
 #pragma unroll
-    for (int p = 0; p < E; ++p) lds[lds_pad_at<T>(t, p)] = x[p];
+    for (int p = 0; p < E; ++p) lds[lds_pad_at<T, PS>(t, p)] = x[p];
     __syncthreads();
 
     {   // coalesced store of the tile
@@ -473,7 +483,7 @@ __device__ __forceinline__ void col_body(const ColPassArgs& a, int blk, int nblk
         if (ok) {
 #pragma unroll
             for (int p = 0; p < E; ++p)
-                elem32(tile, o0 + (uint32_t)p * step) = s_buf[c2 * LEN + lds_pad_at<T>(r0, p)];
+                elem32(tile, o0 + (uint32_t)p * step) = s_buf[c2 * LEN + lds_pad_at<T, PS>(r0, p)];
         }
     }
 }
