@@ -312,10 +312,12 @@ template <int NX, int V = 0> struct ColTile {
 // The alternate 1024 column pass pads every 16 (a 4-column tile of 34.9 KB
 // instead of 37 KB): the pass 45.5 -> 44.0 us and the 1024^2 job +3-4 %
 // (every 32: +2 %), the smaller tiles sharing CUs with the other batch
-// chain's row pass.
+// chain's row pass; the primary 512 column pass likewise (the 512^2 job
+// +1.3 %; every 32: no better), and the alternate 256 one (its pass
+// 40.3-42.4 -> 39.0-39.5 us per 256 positions).
 template <int NX, int V> __host__ __device__ constexpr int col_pad_shift()
 {
-    return (NX == 1024 && V == 1) ? 4 : 3;
+    return ((NX == 1024 && V == 1) || (NX == 512 && V == 0) || (NX == 256 && V == 1)) ? 4 : 3;
 }
 template <int NX, int V, int W> __host__ __device__ constexpr int col_len()
 {
