@@ -305,9 +305,6 @@ template <int NX, int V = 0> struct ColTile {
     static constexpr int W = (NX == 1024 && V == 1) ? 4 : (16 * lds_len(NX) * 8 > 48 * 1024) ? 8 : 16;
 };
 
-// column stride of the LDS tile: a 16-lane group of the coalesced load/store
-// phases covers W' = min(W, 16) columns x 16/W' rows; a stride congruent to
-// 16/W' modulo 16 (8-byte words) puts those 16 lanes on distinct bank pairs
 // padding of a column in the tile: one slot every 2^col_pad_shift values.
 // The alternate 1024 column pass pads every 16 (a 4-column tile of 34.9 KB
 // instead of 37 KB): the pass 45.5 -> 44.0 us and the 1024^2 job +3-4 %
@@ -319,6 +316,9 @@ template <int NX, int V> __host__ __device__ constexpr int col_pad_shift()
 {
     return ((NX == 1024 && V == 1) || (NX == 512 && V == 0) || (NX == 256 && V == 1)) ? 4 : 3;
 }
+// column stride of the LDS tile: a 16-lane group of the coalesced load/store
+// phases covers W' = min(W, 16) columns x 16/W' rows; a stride congruent to
+// 16/W' modulo 16 (8-byte words) puts those 16 lanes on distinct bank pairs
 template <int NX, int V, int W> __host__ __device__ constexpr int col_len()
 {
     constexpr int base = lds_len<col_pad_shift<NX, V>()>(NX);
