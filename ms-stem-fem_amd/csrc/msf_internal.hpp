@@ -76,8 +76,9 @@ struct RunPlan {
     msf_optics optics{};
     bool valid = false;      // vacuum waves of (spec_id, optics) are current
     int ncp = 0, G = 0, nbins = 0;
-    // vacuum waves: u0 of every slice [S][nx][2ny] (s * real space, rows
-    // doubled) and the k-space vacuum exit wave, column-major [ny][nx], FP32
+    // vacuum waves: u0 of every slice [S][nx][ny] pairs (s * real space; for
+    // every shift r of a row, u0[r] and u0[(r + ny/2) mod ny]: pair_pos) and
+    // the k-space vacuum exit wave, column-major [ny][nx], FP32
     DevBuf U0, Psi0;
     float2* u0 = nullptr;
     float2* psi0 = nullptr;
