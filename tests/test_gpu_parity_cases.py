@@ -49,6 +49,29 @@ def test_exit_waves_1024(msf, ctx, O):
     _exit_parity(msf, ctx, O, c, np.array([5.0, 31.3]), np.array([20.0]))
 
 
+@pytest.mark.parametrize("n", [96, 384, 512, 1024])
+def test_shifts_at_the_row_pair_edges(msf, ctx, O, n):
+    """The row pass reads the transmission and u0 rows in pairs (elements j
+    and j + n/2 of a lane in one read, u0 at r = (j - sy) mod n and its
+    partner (r + n/2) mod n): probe shifts sy of 0, 1, n/2 - 1, n/2, n/2 + 1
+    and n - 2 (the largest the shift rule yields) put the wrap of both at
+    every place, on the row passes that interleave reads and products (96,
+    384) and those that read ahead (512 primary, 1024 alternate); exit
+    intensities within 1e-5 of the oracle."""
+    L = n * 0.05
+    cell = (L, L, 3.0)
+    q, Z = random_atoms(30, cell, seed=300 + n, species=(14, 29))
+    c = Case(nx=n, ny=n, cell=cell, voltage=300e3, q=q, Z=Z, min_slice=2.0, alpha_max=0.02)
+    d = L / n
+    want = [0, 1, n // 2 - 1, n // 2, n // 2 + 1, n - 2]
+    yp = d * np.array(want, float)
+    xp = np.array([d * (n // 2 + 3)])
+    sh = msf.probe_shifts(c.grid(msf), xp, yp)
+    assert list(sh[:, 1]) == want and list(sh[:, 0]) == [n // 2 + 3] * len(want)
+    print("grid", n, "shifts", want, flush=True)
+    _exit_parity(msf, ctx, O, c, xp, yp, batch=4)
+
+
 def test_non_square_lengths(msf, ctx, O):
     """nx != ny with different plans on the two axes (192 x 96)."""
     cell = (18.0, 9.0, 4.0)
